@@ -1,0 +1,156 @@
+"""Throughput benchmark of the Metropolis + parallel-tempering hot path.
+
+Workload (one GPU): 64^3 periodic simple-cubic lattice, 64 replicas on a
+geometric temperature ladder 0.2..2.0 J, J = 1, bulk D = 0.5 J, Bz = 0.2 J,
+fixed cone radius 0.5, seeded random-sphere start (synthetic data).
+One step = 10 exact checkerboard sweeps of all 64 replicas followed by one
+replica-exchange round; E and M are recorded every 5 sweeps and the per-layer
+topological charge every 10, all on device inside one replayed hipGraph.
+
+value  = site updates/s (every replica's sites x sweeps / wall time),
+also reported: tau_int-corrected independent samples/s of E and of Q
+(middle layer), the half-sweep kernel's HBM roofline fraction from hipEvent
+timing, and the FP64 CPU oracle's rate on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+L = 64
+N_REP = 64
+T_MIN, T_MAX = 0.2, 2.0
+SWAP_EVERY = 10
+MEASURE_EVERY = 5
+WINDING_EVERY = 10
+CONE = 0.5
+SEED = 20261203
+BYTES_PER_UPDATE = 24  # FP32 (x,y,z) read once + written once per site update
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(seconds: float = 12.0) -> dict:
+    """FP64 OpenMP oracle (exact checkerboard sweep, same Hamiltonian) on one
+    64^3 replica at T = 0.5 J, timed on this host's cores."""
+    from oracle import pyoracle as orc
+    cores = min(16, os.cpu_count() or 1)
+    orc.set_threads(cores)
+    lat = orc.lattice(L, L, L)
+    h = orc.ham(1.0, 0.5, "bulk", 0.2, 0.0)
+    s = orc.init_random(lat, SEED, 0)
+    t0 = time.perf_counter()
+    orc.sweeps(lat, h, s, 2.0, CONE, SEED, 0, 0, 1)
+    one = time.perf_counter() - t0
+    n = max(1, int(seconds / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    orc.sweeps(lat, h, s, 2.0, CONE, SEED, 0, 1, n)
+    dt = time.perf_counter() - t0
+    return {"value": L ** 3 * n / dt, "unit": "site updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 C/OpenMP oracle, 1 replica 64^3, T=0.5, r=0.5, {n} sweeps in {dt:.1f} s"}
+
+
+def traffic_from_profiles() -> float | None:
+    p = ROOT / "profiles" / "sweep_traffic.json"
+    if p.exists():
+        try:
+            return float(json.loads(p.read_text())["hbm_bytes_per_launch"])
+        except Exception:
+            return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the path shards by PT ladder with no data exchange, "
+              "so --gpus N > 1 is not measured here.", file=sys.stderr)
+        return 2
+
+    from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
+    from ising_amd.analysis import independent_samples
+
+    T = geometric_temperatures(T_MIN, T_MAX, N_REP)
+    ham = Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2, Kz=0.0)
+    eng = Engine([Ladder(Lattice(L, L, L), ham, list(1.0 / T), CONE)], seed=SEED)
+    eng.init_random()
+    n_sites = L ** 3 * N_REP
+
+    # Warmup (also instantiates the graph once).
+    if args.warmup > 0:
+        eng.pt_run(args.warmup * SWAP_EVERY, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True)
+
+    n_sweeps = args.steps * SWAP_EVERY
+    # Device-wide synchronisation around the timed region goes through the
+    # library (torch is not imported: it bundles its own HIP runtime).
+    eng.synchronize()
+    t0 = time.perf_counter()
+    out = eng.pt_run(n_sweeps, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+
+    updates_per_s = n_sites * n_sweeps / wall
+    e_series = out["energy"]                                  # [n_meas, R]
+    q_series = out["winding_raw"][:, :, L // 2]               # middle layer, [n_q, R]
+    n_e, tau_e = independent_samples(e_series)
+    n_q, tau_q = independent_samples(q_series)
+    att, acc = eng.swap_stats()
+
+    # Kernel timing: hipEvents around back-to-back half-sweep launches.
+    ms_launch = eng.time_sweeps(50)
+    bytes_launch = BYTES_PER_UPDATE * n_sites / 2
+    achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
+        "value": updates_per_s,
+        "unit": "site updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 spins, fp64 reductions",
+        "data": "synthetic (seeded random-sphere start)",
+        "config": {"workload": "PT 64^3 x 64 replicas, T 0.2-2.0 geometric, J=1 D=0.5 (bulk) Bz=0.2, "
+                               "fixed cone r=0.5, swap every 10 sweeps, E/M every 5, Q every 10",
+                   "sweeps_per_step": SWAP_EVERY, "sampler": "fixed"},
+        "samples_per_s": {"E": n_e / wall, "Q_middle_layer": n_q / wall,
+                          "tau_int_E_sweeps": {"min": float(tau_e.min() * MEASURE_EVERY),
+                                               "max": float(tau_e.max() * MEASURE_EVERY)},
+                          "tau_int_Q_sweeps": {"min": float(tau_q.min() * WINDING_EVERY),
+                                               "max": float(tau_q.max() * WINDING_EVERY)}},
+        "swap_acceptance": {"min": float((acc[:-1] / np.maximum(att[:-1], 1)).min()),
+                            "mean": float((acc[:-1] / np.maximum(att[:-1], 1)).mean())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_half_sweep", "ms_per_launch": ms_launch,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

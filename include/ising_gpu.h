@@ -1,0 +1,180 @@
+/*
+ * ising_gpu.h -- C ABI of libising_hip.so, the MI355X (gfx950) Metropolis +
+ * parallel-tempering engine for classical Heisenberg spins on simple-cubic
+ * lattices (exchange + DMI + Zeeman + uniaxial anisotropy).
+ *
+ * Plain C types only; no exceptions cross this boundary. Every function
+ * returns ISING_OK (0) on success or a negative ISING_ERR_* code; the text of
+ * the last error of a context is available from ising_last_error().
+ *
+ * Ownership: the library owns all device memory of a context. Host buffers
+ * passed in are caller-owned and are read/written synchronously before the
+ * call returns. One host thread per context; contexts are independent.
+ *
+ * Layout of host spin buffers: the reference's `atomSpins(numAtoms,3)`
+ * real(8) array in column-major order (all x components, then all y, then
+ * all z), atoms in the reference's site order
+ *     site = (ix*Ly + iy)*Lz + iz        (0-based, z fastest)
+ * (src/chainMesh.f90:673-685 for the cell index, :878 for one atom per cell).
+ *
+ * Hamiltonian (one bond per nearest-neighbour pair, r_a = unit vector +a):
+ *     E = -J sum_<ij> S_i.S_j + sum_i sum_a d_a.(S_i x S_{i+a})
+ *         - zeeman_total_scale*Bz sum_i S_i^z - Kz sum_i (S_i^z)^2
+ * with d_a = D r_a for bulk DMI and d_a = D (z x (-r_a)) for the reference's
+ * interfacial form (src/energyMin.f90:114,135-137). Single-site energy
+ * changes use Bz with coefficient 1 (src/energyMin.f90:161-162).
+ */
+#ifndef ISING_GPU_H
+#define ISING_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define ISING_OK                 0
+#define ISING_ERR_INVALID_ARG   -1  /* null pointer, bad index, bad size      */
+#define ISING_ERR_BAD_DIMS      -2  /* lattice shape the checkerboard rejects */
+#define ISING_ERR_BAD_SPIN      -3  /* NaN / inf / non-unit spin uploaded     */
+#define ISING_ERR_NO_DEVICE     -4  /* no HIP device or device index invalid  */
+#define ISING_ERR_HIP           -5  /* a HIP runtime call failed              */
+#define ISING_ERR_UNSUPPORTED   -6  /* e.g. demagnetisation requested         */
+#define ISING_ERR_OUT_OF_MEMORY -7
+#define ISING_ERR_STATE         -8  /* call not valid in the current state    */
+
+/* ---- DMI forms -------------------------------------------------------- */
+#define ISING_DMI_NONE            0
+#define ISING_DMI_BULK            1 /* d_a = D r_a                           */
+#define ISING_DMI_INTERFACIAL_REF 2 /* d_a = D z x (-r_a), reference form    */
+
+typedef struct ising_ctx ising_ctx;
+
+typedef struct {
+    int32_t lx, ly, lz;      /* cells per axis, one atom per cell (SC)      */
+    int32_t periodic[3];     /* 1 = periodic, 0 = open, per axis x,y,z      */
+} ising_lattice;
+
+typedef struct {
+    double  J;                   /* E = -J S_i.S_j per bond                  */
+    double  D;                   /* DMI magnitude                            */
+    int32_t dmi_type;            /* ISING_DMI_*                              */
+    int32_t reserved;
+    double  Bz;                  /* Zeeman field along z                     */
+    double  Kz;                  /* uniaxial anisotropy, E -= Kz Sz^2        */
+    double  zeeman_total_scale;  /* Zeeman coefficient in total energies:
+                                    1 (default, consistent) or 2 (reference
+                                    TotalHeisenbergEnergy, energyMin.f90:222) */
+} ising_hamiltonian;
+
+/* One parallel-tempering ladder: n_replicas copies of one lattice at
+ * inverse temperatures betas[0..n_replicas-1]; swaps happen only between
+ * neighbouring entries of betas. */
+typedef struct {
+    ising_lattice     lattice;
+    ising_hamiltonian ham;
+    int32_t           n_replicas;
+    int32_t           reserved;
+    double            cone_radius; /* proposal ball radius r (energyMin.f90:49-68) */
+    const double*     betas;       /* n_replicas values, read at creation          */
+} ising_ladder_desc;
+
+typedef struct {
+    int32_t n_sweeps;        /* sweeps to run                                 */
+    int32_t swap_every;      /* replica-exchange round every k sweeps (0=off)  */
+    int32_t measure_every;   /* E and M recorded every k sweeps (0=off)        */
+    int32_t winding_every;   /* per-layer Q recorded every k sweeps (0=off)    */
+    int32_t use_graph;       /* 1: replay one captured hipGraph per period     */
+    int32_t reserved;
+} ising_schedule;
+
+/* Measurement series of the last ising_pt_run, indexed by temperature
+ * position t in the ladder concatenation (not by storage slot). Any pointer
+ * may be NULL. Sizes: n_meas = n_sweeps/measure_every,
+ * n_q = n_sweeps/winding_every, R = total replicas, Zmax = max lz. */
+typedef struct {
+    double*  energy;         /* [n_meas][R]                                   */
+    double*  magnetisation;  /* [n_meas][R][3] mean spin vector               */
+    int64_t* winding;        /* [n_q][R][Zmax] nint(Q) per layer (0 padded)    */
+    double*  winding_raw;    /* [n_q][R][Zmax] FP64 Q per layer                */
+} ising_series;
+
+/* ---- lifecycle -------------------------------------------------------- */
+const char* ising_version(void);
+int  ising_device_count(int* n);
+/* Replaces: the makeChainMesh + lock-array set-up the reference PT driver does
+ * per (param, T) mesh, src/PT.f90:180-211, batched into one context. */
+int  ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladders,
+                  uint64_t seed, int device);
+void ising_destroy(ising_ctx* ctx);                /* idempotent on NULL */
+/* ctx == NULL: message of the last failed ising_create on this thread. */
+const char* ising_last_error(const ising_ctx* ctx);
+int  ising_num_replicas(const ising_ctx* ctx, int* n);
+int  ising_replica_sites(const ising_ctx* ctx, int replica, int64_t* n);
+
+/* ---- parameters (per ladder / per replica storage slot) --------------- */
+int  ising_set_hamiltonian(ising_ctx* ctx, int ladder, const ising_hamiltonian* h);
+int  ising_set_cone(ising_ctx* ctx, int ladder, double radius);
+int  ising_set_beta(ising_ctx* ctx, int replica, double beta);
+int  ising_get_betas(ising_ctx* ctx, double* beta_of_replica);
+
+/* ---- spins ------------------------------------------------------------ */
+/* Copy-in / copy-out of chainMesh%atomSpins (src/chainMesh.f90:889). */
+int  ising_set_spins(ising_ctx* ctx, int replica, const double* soa, int64_t n);
+int  ising_get_spins(ising_ctx* ctx, int replica, double* soa, int64_t n);
+/* Seeded uniform-on-sphere start (replica < 0: all replicas). */
+int  ising_init_random(ising_ctx* ctx, int replica);
+/* Ferromagnet along +z with a seeded fraction of spins reversed. */
+int  ising_init_ferro(ising_ctx* ctx, int replica, double flip_fraction);
+
+/* ---- hot path --------------------------------------------------------- */
+/* Replaces Metropolis_mcs(chainMesh, beta, numMCSSweeps, ...)
+ * (src/energyMin.f90:234-374) for every replica of the context at once:
+ * n_sweeps exact checkerboard Metropolis sweeps at each replica's beta. */
+int  ising_sweep(ising_ctx* ctx, int n_sweeps);
+/* Replaces TotalHeisenbergEnergy (src/energyMin.f90:499-528): FP64 total
+ * energy of every replica slot, e[R]. */
+int  ising_energy(ising_ctx* ctx, double* e);
+/* Total energy of one replica under an arbitrary Hamiltonian (drop-in for
+ * TotalHeisenbergEnergy's per-call J, Dz, B arguments). */
+int  ising_energy_with(ising_ctx* ctx, int replica, const ising_hamiltonian* h, double* e);
+/* Mean spin vector per replica slot, m[R][3] (src/PT-Utils.f90:59-65). */
+int  ising_magnetisation(ising_ctx* ctx, double* m);
+/* Replaces calculate_winding_number2(chainMesh, Z_index)
+ * (src/reciprocalSpaceProcesses.f90:320-364): z is 0-based here. */
+int  ising_winding(ising_ctx* ctx, int replica, int z, int64_t* q, double* q_raw);
+/* All layers of all replicas: q[R][Zmax], q_raw[R][Zmax]. */
+int  ising_winding_all(ising_ctx* ctx, int64_t* q, double* q_raw);
+/* Test hook standing in for calculateHeisenbergEnergy
+ * (src/energyMin.f90:86-171): FP32 device dE = E(S') - E(S) of single-site
+ * proposals props[n][3] at sites[n] (reference site order) of one replica. */
+int  ising_delta_energy(ising_ctx* ctx, int replica, const int32_t* sites,
+                        const double* props, int n, double* de);
+
+/* ---- parallel tempering ---------------------------------------------- */
+/* Replaces the PT iteration loop src/PT.f90:219-273 (sweeps + swaps) plus
+ * the per-temperature statistics the reference takes at the end. */
+int  ising_pt_run(ising_ctx* ctx, const ising_schedule* sched, const ising_series* out);
+/* slot_of_temp[R]: storage slot currently at temperature position t
+ * (the reference's TemperatureMeshArray, src/PT.f90:58,201,264-268). */
+int  ising_temperature_map(ising_ctx* ctx, int32_t* slot_of_temp);
+/* Per neighbouring pair (t, t+1) of each ladder, at index t of the ladder
+ * concatenation: cumulative attempts and accepts; entry of a ladder's last
+ * temperature is 0. */
+int  ising_swap_stats(ising_ctx* ctx, int64_t* attempts, int64_t* accepts);
+/* Accepted single-site moves per replica slot since creation. */
+int  ising_accept_counts(ising_ctx* ctx, int64_t* accepted);
+
+/* ---- measurement (no reference counterpart) --------------------------- */
+/* Runs n_sweeps sweeps like ising_sweep with one hipEvent pair around the
+ * 2*n_sweeps back-to-back half-sweep launches on the context's stream;
+ * *ms_per_launch = elapsed / (2*n_sweeps). */
+int  ising_time_sweeps(ising_ctx* ctx, int n_sweeps, double* ms_per_launch);
+/* hipDeviceSynchronize on the context's device. */
+int  ising_synchronize(ising_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISING_GPU_H */

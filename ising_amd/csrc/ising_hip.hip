@@ -1,0 +1,1381 @@
+// ising_hip.hip -- CDNA4 (gfx950) kernels and C ABI of libising_hip.so.
+//
+// Hot path: exact two-colour checkerboard Metropolis on simple-cubic lattices
+// for classical unit spins (exchange + DMI + Zeeman + uniaxial anisotropy),
+// many replicas per launch, FP32 spins, Philox4x32-10 keyed by
+// (seed, replica slot) with counter (site, sweep). Energies, magnetisation and
+// topological charge are reduced in FP64; replica exchange runs on device.
+//
+// HBM layout of one replica with N = lx*ly*lz sites, hx = lx/2:
+//   spins[off + c*N + k*N/2 + (iz*ly + iy)*hx + (ix >> 1)]
+// component c in {x,y,z}, colour k = (ix+iy+iz) & 1. One half-sweep of colour
+// k streams the k arrays once (coalesced: consecutive lanes take consecutive
+// ix) and reads the other colour's rows iy+-1, iz+-1 and its own row as
+// neighbours. Replicas are packed back to back; a block table maps each
+// workgroup to (replica, chunk), so replicas of different sizes share one
+// launch.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdio.h>
+#include <stdarg.h>
+#include <math.h>
+#include <cmath>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <numeric>
+
+#include "ising_gpu.h"
+#include "philox.hpp"
+
+namespace ising {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Division by an invariant 32-bit divisor (Granlund & Montgomery, PLDI'94,
+// fig. 4.1): n / d = (t + ((n - t) >> s1)) >> s2 with t = mulhi(n, m).
+struct FastDiv { uint32_t m, s1, s2, d; };
+
+static FastDiv make_fastdiv(uint32_t d)
+{
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    uint64_t m = (((1ull << 32) * ((1ull << l) - d)) / d) + 1;
+    return FastDiv{ (uint32_t)m, l ? 1u : 0u, l ? l - 1u : 0u, d };
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f)
+{
+    uint32_t t = __umulhi(n, f.m);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+// Per replica storage slot.
+struct Rep {
+    int32_t lx, ly, lz, hx;
+    int32_t per;            // bit 0/1/2: axis x/y/z periodic
+    int32_t ladder;
+    int32_t first_block;    // first workgroup in the sweep/measure grid
+    int32_t nblocks;
+    int32_t q_first_block;  // first workgroup in the winding grid
+    int32_t q_blocks_layer; // winding workgroups per z layer
+    int32_t zbase;          // first entry of this slot in [R][Zmax] arrays
+    uint32_t half;          // N/2
+    int64_t off;            // float offset of the x component block
+    int64_t n;              // N
+    FastDiv div_hx, div_ly, div_lx;
+};
+
+// Per ladder Hamiltonian, FP32 copy for sweeps and FP64 copy for energies.
+struct Lad {
+    float J, Bz, Kz, r;
+    float d[9];
+    int32_t dmi;
+    double Jd, Bzd, Kzd, zs;
+    double dd[9];
+};
+
+struct f3 { float x, y, z; };
+
+__device__ __forceinline__ f3 ld3(const float* __restrict__ b, uint32_t i, int64_t n)
+{
+    return f3{ b[i], b[i + n], b[i + 2 * n] };
+}
+__device__ __forceinline__ f3 ld3c(const float* __restrict__ b, uint32_t i, int64_t n, bool ok)
+{
+    f3 v{ 0.f, 0.f, 0.f };
+    if (ok) v = ld3(b, i, n);
+    return v;
+}
+
+// Neighbour indices (inside the other colour's arrays) of the site at half
+// index h of row (iy, iz), parity p = ix & 1. ok[] false across open faces.
+struct Nbr { uint32_t i[6]; bool ok[6]; };
+
+__device__ __forceinline__ Nbr neighbours(const Rep& R, uint32_t row, uint32_t h, uint32_t iy, uint32_t iz, uint32_t p)
+{
+    Nbr nb;
+    const uint32_t hx = (uint32_t)R.hx, ly = (uint32_t)R.ly, lz = (uint32_t)R.lz;
+    const bool px = R.per & 1, py = (R.per >> 1) & 1, pz = (R.per >> 2) & 1;
+    const uint32_t rb = row * hx;
+    // +x: ix+1 has half index h+p unless it wraps (p=1, h=hx-1).
+    const bool xwrap_p = (p == 1u) && (h == hx - 1u);
+    nb.i[0] = xwrap_p ? rb : rb + h + p;
+    nb.ok[0] = !xwrap_p || px;
+    // -x: ix-1 has half index h+p-1 unless it wraps (p=0, h=0).
+    const bool xwrap_m = (p == 0u) && (h == 0u);
+    nb.i[1] = xwrap_m ? rb + hx - 1u : rb + h + p - 1u;
+    nb.ok[1] = !xwrap_m || px;
+    // +-y: same half index on row iy+-1.
+    const bool yp_w = (iy + 1u == ly), ym_w = (iy == 0u);
+    nb.i[2] = (yp_w ? row - (ly - 1u) : row + 1u) * hx + h;
+    nb.ok[2] = !yp_w || py;
+    nb.i[3] = (ym_w ? row + (ly - 1u) : row - 1u) * hx + h;
+    nb.ok[3] = !ym_w || py;
+    // +-z: same half index on row (iz+-1, iy).
+    const bool zp_w = (iz + 1u == lz), zm_w = (iz == 0u);
+    nb.i[4] = (zp_w ? row - (lz - 1u) * ly : row + ly) * hx + h;
+    nb.ok[4] = !zp_w || pz;
+    nb.i[5] = (zm_w ? row + (lz - 1u) * ly : row - ly) * hx + h;
+    nb.ok[5] = !zm_w || pz;
+    return nb;
+}
+
+// Local field h = J sum_j S_j - sum_a (S_{+a} - S_{-a}) x d_a + Bz z.
+__device__ __forceinline__ f3 local_field(const Lad& L, const float* __restrict__ oth, int64_t n, const Nbr& nb)
+{
+    const f3 xp = ld3c(oth, nb.i[0], n, nb.ok[0]);
+    const f3 xm = ld3c(oth, nb.i[1], n, nb.ok[1]);
+    const f3 yp = ld3c(oth, nb.i[2], n, nb.ok[2]);
+    const f3 ym = ld3c(oth, nb.i[3], n, nb.ok[3]);
+    const f3 zp = ld3c(oth, nb.i[4], n, nb.ok[4]);
+    const f3 zm = ld3c(oth, nb.i[5], n, nb.ok[5]);
+    const f3 sum{ (xp.x + xm.x) + (yp.x + ym.x) + (zp.x + zm.x),
+                  (xp.y + xm.y) + (yp.y + ym.y) + (zp.y + zm.y),
+                  (xp.z + xm.z) + (yp.z + ym.z) + (zp.z + zm.z) };
+    const f3 dx{ xp.x - xm.x, xp.y - xm.y, xp.z - xm.z };
+    const f3 dy{ yp.x - ym.x, yp.y - ym.y, yp.z - ym.z };
+    const f3 dz{ zp.x - zm.x, zp.y - zm.y, zp.z - zm.z };
+    f3 dm{ 0.f, 0.f, 0.f };
+    if (L.dmi == ISING_DMI_BULK) {
+        const float D = L.d[0];
+        dm = f3{ D * (dz.y - dy.z), D * (dx.z - dz.x), D * (dy.x - dx.y) };
+    } else if (L.dmi != ISING_DMI_NONE) {
+        const float* d = L.d;
+        dm.x = (dx.y * d[2] - dx.z * d[1]) + (dy.y * d[5] - dy.z * d[4]) + (dz.y * d[8] - dz.z * d[7]);
+        dm.y = (dx.z * d[0] - dx.x * d[2]) + (dy.z * d[3] - dy.x * d[5]) + (dz.z * d[6] - dz.x * d[8]);
+        dm.z = (dx.x * d[1] - dx.y * d[0]) + (dy.x * d[4] - dy.y * d[3]) + (dz.x * d[7] - dz.y * d[6]);
+    }
+    return f3{ L.J * sum.x - dm.x, L.J * sum.y - dm.y, L.J * sum.z - dm.z + L.Bz };
+}
+
+// dE = -(S' - S).h - K (S'_z - S_z)(S'_z + S_z): the difference is formed
+// first so small moves keep their relative accuracy.
+__device__ __forceinline__ float delta_e(const f3& s, const f3& sn, const f3& h, float K)
+{
+    const float ddx = sn.x - s.x, ddy = sn.y - s.y, ddz = sn.z - s.z;
+    return -(ddx * h.x + ddy * h.y + ddz * h.z) - K * ddz * (sn.z + s.z);
+}
+
+// ---------------------------------------------------------------------------
+// Half-sweep: one Metropolis proposal for every site of one colour of every
+// replica. Proposal: u uniform in a ball of radius r (cos(theta) = 2U-1,
+// phi = 2 pi U', radius r U''^(1/3)), S' = (S+u)/|S+u|; accept iff
+// U''' < exp(-beta dE). One Philox block supplies the four uniforms.
+__global__ __launch_bounds__(kBlock) void k_half_sweep(
+    float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const float* __restrict__ beta_slot, const int32_t* __restrict__ block_rep,
+    const uint32_t* __restrict__ sweep_ctr, uint32_t sweep_off, int colour,
+    uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count)
+{
+    const int slot = block_rep[blockIdx.x];
+    const Rep R = reps[slot];
+    const Lad L = lads[R.ladder];
+    const float beta = beta_slot[slot];
+    const uint32_t sweep = *sweep_ctr + sweep_off;
+    const uint32_t q = (uint32_t)(blockIdx.x - R.first_block) * kBlock + threadIdx.x;
+    const bool active = q < R.half;
+    bool accepted = false;
+    if (active) {
+        const int64_t n = R.n;
+        float* own = spins + R.off + (int64_t)colour * R.half;
+        const float* oth = spins + R.off + (int64_t)(colour ^ 1) * R.half;
+        const uint32_t row = fdiv(q, R.div_hx);
+        const uint32_t h = q - row * (uint32_t)R.hx;
+        const uint32_t iz = fdiv(row, R.div_ly);
+        const uint32_t iy = row - iz * (uint32_t)R.ly;
+        const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
+        const uint32_t ix = 2u * h + p;
+        const Nbr nb = neighbours(R, row, h, iy, iz, p);
+        const f3 s = ld3(own, q, n);
+        const f3 hf = local_field(L, oth, n, nb);
+
+        const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
+        const u32x4 rnd = philox4x32_10(u32x4{ site, sweep, kStreamMetropolis, seed_hi }, seed_lo, (uint32_t)slot);
+        const float ct = s24f(rnd.x);
+        const float uphi = u24f(rnd.y);
+        const float urad = u24f(rnd.z);
+        const float uacc = u24f(rnd.w);
+        const float st = sqrtf(fmaxf(0.f, (1.f - ct) * (1.f + ct)));
+        const float sp = __builtin_amdgcn_sinf(uphi);   // sin(2 pi uphi)
+        const float cp = __builtin_amdgcn_cosf(uphi);   // cos(2 pi uphi)
+        const float rr = L.r * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(urad) * (1.f / 3.f));
+        const f3 t{ s.x + rr * st * cp, s.y + rr * st * sp, s.z + rr * ct };
+        const float n2 = t.x * t.x + t.y * t.y + t.z * t.z;
+        if (n2 > 1e-30f) {
+            const float inv = __builtin_amdgcn_rsqf(n2);
+            const f3 sn{ t.x * inv, t.y * inv, t.z * inv };
+            const float de = delta_e(s, sn, hf, L.Kz);
+            const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
+            if (uacc < w) {
+                own[q] = sn.x;
+                own[q + n] = sn.y;
+                own[q + 2 * n] = sn.z;
+                accepted = true;
+            }
+        }
+    }
+    const unsigned long long ball = __ballot(accepted);
+    if ((threadIdx.x & (kWave - 1)) == 0 && ball)
+        atomicAdd(&acc_count[slot], (unsigned long long)__popcll(ball));
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions.
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Sum of 4 values over a workgroup of kBlock threads; result valid in thread 0.
+__device__ __forceinline__ void block_sum4(double v[4], double* lds /* [4][kBlock/kWave] */)
+{
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lds[k * (kBlock / kWave) + w] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double a = 0.0;
+            for (int j = 0; j < kBlock / kWave; ++j) a += lds[k * (kBlock / kWave) + j];
+            v[k] = a;
+        }
+    }
+}
+
+// Per-site energy of the +x, +y, +z bonds plus on-site terms, and the spin,
+// for both colours at half index q; per-workgroup partial sums.
+__global__ __launch_bounds__(kBlock) void k_measure(
+    const float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const Lad* __restrict__ lad_override, const int32_t* __restrict__ block_rep, int block_base,
+    double* __restrict__ partial)
+{
+    __shared__ double lds[4 * (kBlock / kWave)];
+    const int b = blockIdx.x + block_base;
+    const int slot = block_rep[b];
+    const Rep R = reps[slot];
+    const Lad& L = lad_override ? *lad_override : lads[R.ladder];
+    const uint32_t q = (uint32_t)(b - R.first_block) * kBlock + threadIdx.x;
+    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+    if (q < R.half) {
+        const int64_t n = R.n;
+        const uint32_t row = fdiv(q, R.div_hx);
+        const uint32_t h = q - row * (uint32_t)R.hx;
+        const uint32_t iz = fdiv(row, R.div_ly);
+        const uint32_t iy = row - iz * (uint32_t)R.ly;
+        for (int colour = 0; colour < 2; ++colour) {
+            const float* own = spins + R.off + (int64_t)colour * R.half;
+            const float* oth = spins + R.off + (int64_t)(colour ^ 1) * R.half;
+            const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
+            const Nbr nb = neighbours(R, row, h, iy, iz, p);
+            const f3 sf = ld3(own, q, n);
+            const double sx = sf.x, sy = sf.y, sz = sf.z;
+            double e = 0.0;
+            const int plus[3] = { 0, 2, 4 };
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int k = plus[a];
+                if (!nb.ok[k]) continue;
+                const f3 tf = ld3(oth, nb.i[k], n);
+                const double tx = tf.x, ty = tf.y, tz = tf.z;
+                const double cx = sy * tz - sz * ty, cy = sz * tx - sx * tz, cz = sx * ty - sy * tx;
+                e += -L.Jd * (sx * tx + sy * ty + sz * tz)
+                     + (L.dd[3 * a] * cx + L.dd[3 * a + 1] * cy + L.dd[3 * a + 2] * cz);
+            }
+            e -= L.zs * L.Bzd * sz + L.Kzd * sz * sz;
+            v[0] += e; v[1] += sx; v[2] += sy; v[3] += sz;
+        }
+    }
+    block_sum4(v, lds);
+    if (threadIdx.x == 0) {
+        double* o = partial + 4 * (int64_t)b;
+        o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    }
+}
+
+// One workgroup per slot: fixed-order sum of its partials (deterministic).
+__global__ __launch_bounds__(kBlock) void k_measure_finalize(
+    const Rep* __restrict__ reps, const double* __restrict__ partial, int slot_base,
+    double* __restrict__ energy, double* __restrict__ mag)
+{
+    __shared__ double lds[4 * (kBlock / kWave)];
+    const int slot = blockIdx.x + slot_base;
+    const Rep R = reps[slot];
+    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+    for (int j = threadIdx.x; j < R.nblocks; j += kBlock) {
+        const double* p = partial + 4 * (int64_t)(R.first_block + j);
+        v[0] += p[0]; v[1] += p[1]; v[2] += p[2]; v[3] += p[3];
+    }
+    block_sum4(v, lds);
+    if (threadIdx.x == 0) {
+        energy[slot] = v[0];
+        const double inv = 1.0 / (double)R.n;
+        mag[3 * slot] = v[1] * inv; mag[3 * slot + 1] = v[2] * inv; mag[3 * slot + 2] = v[3] * inv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Topological charge. Every operation of a triangle is FP64 in a fixed order
+// with contraction off, so it repeats bit for bit on any IEEE FP64 unit.
+__device__ __forceinline__ void renorm3(const f3& v, double o[3])
+{
+#pragma clang fp contract(off)
+    const double x = v.x, y = v.y, z = v.z;
+    const double nn = sqrt(x * x + y * y + z * z);
+    o[0] = x / nn; o[1] = y / nn; o[2] = z / nn;
+}
+
+__device__ __noinline__ double triangle_sigma(const f3 af, const f3 bf, const f3 cf)
+{
+#pragma clang fp contract(off)
+    double a[3], b[3], c[3];
+    renorm3(af, a); renorm3(bf, b); renorm3(cf, c);
+    const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    const double bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+    const double ca = c[0] * a[0] + c[1] * a[1] + c[2] * a[2];
+    const double x0 = b[1] * c[2] - b[2] * c[1];
+    const double x1 = b[2] * c[0] - b[0] * c[2];
+    const double x2 = b[0] * c[1] - b[1] * c[0];
+    const double re = ((1.0 + ab) + bc) + ca;
+    const double im = a[0] * x0 + a[1] * x1 + a[2] * x2;
+    return 2.0 * atan2(im, re);
+}
+
+__device__ __forceinline__ f3 spin_at(const float* __restrict__ spins, const Rep& R, uint32_t ix, uint32_t iy, uint32_t iz)
+{
+    const uint32_t k = (ix + iy + iz) & 1u;
+    const uint32_t idx = (iz * (uint32_t)R.ly + iy) * (uint32_t)R.hx + (ix >> 1);
+    return ld3(spins + R.off + (int64_t)k * R.half, idx, R.n);
+}
+
+__global__ __launch_bounds__(kBlock) void k_winding(
+    const float* __restrict__ spins, const Rep* __restrict__ reps, const int32_t* __restrict__ qblock_rep,
+    double* __restrict__ qpartial)
+{
+    __shared__ double lds[4 * (kBlock / kWave)];
+    const int slot = qblock_rep[blockIdx.x];
+    const Rep R = reps[slot];
+    const uint32_t bi = (uint32_t)(blockIdx.x - R.q_first_block);
+    const uint32_t z = bi / (uint32_t)R.q_blocks_layer;
+    const uint32_t pl = (bi - z * (uint32_t)R.q_blocks_layer) * kBlock + threadIdx.x;
+    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+    const uint32_t lx = (uint32_t)R.lx, ly = (uint32_t)R.ly;
+    if (pl < lx * ly) {
+        const uint32_t j = fdiv(pl, R.div_lx);
+        const uint32_t i = pl - j * lx;
+        const bool px = R.per & 1, py = (R.per >> 1) & 1;
+        const bool okx = (i + 1u < lx) || px, oky = (j + 1u < ly) || py;
+        if (okx && oky) {
+            const uint32_t i2 = (i + 1u == lx) ? 0u : i + 1u;
+            const uint32_t j2 = (j + 1u == ly) ? 0u : j + 1u;
+            const f3 s1 = spin_at(spins, R, i, j, z);
+            const f3 s2 = spin_at(spins, R, i2, j, z);
+            const f3 s3 = spin_at(spins, R, i, j2, z);
+            const f3 s4 = spin_at(spins, R, i2, j2, z);
+            v[0] = triangle_sigma(s1, s2, s4) + triangle_sigma(s1, s4, s3);
+        }
+    }
+    block_sum4(v, lds);
+    if (threadIdx.x == 0) qpartial[blockIdx.x] = v[0];
+}
+
+// One workgroup per (slot, layer): fixed-order sum, Q = sum / (4 pi).
+__global__ __launch_bounds__(kBlock) void k_winding_finalize(
+    const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax,
+    double* __restrict__ qraw, long long* __restrict__ qint)
+{
+    __shared__ double lds[4 * (kBlock / kWave)];
+    const int slot = blockIdx.x / zmax;
+    const int z = blockIdx.x - slot * zmax;
+    const Rep R = reps[slot];
+    if (z >= R.lz) {
+        if (threadIdx.x == 0) { qraw[blockIdx.x] = 0.0; qint[blockIdx.x] = 0; }
+        return;
+    }
+    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+    const int base = R.q_first_block + z * R.q_blocks_layer;
+    for (int j = threadIdx.x; j < R.q_blocks_layer; j += kBlock) v[0] += qpartial[base + j];
+    block_sum4(v, lds);
+    if (threadIdx.x == 0) {
+        const double q = v[0] / (4.0 * 3.14159265358979323846);
+        qraw[blockIdx.x] = q;
+        qint[blockIdx.x] = llrint(q);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Replica exchange: one round over every ladder. Pairs (t, t+1) with
+// t = round mod 2 inside each ladder; accept iff log u < (b_t - b_{t+1})(E_t - E_{t+1})
+// in FP64. Labels move (which slot sits at which temperature), spins stay.
+__global__ __launch_bounds__(kBlock) void k_swap(
+    const double* __restrict__ energy, const double* __restrict__ beta_temp,
+    int32_t* __restrict__ slot_of_temp, float* __restrict__ beta_slot,
+    const int32_t* __restrict__ lad_of_pos, const int32_t* __restrict__ lad_start,
+    const int32_t* __restrict__ lad_n, int R, uint32_t* __restrict__ round_ctr,
+    uint32_t seed_lo, uint32_t seed_hi,
+    unsigned long long* __restrict__ attempts, unsigned long long* __restrict__ accepts)
+{
+    const uint32_t round = *round_ctr;
+    __syncthreads();
+    for (int t = threadIdx.x; t < R; t += blockDim.x) {
+        const int l = lad_of_pos[t];
+        const int tl = t - lad_start[l];
+        if (((uint32_t)tl & 1u) != (round & 1u) || tl + 1 >= lad_n[l]) continue;
+        const int a = slot_of_temp[t], b = slot_of_temp[t + 1];
+        const double delta = (beta_temp[t] - beta_temp[t + 1]) * (energy[a] - energy[b]);
+        const u32x4 r = philox4x32_10(u32x4{ (uint32_t)tl, round, kStreamSwap, seed_hi }, seed_lo, (uint32_t)l);
+        const uint64_t m = ((uint64_t)(r.x >> 5) << 26) | (uint64_t)(r.y >> 6);
+        const double u = ((double)m + 1.0) * 0x1p-53;
+        attempts[t] += 1ull;
+        if (log(u) < delta) {
+            slot_of_temp[t] = b;
+            slot_of_temp[t + 1] = a;
+            beta_slot[b] = (float)beta_temp[t];
+            beta_slot[a] = (float)beta_temp[t + 1];
+            accepts[t] += 1ull;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *round_ctr = round + 1u;
+}
+
+// Series records (single workgroup): value at temperature position t.
+__global__ __launch_bounds__(kBlock) void k_record(
+    const double* __restrict__ energy, const double* __restrict__ mag, const int32_t* __restrict__ slot_of_temp,
+    int R, uint32_t* __restrict__ meas_ctr, uint32_t cap, double* __restrict__ se, double* __restrict__ sm)
+{
+    const uint32_t m = *meas_ctr;
+    __syncthreads();
+    if (m < cap) {
+        for (int t = threadIdx.x; t < R; t += blockDim.x) {
+            const int s = slot_of_temp[t];
+            se[(int64_t)m * R + t] = energy[s];
+            for (int c = 0; c < 3; ++c) sm[((int64_t)m * R + t) * 3 + c] = mag[3 * s + c];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *meas_ctr = m + 1u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_record_q(
+    const double* __restrict__ qraw, const long long* __restrict__ qint, const int32_t* __restrict__ slot_of_temp,
+    int R, int zmax, uint32_t* __restrict__ q_ctr, uint32_t cap, double* __restrict__ sqr, long long* __restrict__ sqi)
+{
+    const uint32_t m = *q_ctr;
+    __syncthreads();
+    if (m < cap) {
+        for (int e = threadIdx.x; e < R * zmax; e += blockDim.x) {
+            const int t = e / zmax, z = e - t * zmax;
+            const int s = slot_of_temp[t];
+            sqr[(int64_t)m * R * zmax + e] = qraw[s * zmax + z];
+            sqi[(int64_t)m * R * zmax + e] = qint[s * zmax + z];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *q_ctr = m + 1u;
+}
+
+__global__ void k_add_u32(uint32_t* __restrict__ p, uint32_t v) { if (threadIdx.x == 0) *p += v; }
+__global__ void k_set_u32(uint32_t* __restrict__ p, int n, uint32_t v)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Initial states (both colours at half index q).
+__global__ __launch_bounds__(kBlock) void k_init(
+    float* __restrict__ spins, const Rep* __restrict__ reps, const int32_t* __restrict__ block_rep,
+    int block_base, int mode, double frac, uint32_t seed_lo, uint32_t seed_hi)
+{
+    const int b = blockIdx.x + block_base;
+    const int slot = block_rep[b];
+    const Rep R = reps[slot];
+    const uint32_t q = (uint32_t)(b - R.first_block) * kBlock + threadIdx.x;
+    if (q >= R.half) return;
+    const uint32_t row = fdiv(q, R.div_hx);
+    const uint32_t h = q - row * (uint32_t)R.hx;
+    const uint32_t iz = fdiv(row, R.div_ly);
+    const uint32_t iy = row - iz * (uint32_t)R.ly;
+    for (int colour = 0; colour < 2; ++colour) {
+        float* own = spins + R.off + (int64_t)colour * R.half;
+        const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
+        const uint32_t ix = 2u * h + p;
+        const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
+        const u32x4 r = philox4x32_10(u32x4{ site, 0xFFFFFFFFu, kStreamInit, seed_hi }, seed_lo, (uint32_t)slot);
+        float x, y, z;
+        if (mode == 0) {
+            z = s24f(r.x);
+            const float u = u24f(r.y);
+            const float st = sqrtf(fmaxf(0.f, (1.f - z) * (1.f + z)));
+            x = st * __builtin_amdgcn_cosf(u);
+            y = st * __builtin_amdgcn_sinf(u);
+            const float inv = 1.0f / sqrtf(x * x + y * y + z * z);
+            x *= inv; y *= inv; z *= inv;
+        } else {
+            x = 0.f; y = 0.f;
+            z = ((double)(r.z >> 8) * 0x1p-24 < frac) ? -1.f : 1.f;
+        }
+        own[q] = x; own[q + R.n] = y; own[q + 2 * R.n] = z;
+    }
+}
+
+// Single-site dE test hook: same local field and dE code as the sweep.
+__global__ __launch_bounds__(kBlock) void k_delta_energy(
+    const float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads, int slot,
+    const int32_t* __restrict__ sites, const float* __restrict__ props, int count, double* __restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const Rep R = reps[slot];
+    const Lad L = lads[R.ladder];
+    const uint32_t site = (uint32_t)sites[e];
+    const uint32_t iz = site % (uint32_t)R.lz;
+    const uint32_t iy = (site / (uint32_t)R.lz) % (uint32_t)R.ly;
+    const uint32_t ix = site / ((uint32_t)R.lz * (uint32_t)R.ly);
+    const uint32_t colour = (ix + iy + iz) & 1u;
+    const uint32_t row = iz * (uint32_t)R.ly + iy;
+    const uint32_t h = ix >> 1;
+    const uint32_t q = row * (uint32_t)R.hx + h;
+    const float* own = spins + R.off + (int64_t)colour * R.half;
+    const float* oth = spins + R.off + (int64_t)(colour ^ 1u) * R.half;
+    const Nbr nb = neighbours(R, row, h, iy, iz, ix & 1u);
+    const f3 s = ld3(own, q, R.n);
+    const f3 hf = local_field(L, oth, R.n, nb);
+    const f3 sn{ props[3 * e], props[3 * e + 1], props[3 * e + 2] };
+    out[e] = (double)delta_e(s, sn, hf, L.Kz);
+}
+
+} // namespace ising
+
+// ===========================================================================
+// Host side.
+using namespace ising;
+
+struct LadHost {
+    ising_lattice lat;
+    ising_hamiltonian ham;
+    int n;
+    int start;       // first temperature position / slot
+    double cone;
+    std::vector<double> betas;
+};
+
+struct ising_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint64_t seed = 0;
+    int R = 0, zmax = 0, n_blocks = 0, n_qblocks = 0;
+    std::vector<LadHost> ladders;
+    std::vector<Rep> reps;
+    std::vector<int32_t> block_rep, qblock_rep;
+    int64_t total_floats = 0;
+    uint32_t host_sweep = 0;
+
+    float* d_spins = nullptr;
+    Rep* d_reps = nullptr;
+    Lad* d_lads = nullptr;
+    Lad* d_lad_override = nullptr;
+    int32_t* d_block_rep = nullptr;
+    int32_t* d_qblock_rep = nullptr;
+    float* d_beta_slot = nullptr;
+    double* d_beta_temp = nullptr;
+    int32_t* d_slot_of_temp = nullptr;
+    int32_t* d_lad_of_pos = nullptr;
+    int32_t* d_lad_start = nullptr;
+    int32_t* d_lad_n = nullptr;
+    double* d_partial = nullptr;
+    double* d_energy = nullptr;
+    double* d_mag = nullptr;
+    double* d_qpartial = nullptr;
+    double* d_qraw = nullptr;
+    long long* d_qint = nullptr;
+    uint32_t* d_ctr = nullptr;  // [0] sweep, [1] meas, [2] winding meas, [3] swap round
+    unsigned long long* d_acc = nullptr;
+    unsigned long long* d_swap_att = nullptr;
+    unsigned long long* d_swap_acc = nullptr;
+    int32_t* d_sites = nullptr; float* d_props = nullptr; double* d_de = nullptr; int de_cap = 0;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int set_err(ising_ctx* c, int code, const char* fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                               \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return set_err((ctx), ISING_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK(ctx)                                                                \
+    do {                                                                                 \
+        hipError_t e_ = hipGetLastError();                                               \
+        if (e_ != hipSuccess)                                                            \
+            return set_err((ctx), ISING_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(ising_ctx* c, T** p, size_t count)
+{
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(c, e == hipErrorOutOfMemory ? ISING_ERR_OUT_OF_MEMORY : ISING_ERR_HIP,
+                       "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return ISING_OK;
+}
+
+Lad make_lad(const ising_hamiltonian& h, double cone)
+{
+    Lad L;
+    memset(&L, 0, sizeof L);
+    double d[9] = { 0 };
+    if (h.dmi_type == ISING_DMI_BULK) { d[0] = h.D; d[4] = h.D; d[8] = h.D; }
+    else if (h.dmi_type == ISING_DMI_INTERFACIAL_REF) { d[1] = -h.D; d[3] = h.D; }
+    L.J = (float)h.J; L.Bz = (float)h.Bz; L.Kz = (float)h.Kz; L.r = (float)cone;
+    for (int k = 0; k < 9; ++k) { L.d[k] = (float)d[k]; L.dd[k] = d[k]; }
+    L.dmi = h.dmi_type;
+    L.Jd = h.J; L.Bzd = h.Bz; L.Kzd = h.Kz; L.zs = h.zeeman_total_scale;
+    return L;
+}
+
+int check_lattice(ising_ctx* c, const ising_lattice& l)
+{
+    const int n[3] = { l.lx, l.ly, l.lz };
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 1) return set_err(c, ISING_ERR_BAD_DIMS, "axis %d has %d cells", a, n[a]);
+        if (l.periodic[a] && (n[a] < 4 || (n[a] & 1)))
+            return set_err(c, ISING_ERR_BAD_DIMS,
+                           "periodic axis %d needs an even length >= 4 for the two-colour sweep (got %d)", a, n[a]);
+    }
+    if (l.lx & 1) return set_err(c, ISING_ERR_BAD_DIMS, "lx must be even (got %d)", l.lx);
+    const int64_t N = (int64_t)l.lx * l.ly * l.lz;
+    if (N >= (1ll << 31)) return set_err(c, ISING_ERR_BAD_DIMS, "lattice of %lld sites too large", (long long)N);
+    return ISING_OK;
+}
+
+int check_ham(ising_ctx* c, const ising_hamiltonian& h)
+{
+    if (h.dmi_type < ISING_DMI_NONE || h.dmi_type > ISING_DMI_INTERFACIAL_REF)
+        return set_err(c, ISING_ERR_INVALID_ARG, "unknown dmi_type %d", h.dmi_type);
+    const double v[5] = { h.J, h.D, h.Bz, h.Kz, h.zeeman_total_scale };
+    for (double x : v)
+        if (!std::isfinite(x)) return set_err(c, ISING_ERR_INVALID_ARG, "non-finite Hamiltonian parameter");
+    return ISING_OK;
+}
+
+int upload_lads(ising_ctx* c)
+{
+    std::vector<Lad> l;
+    for (auto& L : c->ladders) l.push_back(make_lad(L.ham, L.cone));
+    HIP_TRY(c, hipMemcpyAsync(c->d_lads, l.data(), l.size() * sizeof(Lad), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int check_replica(ising_ctx* c, int r)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (r < 0 || r >= c->R) return set_err(c, ISING_ERR_INVALID_ARG, "replica %d out of range [0,%d)", r, c->R);
+    return ISING_OK;
+}
+
+// ---- launch helpers --------------------------------------------------------
+
+int launch_half_sweeps(ising_ctx* c, uint32_t off, int colour)
+{
+    hipLaunchKernelGGL(k_half_sweep, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
+                       c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
+                       c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc);
+    LAUNCH_CHECK(c);
+    return ISING_OK;
+}
+
+int launch_measure_all(ising_ctx* c)
+{
+    hipLaunchKernelGGL(k_measure, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
+                       (const float*)c->d_spins, c->d_reps, c->d_lads, (const Lad*)nullptr, c->d_block_rep, 0, c->d_partial);
+    LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_measure_finalize, dim3(c->R), dim3(kBlock), 0, c->stream,
+                       c->d_reps, c->d_partial, 0, c->d_energy, c->d_mag);
+    LAUNCH_CHECK(c);
+    return ISING_OK;
+}
+
+int launch_winding_all(ising_ctx* c)
+{
+    hipLaunchKernelGGL(k_winding, dim3(c->n_qblocks), dim3(kBlock), 0, c->stream,
+                       (const float*)c->d_spins, c->d_reps, c->d_qblock_rep, c->d_qpartial);
+    LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_winding_finalize, dim3(c->R * c->zmax), dim3(kBlock), 0, c->stream,
+                       c->d_reps, c->d_qpartial, c->zmax, c->d_qraw, c->d_qint);
+    LAUNCH_CHECK(c);
+    return ISING_OK;
+}
+
+int launch_swap(ising_ctx* c)
+{
+    hipLaunchKernelGGL(k_swap, dim3(1), dim3(kBlock), 0, c->stream,
+                       (const double*)c->d_energy, (const double*)c->d_beta_temp, c->d_slot_of_temp, c->d_beta_slot,
+                       c->d_lad_of_pos, c->d_lad_start, c->d_lad_n, c->R, c->d_ctr + 3,
+                       (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_swap_att, c->d_swap_acc);
+    LAUNCH_CHECK(c);
+    return ISING_OK;
+}
+
+int launch_add(ising_ctx* c, int which, uint32_t v)
+{
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, c->stream, c->d_ctr + which, v);
+    LAUNCH_CHECK(c);
+    return ISING_OK;
+}
+
+struct SeriesDev {
+    double* e = nullptr; double* m = nullptr; double* qr = nullptr; long long* qi = nullptr;
+    uint32_t n_meas = 0, n_q = 0;
+};
+
+// Sweeps [0, count) of a schedule period, with swaps / records after every
+// sweep k+1 that the schedule names. Offsets are relative to the device sweep
+// counter, which is advanced by `count` at the end, so the same launch
+// sequence can be captured once and replayed.
+int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const SeriesDev& sd)
+{
+    int rc;
+    for (int k = 0; k < count; ++k) {
+        if ((rc = launch_half_sweeps(c, (uint32_t)k, 0))) return rc;
+        if ((rc = launch_half_sweeps(c, (uint32_t)k, 1))) return rc;
+        const int done = k + 1;
+        const bool do_swap = s.swap_every > 0 && done % s.swap_every == 0;
+        const bool do_meas = s.measure_every > 0 && done % s.measure_every == 0;
+        const bool do_q = s.winding_every > 0 && done % s.winding_every == 0;
+        if (do_swap || do_meas)
+            if ((rc = launch_measure_all(c))) return rc;
+        if (do_swap)
+            if ((rc = launch_swap(c))) return rc;
+        if (do_meas && sd.e) {
+            hipLaunchKernelGGL(k_record, dim3(1), dim3(kBlock), 0, c->stream,
+                               (const double*)c->d_energy, (const double*)c->d_mag, (const int32_t*)c->d_slot_of_temp,
+                               c->R, c->d_ctr + 1, sd.n_meas, sd.e, sd.m);
+            LAUNCH_CHECK(c);
+        }
+        if (do_q && sd.qr) {
+            if ((rc = launch_winding_all(c))) return rc;
+            hipLaunchKernelGGL(k_record_q, dim3(1), dim3(kBlock), 0, c->stream,
+                               (const double*)c->d_qraw, (const long long*)c->d_qint, (const int32_t*)c->d_slot_of_temp,
+                               c->R, c->zmax, c->d_ctr + 2, sd.n_q, sd.qr, sd.qi);
+            LAUNCH_CHECK(c);
+        }
+    }
+    return launch_add(c, 0, (uint32_t)count);
+}
+
+int64_t gcd64(int64_t a, int64_t b) { while (b) { int64_t t = a % b; a = b; b = t; } return a; }
+
+} // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* ising_version(void) { return "ising_hip 0.1 (gfx950)"; }
+
+int ising_device_count(int* n)
+{
+    if (!n) return ISING_ERR_INVALID_ARG;
+    int k = 0;
+    if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+    *n = k;
+    return ISING_OK;
+}
+
+const char* ising_last_error(const ising_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int ising_num_replicas(const ising_ctx* ctx, int* n)
+{
+    if (!ctx || !n) return ISING_ERR_INVALID_ARG;
+    *n = ctx->R;
+    return ISING_OK;
+}
+
+int ising_replica_sites(const ising_ctx* ctx, int replica, int64_t* n)
+{
+    if (!ctx || !n || replica < 0 || replica >= ctx->R) return ISING_ERR_INVALID_ARG;
+    *n = ctx->reps[replica].n;
+    return ISING_OK;
+}
+
+void ising_destroy(ising_ctx* c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void* ptrs[] = { c->d_spins, c->d_reps, c->d_lads, c->d_lad_override, c->d_block_rep, c->d_qblock_rep,
+                     c->d_beta_slot, c->d_beta_temp, c->d_slot_of_temp, c->d_lad_of_pos, c->d_lad_start,
+                     c->d_lad_n, c->d_partial, c->d_energy, c->d_mag, c->d_qpartial, c->d_qraw, c->d_qint,
+                     c->d_ctr, c->d_acc, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de };
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladders, uint64_t seed, int device)
+{
+    if (!out) return ISING_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!ladders || n_ladders < 1) return ISING_ERR_INVALID_ARG;
+    ising_ctx* c = new ising_ctx();
+    c->seed = seed;
+    c->device = device;
+    int rc = ISING_OK;
+    // On failure the partial context is released and its message is kept for
+    // ising_last_error(NULL) on this thread.
+    auto fail = [&](int code) {
+        g_create_err = c->err;
+        ising_destroy(c);
+        *out = nullptr;
+        return code;
+    };
+
+    // Validate and lay out replicas.
+    int R = 0;
+    for (int l = 0; l < n_ladders; ++l) {
+        const ising_ladder_desc& d = ladders[l];
+        if ((rc = check_lattice(c, d.lattice))) return fail(rc);
+        if ((rc = check_ham(c, d.ham))) return fail(rc);
+        if (d.n_replicas < 1 || !d.betas) return fail(set_err(c, ISING_ERR_INVALID_ARG, "ladder %d: n_replicas/betas", l));
+        if (!(d.cone_radius >= 0.0) || !std::isfinite(d.cone_radius))
+            return fail(set_err(c, ISING_ERR_INVALID_ARG, "ladder %d: cone radius", l));
+        LadHost L;
+        L.lat = d.lattice; L.ham = d.ham; L.n = d.n_replicas; L.start = R; L.cone = d.cone_radius;
+        L.betas.assign(d.betas, d.betas + d.n_replicas);
+        for (double b : L.betas)
+            if (!std::isfinite(b) || b < 0) return fail(set_err(c, ISING_ERR_INVALID_ARG, "ladder %d: bad beta", l));
+        c->ladders.push_back(L);
+        R += d.n_replicas;
+    }
+    c->R = R;
+    int64_t off = 0;
+    int blk = 0, qblk = 0, zmax = 0;
+    for (int l = 0; l < n_ladders; ++l) {
+        const LadHost& L = c->ladders[l];
+        for (int k = 0; k < L.n; ++k) {
+            Rep r;
+            memset(&r, 0, sizeof r);
+            r.lx = L.lat.lx; r.ly = L.lat.ly; r.lz = L.lat.lz; r.hx = r.lx / 2;
+            r.per = (L.lat.periodic[0] ? 1 : 0) | (L.lat.periodic[1] ? 2 : 0) | (L.lat.periodic[2] ? 4 : 0);
+            r.ladder = l;
+            r.n = (int64_t)r.lx * r.ly * r.lz;
+            r.half = (uint32_t)(r.n / 2);
+            r.off = off;
+            off += 3 * r.n;
+            off = (off + 63) & ~(int64_t)63;      // 256-byte aligned replica blocks
+            r.first_block = blk;
+            r.nblocks = (int)((r.half + kBlock - 1) / kBlock);
+            blk += r.nblocks;
+            r.q_first_block = qblk;
+            r.q_blocks_layer = (r.lx * r.ly + kBlock - 1) / kBlock;
+            qblk += r.q_blocks_layer * r.lz;
+            r.div_hx = make_fastdiv((uint32_t)r.hx);
+            r.div_ly = make_fastdiv((uint32_t)r.ly);
+            r.div_lx = make_fastdiv((uint32_t)r.lx);
+            zmax = std::max(zmax, r.lz);
+            c->reps.push_back(r);
+            for (int b = 0; b < r.nblocks; ++b) c->block_rep.push_back((int32_t)c->reps.size() - 1);
+            for (int b = 0; b < r.q_blocks_layer * r.lz; ++b) c->qblock_rep.push_back((int32_t)c->reps.size() - 1);
+        }
+    }
+    for (auto& r : c->reps) r.zbase = 0;
+    c->zmax = zmax;
+    c->n_blocks = blk;
+    c->n_qblocks = qblk;
+    c->total_floats = off;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(set_err(c, ISING_ERR_NO_DEVICE, "no HIP device visible"));
+    if (device < 0 || device >= ndev)
+        return fail(set_err(c, ISING_ERR_NO_DEVICE, "device %d out of range (%d devices)", device, ndev));
+    if (hipSetDevice(device) != hipSuccess) return fail(set_err(c, ISING_ERR_NO_DEVICE, "hipSetDevice(%d)", device));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(c, ISING_ERR_HIP, "hipStreamCreate"));
+
+    if ((rc = dalloc(c, &c->d_spins, (size_t)c->total_floats))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_reps, c->reps.size()))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_lads, c->ladders.size()))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_lad_override, 1))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_block_rep, c->block_rep.size()))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_qblock_rep, c->qblock_rep.size()))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_beta_slot, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_beta_temp, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_slot_of_temp, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_lad_of_pos, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_lad_start, n_ladders))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_lad_n, n_ladders))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_partial, 4 * (size_t)blk))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_energy, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_mag, 3 * (size_t)R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_qpartial, (size_t)qblk))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_qraw, (size_t)R * zmax))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_qint, (size_t)R * zmax))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_ctr, 4))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_acc, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_swap_att, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_swap_acc, R))) return fail(rc);
+
+    std::vector<float> beta_slot(R);
+    std::vector<double> beta_temp(R);
+    std::vector<int32_t> slot_of_temp(R), lad_of_pos(R), lad_start(n_ladders), lad_n(n_ladders);
+    for (int l = 0; l < n_ladders; ++l) {
+        const LadHost& L = c->ladders[l];
+        lad_start[l] = L.start; lad_n[l] = L.n;
+        for (int k = 0; k < L.n; ++k) {
+            beta_slot[L.start + k] = (float)L.betas[k];
+            beta_temp[L.start + k] = L.betas[k];
+            slot_of_temp[L.start + k] = L.start + k;
+            lad_of_pos[L.start + k] = l;
+        }
+    }
+    hipStream_t s = c->stream;
+#define UP(dst, src, cnt) \
+    if (hipMemcpyAsync(dst, src, (cnt) * sizeof(*(src)), hipMemcpyHostToDevice, s) != hipSuccess) \
+        return fail(set_err(c, ISING_ERR_HIP, "upload " #dst));
+    UP(c->d_reps, c->reps.data(), c->reps.size());
+    UP(c->d_block_rep, c->block_rep.data(), c->block_rep.size());
+    UP(c->d_qblock_rep, c->qblock_rep.data(), c->qblock_rep.size());
+    UP(c->d_beta_slot, beta_slot.data(), R);
+    UP(c->d_beta_temp, beta_temp.data(), R);
+    UP(c->d_slot_of_temp, slot_of_temp.data(), R);
+    UP(c->d_lad_of_pos, lad_of_pos.data(), R);
+    UP(c->d_lad_start, lad_start.data(), n_ladders);
+    UP(c->d_lad_n, lad_n.data(), n_ladders);
+#undef UP
+    if (hipMemsetAsync(c->d_spins, 0, (size_t)c->total_floats * sizeof(float), s) != hipSuccess ||
+        hipMemsetAsync(c->d_ctr, 0, 4 * sizeof(uint32_t), s) != hipSuccess ||
+        hipMemsetAsync(c->d_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
+        hipMemsetAsync(c->d_swap_att, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
+        hipMemsetAsync(c->d_swap_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess)
+        return fail(set_err(c, ISING_ERR_HIP, "hipMemsetAsync"));
+    if ((rc = upload_lads(c))) return fail(rc);
+    *out = c;
+    return ISING_OK;
+}
+
+int ising_set_hamiltonian(ising_ctx* c, int ladder, const ising_hamiltonian* h)
+{
+    if (!c || !h) return ISING_ERR_INVALID_ARG;
+    if (ladder < 0 || ladder >= (int)c->ladders.size()) return set_err(c, ISING_ERR_INVALID_ARG, "ladder %d", ladder);
+    int rc = check_ham(c, *h);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->ladders[ladder].ham = *h;
+    return upload_lads(c);
+}
+
+int ising_set_cone(ising_ctx* c, int ladder, double radius)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (ladder < 0 || ladder >= (int)c->ladders.size()) return set_err(c, ISING_ERR_INVALID_ARG, "ladder %d", ladder);
+    if (!(radius >= 0.0) || !std::isfinite(radius)) return set_err(c, ISING_ERR_INVALID_ARG, "cone radius");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->ladders[ladder].cone = radius;
+    return upload_lads(c);
+}
+
+int ising_set_beta(ising_ctx* c, int replica, double beta)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    if (!std::isfinite(beta) || beta < 0) return set_err(c, ISING_ERR_INVALID_ARG, "beta");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // The slot keeps its temperature position: set both the slot's beta and
+    // the beta of the temperature position it currently occupies.
+    std::vector<int32_t> sot(c->R);
+    HIP_TRY(c, hipMemcpyAsync(sot.data(), c->d_slot_of_temp, c->R * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int t = -1;
+    for (int k = 0; k < c->R; ++k) if (sot[k] == replica) t = k;
+    float bf = (float)beta;
+    HIP_TRY(c, hipMemcpyAsync(c->d_beta_slot + replica, &bf, sizeof bf, hipMemcpyHostToDevice, c->stream));
+    if (t >= 0) HIP_TRY(c, hipMemcpyAsync(c->d_beta_temp + t, &beta, sizeof beta, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto& L : c->ladders)
+        if (t >= L.start && t < L.start + L.n) L.betas[t - L.start] = beta;
+    return ISING_OK;
+}
+
+int ising_get_betas(ising_ctx* c, double* out)
+{
+    if (!c || !out) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    std::vector<float> b(c->R);
+    HIP_TRY(c, hipMemcpyAsync(b.data(), c->d_beta_slot, c->R * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->R; ++k) out[k] = b[k];
+    return ISING_OK;
+}
+
+int ising_set_spins(ising_ctx* c, int replica, const double* soa, int64_t n)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    const Rep& r = c->reps[replica];
+    if (!soa || n != r.n) return set_err(c, ISING_ERR_INVALID_ARG, "spin buffer of %lld sites, replica has %lld",
+                                         (long long)n, (long long)r.n);
+    std::vector<float> h((size_t)3 * r.n);
+    for (int ix = 0; ix < r.lx; ++ix)
+        for (int iy = 0; iy < r.ly; ++iy)
+            for (int iz = 0; iz < r.lz; ++iz) {
+                const int64_t site = ((int64_t)ix * r.ly + iy) * r.lz + iz;
+                const double x = soa[site], y = soa[n + site], z = soa[2 * n + site];
+                const double nn = sqrt(x * x + y * y + z * z);
+                if (!std::isfinite(nn) || fabs(nn - 1.0) > 1e-3)
+                    return set_err(c, ISING_ERR_BAD_SPIN, "site %lld: |S| = %g (NaN, inf or not unit)", (long long)site, nn);
+                const int k = (ix + iy + iz) & 1;
+                const int64_t idx = (int64_t)k * r.half + ((int64_t)iz * r.ly + iy) * r.hx + (ix >> 1);
+                h[idx] = (float)x; h[r.n + idx] = (float)y; h[2 * r.n + idx] = (float)z;
+            }
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->d_spins + r.off, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_get_spins(ising_ctx* c, int replica, double* soa, int64_t n)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    const Rep& r = c->reps[replica];
+    if (!soa || n != r.n) return set_err(c, ISING_ERR_INVALID_ARG, "spin buffer size");
+    std::vector<float> h((size_t)3 * r.n);
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_spins + r.off, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int ix = 0; ix < r.lx; ++ix)
+        for (int iy = 0; iy < r.ly; ++iy)
+            for (int iz = 0; iz < r.lz; ++iz) {
+                const int64_t site = ((int64_t)ix * r.ly + iy) * r.lz + iz;
+                const int k = (ix + iy + iz) & 1;
+                const int64_t idx = (int64_t)k * r.half + ((int64_t)iz * r.ly + iy) * r.hx + (ix >> 1);
+                soa[site] = h[idx]; soa[n + site] = h[r.n + idx]; soa[2 * n + site] = h[2 * r.n + idx];
+            }
+    return ISING_OK;
+}
+
+static int init_common(ising_ctx* c, int replica, int mode, double frac)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (replica >= c->R) return set_err(c, ISING_ERR_INVALID_ARG, "replica %d", replica);
+    HIP_TRY(c, hipSetDevice(c->device));
+    int base = 0, nb = c->n_blocks;
+    if (replica >= 0) { base = c->reps[replica].first_block; nb = c->reps[replica].nblocks; }
+    hipLaunchKernelGGL(k_init, dim3(nb), dim3(kBlock), 0, c->stream, c->d_spins, c->d_reps, c->d_block_rep, base,
+                       mode, frac, (uint32_t)c->seed, (uint32_t)(c->seed >> 32));
+    LAUNCH_CHECK(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_init_random(ising_ctx* c, int replica) { return init_common(c, replica, 0, 0.0); }
+
+int ising_init_ferro(ising_ctx* c, int replica, double frac)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (!(frac >= 0.0 && frac <= 1.0)) return set_err(c, ISING_ERR_INVALID_ARG, "flip fraction %g", frac);
+    return init_common(c, replica, 1, frac);
+}
+
+int ising_sweep(ising_ctx* c, int n_sweeps)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (n_sweeps < 0) return set_err(c, ISING_ERR_INVALID_ARG, "n_sweeps %d", n_sweeps);
+    if (n_sweeps == 0) return ISING_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc;
+    for (int k = 0; k < n_sweeps; ++k) {
+        if ((rc = launch_half_sweeps(c, (uint32_t)k, 0))) return rc;
+        if ((rc = launch_half_sweeps(c, (uint32_t)k, 1))) return rc;
+    }
+    if ((rc = launch_add(c, 0, (uint32_t)n_sweeps))) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->host_sweep += (uint32_t)n_sweeps;
+    return ISING_OK;
+}
+
+int ising_energy(ising_ctx* c, double* e)
+{
+    if (!c || !e) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = launch_measure_all(c);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(e, c->d_energy, c->R * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_magnetisation(ising_ctx* c, double* m)
+{
+    if (!c || !m) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = launch_measure_all(c);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(m, c->d_mag, 3 * c->R * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_energy_with(ising_ctx* c, int replica, const ising_hamiltonian* h, double* e)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    if (!h || !e) return set_err(c, ISING_ERR_INVALID_ARG, "null argument");
+    if ((rc = check_ham(c, *h))) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const Rep& r = c->reps[replica];
+    Lad L = make_lad(*h, c->ladders[r.ladder].cone);
+    HIP_TRY(c, hipMemcpyAsync(c->d_lad_override, &L, sizeof L, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_measure, dim3(r.nblocks), dim3(kBlock), 0, c->stream, (const float*)c->d_spins, c->d_reps,
+                       c->d_lads, (const Lad*)c->d_lad_override, c->d_block_rep, r.first_block, c->d_partial);
+    LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_measure_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->d_reps, c->d_partial, replica,
+                       c->d_energy, c->d_mag);
+    LAUNCH_CHECK(c);
+    HIP_TRY(c, hipMemcpyAsync(e, c->d_energy + replica, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_winding_all(ising_ctx* c, int64_t* q, double* q_raw)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = launch_winding_all(c);
+    if (rc) return rc;
+    const size_t cnt = (size_t)c->R * c->zmax;
+    if (q) HIP_TRY(c, hipMemcpyAsync(q, c->d_qint, cnt * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    if (q_raw) HIP_TRY(c, hipMemcpyAsync(q_raw, c->d_qraw, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_winding(ising_ctx* c, int replica, int z, int64_t* q, double* q_raw)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    if (z < 0 || z >= c->reps[replica].lz) return set_err(c, ISING_ERR_INVALID_ARG, "layer %d out of range", z);
+    std::vector<int64_t> qi((size_t)c->R * c->zmax);
+    std::vector<double> qr((size_t)c->R * c->zmax);
+    if ((rc = ising_winding_all(c, qi.data(), qr.data()))) return rc;
+    if (q) *q = qi[(size_t)replica * c->zmax + z];
+    if (q_raw) *q_raw = qr[(size_t)replica * c->zmax + z];
+    return ISING_OK;
+}
+
+int ising_delta_energy(ising_ctx* c, int replica, const int32_t* sites, const double* props, int n, double* de)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!sites || !props || !de))) return set_err(c, ISING_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return ISING_OK;
+    const Rep& r = c->reps[replica];
+    std::vector<float> pf((size_t)3 * n);
+    for (int k = 0; k < n; ++k) {
+        if (sites[k] < 0 || sites[k] >= r.n) return set_err(c, ISING_ERR_INVALID_ARG, "site %d out of range", sites[k]);
+        for (int m = 0; m < 3; ++m) pf[3 * k + m] = (float)props[3 * k + m];
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (n > c->de_cap) {
+        if (c->d_sites) hipFree(c->d_sites);
+        if (c->d_props) hipFree(c->d_props);
+        if (c->d_de) hipFree(c->d_de);
+        c->d_sites = nullptr; c->d_props = nullptr; c->d_de = nullptr; c->de_cap = 0;
+        if ((rc = dalloc(c, &c->d_sites, n)) || (rc = dalloc(c, &c->d_props, 3 * (size_t)n)) || (rc = dalloc(c, &c->d_de, n)))
+            return rc;
+        c->de_cap = n;
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->d_sites, sites, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_props, pf.data(), pf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_delta_energy, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
+                       (const float*)c->d_spins, c->d_reps, c->d_lads, replica, (const int32_t*)c->d_sites,
+                       (const float*)c->d_props, n, c->d_de);
+    LAUNCH_CHECK(c);
+    HIP_TRY(c, hipMemcpyAsync(de, c->d_de, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* out)
+{
+    if (!c || !sched) return ISING_ERR_INVALID_ARG;
+    const ising_schedule s = *sched;
+    if (s.n_sweeps < 0 || s.swap_every < 0 || s.measure_every < 0 || s.winding_every < 0)
+        return set_err(c, ISING_ERR_INVALID_ARG, "negative schedule entry");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc;
+    SeriesDev sd;
+    sd.n_meas = s.measure_every ? (uint32_t)(s.n_sweeps / s.measure_every) : 0u;
+    sd.n_q = s.winding_every ? (uint32_t)(s.n_sweeps / s.winding_every) : 0u;
+    const size_t R = (size_t)c->R, Z = (size_t)c->zmax;
+    auto cleanup = [&]() {
+        if (sd.e) hipFree(sd.e);
+        if (sd.m) hipFree(sd.m);
+        if (sd.qr) hipFree(sd.qr);
+        if (sd.qi) hipFree(sd.qi);
+    };
+    if (sd.n_meas) {
+        if ((rc = dalloc(c, &sd.e, sd.n_meas * R)) || (rc = dalloc(c, &sd.m, sd.n_meas * R * 3))) { cleanup(); return rc; }
+    }
+    if (sd.n_q) {
+        if ((rc = dalloc(c, &sd.qr, sd.n_q * R * Z)) || (rc = dalloc(c, &sd.qi, sd.n_q * R * Z))) { cleanup(); return rc; }
+    }
+    // Record counters restart at zero for this run (plain kernel, no memset).
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, c->d_ctr + 1, 2, 0u);
+    if (hipGetLastError() != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "counter reset"); }
+
+    int64_t period = 1;
+    for (int v : { s.swap_every, s.measure_every, s.winding_every })
+        if (v > 0) period = period / gcd64(period, v) * v;
+    const bool graph = s.use_graph && period <= 512 && s.n_sweeps >= 2 * period;
+    int done = 0;
+    if (graph) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            cleanup();
+            return set_err(c, ISING_ERR_HIP, "hipStreamBeginCapture");
+        }
+        rc = enqueue_period(c, s, (int)period, sd);
+        hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        if (rc || ec != hipSuccess) { if (g) hipGraphDestroy(g); cleanup(); return rc ? rc : set_err(c, ISING_ERR_HIP, "capture"); }
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(g); cleanup();
+            return set_err(c, ISING_ERR_HIP, "hipGraphInstantiate");
+        }
+        const int reps = s.n_sweeps / (int)period;
+        for (int k = 0; k < reps; ++k) {
+            if (hipGraphLaunch(ge, c->stream) != hipSuccess) {
+                hipGraphExecDestroy(ge); hipGraphDestroy(g); cleanup();
+                return set_err(c, ISING_ERR_HIP, "hipGraphLaunch");
+            }
+        }
+        done = reps * (int)period;
+        hipError_t es = hipStreamSynchronize(c->stream);
+        hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        if (es != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "graph replay: %s", hipGetErrorString(es)); }
+    }
+    // Remaining sweeps eagerly, period by period (records keep their order).
+    while (done < s.n_sweeps) {
+        const int cnt = (int)std::min<int64_t>(period, s.n_sweeps - done);
+        if ((rc = enqueue_period(c, s, cnt, sd))) { cleanup(); return rc; }
+        done += cnt;
+    }
+    hipError_t es = hipStreamSynchronize(c->stream);
+    if (es != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "pt_run: %s", hipGetErrorString(es)); }
+    c->host_sweep += (uint32_t)s.n_sweeps;
+    if (out) {
+        hipError_t e1 = hipSuccess;
+        if (out->energy && sd.n_meas)
+            e1 = hipMemcpy(out->energy, sd.e, sd.n_meas * R * sizeof(double), hipMemcpyDeviceToHost);
+        if (e1 == hipSuccess && out->magnetisation && sd.n_meas)
+            e1 = hipMemcpy(out->magnetisation, sd.m, sd.n_meas * R * 3 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e1 == hipSuccess && out->winding && sd.n_q)
+            e1 = hipMemcpy(out->winding, sd.qi, sd.n_q * R * Z * sizeof(int64_t), hipMemcpyDeviceToHost);
+        if (e1 == hipSuccess && out->winding_raw && sd.n_q)
+            e1 = hipMemcpy(out->winding_raw, sd.qr, sd.n_q * R * Z * sizeof(double), hipMemcpyDeviceToHost);
+        if (e1 != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "series copy-out"); }
+    }
+    cleanup();
+    return ISING_OK;
+}
+
+int ising_temperature_map(ising_ctx* c, int32_t* slot_of_temp)
+{
+    if (!c || !slot_of_temp) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(slot_of_temp, c->d_slot_of_temp, c->R * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_swap_stats(ising_ctx* c, int64_t* attempts, int64_t* accepts)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (attempts) HIP_TRY(c, hipMemcpyAsync(attempts, c->d_swap_att, c->R * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    if (accepts) HIP_TRY(c, hipMemcpyAsync(accepts, c->d_swap_acc, c->R * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_time_sweeps(ising_ctx* c, int n_sweeps, double* ms_per_launch)
+{
+    if (!c || !ms_per_launch || n_sweeps < 1) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
+    int rc = ISING_OK;
+    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
+    for (int k = 0; k < n_sweeps && !rc; ++k) {
+        rc = launch_half_sweeps(c, (uint32_t)k, 0);
+        if (!rc) rc = launch_half_sweeps(c, (uint32_t)k, 1);
+    }
+    if (!rc && hipEventRecord(e1, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
+    if (!rc) rc = launch_add(c, 0, (uint32_t)n_sweeps);
+    float ms = 0.f;
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "sync");
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventElapsedTime");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    c->host_sweep += (uint32_t)n_sweeps;
+    *ms_per_launch = (double)ms / (2.0 * n_sweeps);
+    return ISING_OK;
+}
+
+int ising_synchronize(ising_ctx* c)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipDeviceSynchronize());
+    return ISING_OK;
+}
+
+int ising_accept_counts(ising_ctx* c, int64_t* accepted)
+{
+    if (!c || !accepted) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(accepted, c->d_acc, c->R * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,289 @@
+"""Python host mirror of the C ABI (include/ising_gpu.h).
+
+The classes keep the reference's vocabulary (ladders of temperatures,
+replicas, sweeps, swaps) and map one to one onto the C entry points:
+
+    Engine.sweep            -> ising_sweep           (Metropolis_mcs, src/energyMin.f90:234-374)
+    Engine.energy           -> ising_energy          (TotalHeisenbergEnergy, src/energyMin.f90:499-528)
+    Engine.delta_energy     -> ising_delta_energy    (calculateHeisenbergEnergy, src/energyMin.f90:86-171)
+    Engine.winding          -> ising_winding         (calculate_winding_number2, src/reciprocalSpaceProcesses.f90:320-364)
+    Engine.magnetisation    -> ising_magnetisation   (src/PT-Utils.f90:59-65)
+    Engine.pt_run           -> ising_pt_run          (PT loop, src/PT.f90:219-273)
+
+Spin arrays use the reference's atomSpins(numAtoms,3) column-major layout,
+i.e. numpy arrays of shape (3, N), sites in the order (ix*Ly + iy)*Lz + iz.
+Every error from the library raises IsingError; nothing falls back to CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from . import _capi
+from ._capi import DMI_BULK, DMI_INTERFACIAL_REF, DMI_NONE
+
+_DMI = {"none": DMI_NONE, "bulk": DMI_BULK, "interfacial_ref": DMI_INTERFACIAL_REF}
+
+
+class IsingError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_capi.ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+@dataclass
+class Lattice:
+    lx: int
+    ly: int
+    lz: int
+    periodic: tuple = (True, True, True)
+
+    @property
+    def sites(self) -> int:
+        return self.lx * self.ly * self.lz
+
+    def _c(self) -> _capi.ising_lattice:
+        lat = _capi.ising_lattice(self.lx, self.ly, self.lz)
+        for a in range(3):
+            lat.periodic[a] = 1 if self.periodic[a] else 0
+        return lat
+
+
+@dataclass
+class Hamiltonian:
+    """E = -J sum S_i.S_j + sum d_a.(S_i x S_{i+a}) - s*Bz sum S_z - Kz sum S_z^2."""
+    J: float = 1.0
+    D: float = 0.0
+    dmi: str = "bulk"
+    Bz: float = 0.0
+    Kz: float = 0.0
+    zeeman_total_scale: float = 1.0
+
+    @classmethod
+    def reference(cls, J: float, Dz: float, B: float, total_energy_convention: bool = True) -> "Hamiltonian":
+        """The reference's parameters: exchange +J S.S (src/energyMin.f90:139),
+        interfacial DMI Dz (c x r) (:114,137), Zeeman -B S_z in single-site
+        changes (:161) and -2 B S_z in total energies (:222)."""
+        return cls(J=-J, D=Dz, dmi="interfacial_ref", Bz=B, Kz=0.0,
+                   zeeman_total_scale=2.0 if total_energy_convention else 1.0)
+
+    def _c(self) -> _capi.ising_hamiltonian:
+        if self.dmi not in _DMI:
+            raise ValueError(f"dmi must be one of {sorted(_DMI)}")
+        return _capi.ising_hamiltonian(J=self.J, D=self.D, dmi_type=_DMI[self.dmi], reserved=0,
+                                       Bz=self.Bz, Kz=self.Kz, zeeman_total_scale=self.zeeman_total_scale)
+
+
+@dataclass
+class Ladder:
+    lattice: Lattice
+    ham: Hamiltonian
+    betas: Sequence[float]
+    cone_radius: float = 0.5
+
+
+def geometric_temperatures(t_min: float, t_max: float, n: int) -> np.ndarray:
+    """T_j = Tmin (Tmax/Tmin)^(j/(n-1)), the ladder of src/PT.f90:202."""
+    if n == 1:
+        return np.array([t_min], dtype=np.float64)
+    j = np.arange(n, dtype=np.float64)
+    return t_min * (t_max / t_min) ** (j / (n - 1))
+
+
+class Engine:
+    """A context of one or more PT ladders resident on one GPU."""
+
+    def __init__(self, ladders: Sequence[Ladder] | Ladder, seed: int = 20261203, device: int = 0,
+                 lib_path=None):
+        if isinstance(ladders, Ladder):
+            ladders = [ladders]
+        self._lib = _capi.load(lib_path)
+        self.ladders = list(ladders)
+        descs = (_capi.ising_ladder_desc * len(self.ladders))()
+        self._beta_bufs = []
+        for k, L in enumerate(self.ladders):
+            b = np.ascontiguousarray(L.betas, dtype=np.float64)
+            self._beta_bufs.append(b)
+            descs[k].lattice = L.lattice._c()
+            descs[k].ham = L.ham._c()
+            descs[k].n_replicas = len(b)
+            descs[k].cone_radius = float(L.cone_radius)
+            descs[k].betas = b.ctypes.data_as(C.POINTER(C.c_double))
+        h = C.c_void_p()
+        rc = self._lib.ising_create(C.byref(h), descs, len(self.ladders), C.c_uint64(seed), device)
+        if rc != 0:
+            raise IsingError(rc, self._lib.ising_last_error(None).decode())
+        self._h = h
+        self.seed = seed
+        n = C.c_int()
+        self._check(self._lib.ising_num_replicas(h, C.byref(n)))
+        self.n_replicas = n.value
+        self.replica_lattice = []
+        self.replica_ladder = []
+        for li, L in enumerate(self.ladders):
+            for _ in range(len(L.betas)):
+                self.replica_lattice.append(L.lattice)
+                self.replica_ladder.append(li)
+        self.zmax = max(L.lattice.lz for L in self.ladders)
+
+    # -- plumbing -----------------------------------------------------------
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise IsingError(rc, self._lib.ising_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.ising_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- parameters ---------------------------------------------------------
+    def set_hamiltonian(self, ladder: int, ham: Hamiltonian) -> None:
+        hc = ham._c()
+        self._check(self._lib.ising_set_hamiltonian(self._h, ladder, C.byref(hc)))
+        self.ladders[ladder].ham = ham
+
+    def set_cone(self, ladder: int, radius: float) -> None:
+        self._check(self._lib.ising_set_cone(self._h, ladder, float(radius)))
+        self.ladders[ladder].cone_radius = radius
+
+    def set_beta(self, replica: int, beta: float) -> None:
+        self._check(self._lib.ising_set_beta(self._h, replica, float(beta)))
+
+    def betas(self) -> np.ndarray:
+        out = np.zeros(self.n_replicas)
+        self._check(self._lib.ising_get_betas(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    # -- spins --------------------------------------------------------------
+    def sites(self, replica: int) -> int:
+        return self.replica_lattice[replica].sites
+
+    def set_spins(self, replica: int, spins: np.ndarray) -> None:
+        s = np.ascontiguousarray(spins, dtype=np.float64)
+        n = self.sites(replica)
+        if s.shape != (3, n):
+            raise ValueError(f"spins must have shape (3, {n})")
+        self._check(self._lib.ising_set_spins(self._h, replica, s.ctypes.data_as(C.POINTER(C.c_double)), n))
+
+    def get_spins(self, replica: int) -> np.ndarray:
+        n = self.sites(replica)
+        s = np.empty((3, n), dtype=np.float64)
+        self._check(self._lib.ising_get_spins(self._h, replica, s.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return s
+
+    def init_random(self, replica: int = -1) -> None:
+        self._check(self._lib.ising_init_random(self._h, replica))
+
+    def init_ferro(self, replica: int = -1, flip_fraction: float = 0.0) -> None:
+        self._check(self._lib.ising_init_ferro(self._h, replica, float(flip_fraction)))
+
+    # -- hot path -----------------------------------------------------------
+    def sweep(self, n_sweeps: int = 1) -> None:
+        self._check(self._lib.ising_sweep(self._h, int(n_sweeps)))
+
+    def energy(self) -> np.ndarray:
+        e = np.empty(self.n_replicas)
+        self._check(self._lib.ising_energy(self._h, e.ctypes.data_as(C.POINTER(C.c_double))))
+        return e
+
+    def energy_with(self, replica: int, ham: Hamiltonian) -> float:
+        e = C.c_double()
+        hc = ham._c()
+        self._check(self._lib.ising_energy_with(self._h, replica, C.byref(hc), C.byref(e)))
+        return e.value
+
+    def magnetisation(self) -> np.ndarray:
+        m = np.empty((self.n_replicas, 3))
+        self._check(self._lib.ising_magnetisation(self._h, m.ctypes.data_as(C.POINTER(C.c_double))))
+        return m
+
+    def winding(self, replica: int, z: int) -> tuple[int, float]:
+        q = C.c_int64()
+        qr = C.c_double()
+        self._check(self._lib.ising_winding(self._h, replica, z, C.byref(q), C.byref(qr)))
+        return q.value, qr.value
+
+    def winding_all(self) -> tuple[np.ndarray, np.ndarray]:
+        q = np.empty((self.n_replicas, self.zmax), dtype=np.int64)
+        qr = np.empty((self.n_replicas, self.zmax))
+        self._check(self._lib.ising_winding_all(self._h, q.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                qr.ctypes.data_as(C.POINTER(C.c_double))))
+        return q, qr
+
+    def delta_energy(self, replica: int, sites: np.ndarray, props: np.ndarray) -> np.ndarray:
+        s = np.ascontiguousarray(sites, dtype=np.int32)
+        p = np.ascontiguousarray(props, dtype=np.float64).reshape(-1, 3)
+        if len(s) != len(p):
+            raise ValueError("sites and proposals differ in length")
+        out = np.empty(len(s))
+        self._check(self._lib.ising_delta_energy(self._h, replica, s.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                 p.ctypes.data_as(C.POINTER(C.c_double)), len(s),
+                                                 out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    # -- parallel tempering -------------------------------------------------
+    def pt_run(self, n_sweeps: int, swap_every: int = 10, measure_every: int = 1, winding_every: int = 0,
+               use_graph: bool = True, want: Sequence[str] = ("energy", "magnetisation", "winding")) -> dict:
+        """Run the PT schedule on device; returns series indexed by temperature
+        position: energy [n_meas, R], magnetisation [n_meas, R, 3],
+        winding / winding_raw [n_q, R, Zmax]."""
+        R, Z = self.n_replicas, self.zmax
+        n_meas = n_sweeps // measure_every if measure_every else 0
+        n_q = n_sweeps // winding_every if winding_every else 0
+        out = {}
+        ser = _capi.ising_series()
+        if n_meas and "energy" in want:
+            out["energy"] = np.zeros((n_meas, R))
+            ser.energy = out["energy"].ctypes.data_as(C.POINTER(C.c_double))
+        if n_meas and "magnetisation" in want:
+            out["magnetisation"] = np.zeros((n_meas, R, 3))
+            ser.magnetisation = out["magnetisation"].ctypes.data_as(C.POINTER(C.c_double))
+        if n_q and "winding" in want:
+            out["winding"] = np.zeros((n_q, R, Z), dtype=np.int64)
+            out["winding_raw"] = np.zeros((n_q, R, Z))
+            ser.winding = out["winding"].ctypes.data_as(C.POINTER(C.c_int64))
+            ser.winding_raw = out["winding_raw"].ctypes.data_as(C.POINTER(C.c_double))
+        sch = _capi.ising_schedule(n_sweeps, swap_every, measure_every, winding_every, 1 if use_graph else 0, 0)
+        self._check(self._lib.ising_pt_run(self._h, C.byref(sch), C.byref(ser)))
+        return out
+
+    def temperature_map(self) -> np.ndarray:
+        m = np.empty(self.n_replicas, dtype=np.int32)
+        self._check(self._lib.ising_temperature_map(self._h, m.ctypes.data_as(C.POINTER(C.c_int32))))
+        return m
+
+    def swap_stats(self) -> tuple[np.ndarray, np.ndarray]:
+        a = np.zeros(self.n_replicas, dtype=np.int64)
+        b = np.zeros(self.n_replicas, dtype=np.int64)
+        self._check(self._lib.ising_swap_stats(self._h, a.ctypes.data_as(C.POINTER(C.c_int64)),
+                                               b.ctypes.data_as(C.POINTER(C.c_int64))))
+        return a, b
+
+    def time_sweeps(self, n_sweeps: int) -> float:
+        """Average device time (ms) of one half-sweep launch over n_sweeps sweeps."""
+        ms = C.c_double()
+        self._check(self._lib.ising_time_sweeps(self._h, int(n_sweeps), C.byref(ms)))
+        return ms.value
+
+    def synchronize(self) -> None:
+        self._check(self._lib.ising_synchronize(self._h))
+
+    def accept_counts(self) -> np.ndarray:
+        a = np.zeros(self.n_replicas, dtype=np.int64)
+        self._check(self._lib.ising_accept_counts(self._h, a.ctypes.data_as(C.POINTER(C.c_int64))))
+        return a

@@ -1,0 +1,88 @@
+/*
+ * ising_oracle.h -- FP64 CPU restatement of the reference's Metropolis /
+ * parallel-tempering hot path (SamuelJosephs/ISING), used ONLY as test
+ * infrastructure: by tests/, by __graft_entry__.smoke() as the checker, and
+ * by bench.py's cpu_baseline leg. The product (libising_hip.so and the
+ * ising_amd package) never links or calls it.
+ *
+ * Two families of functions:
+ *   orc_ref_*  follow the reference's own code path line by line (positions,
+ *              minimum-image bond vectors, cross products, its RNG);
+ *   orc_*      state the same physics in the form the GPU path uses
+ *              (checkerboard sweep, Philox counters, per-axis DMI vectors),
+ *              in FP64, so GPU results can be compared to them.
+ *
+ * Spin arrays: double s[3*N], column-major (all x, then y, then z), sites in
+ * the reference order site = (ix*Ly + iy)*Lz + iz (src/chainMesh.f90:673-685).
+ *
+ * Pinning (which check fixes each function) is listed per function in
+ * ising_oracle.c and in the test files under tests/.
+ */
+#ifndef ISING_ORACLE_H
+#define ISING_ORACLE_H
+#include <stdint.h>
+
+typedef struct { int32_t lx, ly, lz; int32_t periodic[3]; } orc_lattice;
+typedef struct {
+    double J, D; int32_t dmi_type; int32_t reserved;
+    double Bz, Kz, zeeman_total_scale;
+} orc_ham;
+
+#define ORC_DMI_NONE 0
+#define ORC_DMI_BULK 1
+#define ORC_DMI_INTERFACIAL_REF 2
+
+/* Philox counter streams (word 2 of the counter). */
+#define ORC_STREAM_METROPOLIS 0u
+#define ORC_STREAM_INIT       1u
+#define ORC_STREAM_SWAP       2u
+
+/* ---- RNG ---- */
+void    orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void    orc_metropolis_random(uint64_t seed, uint32_t slot, uint32_t site, uint32_t sweep, uint32_t out[4]);
+typedef struct { int32_t seed, ix, iy; } orc_ref_rng;
+orc_ref_rng orc_ref_make_random(int32_t seed);
+double  orc_ref_uniform(orc_ref_rng* r);
+
+/* ---- geometry ---- */
+int64_t orc_site(const orc_lattice* L, int ix, int iy, int iz);
+int64_t orc_neighbour(const orc_lattice* L, int64_t site, int axis, int dir);
+void    orc_dmi_vectors(const orc_ham* h, double d[3][3]);
+
+/* ---- energies (build form) ---- */
+void    orc_local_field(const orc_lattice* L, const orc_ham* h, const double* s, int64_t site, double out[3]);
+double  orc_delta_energy(const orc_lattice* L, const orc_ham* h, const double* s, int64_t site, const double sn[3]);
+double  orc_total_energy(const orc_lattice* L, const orc_ham* h, const double* s);
+void    orc_magnetisation(const orc_lattice* L, const double* s, double m[3]);
+
+/* ---- reference restatements ---- */
+void    orc_ref_local_energy(const orc_lattice* L, double J_r, double Dz, double B, const double* s,
+                             int64_t site, const double sp[3], double* old_e, double* new_e);
+double  orc_ref_total_energy(const orc_lattice* L, double J_r, double Dz, double B, const double* s);
+double  orc_ref_arc_winding(const double s1[3], const double s2[3], const double s3[3]);
+double  orc_ref_winding_number(const orc_lattice* L, const double* s, int z_index_1based);
+
+/* ---- topological charge (build form, same op order as the GPU) ---- */
+double  orc_triangle_sigma(const float a[3], const float b[3], const float c[3]);
+double  orc_winding_layer(const orc_lattice* L, const double* s, int z, int64_t* q_int);
+
+/* ---- Metropolis ---- */
+int     orc_propose(const double sold[3], double radius, const uint32_t rnd[4], double snew[3], double* u_acc);
+int64_t orc_half_sweep(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
+                       uint64_t seed, uint32_t slot, uint32_t sweep, int colour);
+int64_t orc_sweeps(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
+                   uint64_t seed, uint32_t slot, uint32_t sweep0, int n_sweeps);
+void    orc_init_random(const orc_lattice* L, double* s, uint64_t seed, uint32_t slot);
+void    orc_init_ferro(const orc_lattice* L, double* s, uint64_t seed, uint32_t slot, double frac);
+
+/* ---- parallel tempering ---- */
+double  orc_swap_uniform(uint64_t seed, uint32_t ladder, uint32_t pair, uint32_t round);
+int     orc_pt_swap_round(int n_temps, const double* beta_of_temp, const double* e_of_slot,
+                          int32_t* slot_of_temp, uint64_t seed, uint32_t ladder, uint32_t round,
+                          int64_t* attempts, int64_t* accepts);
+
+/* ---- threads ---- */
+int     orc_num_threads(void);
+void    orc_set_threads(int n);
+
+#endif

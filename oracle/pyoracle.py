@@ -1,0 +1,251 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/libising_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py's cpu_baseline leg. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libising_oracle.so"
+
+DMI = {"none": 0, "bulk": 1, "interfacial_ref": 2}
+
+
+class orc_lattice(C.Structure):
+    _fields_ = [("lx", C.c_int32), ("ly", C.c_int32), ("lz", C.c_int32), ("periodic", C.c_int32 * 3)]
+
+
+class orc_ham(C.Structure):
+    _fields_ = [("J", C.c_double), ("D", C.c_double), ("dmi_type", C.c_int32), ("reserved", C.c_int32),
+                ("Bz", C.c_double), ("Kz", C.c_double), ("zeeman_total_scale", C.c_double)]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE)], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        u32p = C.POINTER(C.c_uint32)
+        dp = C.POINTER(C.c_double)
+        lp, hp = C.POINTER(orc_lattice), C.POINTER(orc_ham)
+        sig = {
+            "orc_philox4x32_10": (None, [u32p, u32p, u32p]),
+            "orc_metropolis_random": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, u32p]),
+            "orc_site": (C.c_int64, [lp, C.c_int, C.c_int, C.c_int]),
+            "orc_neighbour": (C.c_int64, [lp, C.c_int64, C.c_int, C.c_int]),
+            "orc_local_field": (None, [lp, hp, dp, C.c_int64, dp]),
+            "orc_delta_energy": (C.c_double, [lp, hp, dp, C.c_int64, dp]),
+            "orc_total_energy": (C.c_double, [lp, hp, dp]),
+            "orc_magnetisation": (None, [lp, dp, dp]),
+            "orc_ref_local_energy": (None, [lp, C.c_double, C.c_double, C.c_double, dp, C.c_int64, dp, dp, dp]),
+            "orc_ref_total_energy": (C.c_double, [lp, C.c_double, C.c_double, C.c_double, dp]),
+            "orc_ref_arc_winding": (C.c_double, [dp, dp, dp]),
+            "orc_ref_winding_number": (C.c_double, [lp, dp, C.c_int]),
+            "orc_winding_layer": (C.c_double, [lp, dp, C.c_int, C.POINTER(C.c_int64)]),
+            "orc_propose": (C.c_int, [dp, C.c_double, u32p, dp, dp]),
+            "orc_half_sweep": (C.c_int64, [lp, hp, dp, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                                           C.c_uint32, C.c_int]),
+            "orc_sweeps": (C.c_int64, [lp, hp, dp, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                                       C.c_uint32, C.c_int]),
+            "orc_init_random": (None, [lp, dp, C.c_uint64, C.c_uint32]),
+            "orc_init_ferro": (None, [lp, dp, C.c_uint64, C.c_uint32, C.c_double]),
+            "orc_swap_uniform": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
+            "orc_pt_swap_round": (C.c_int, [C.c_int, dp, dp, C.POINTER(C.c_int32), C.c_uint64, C.c_uint32,
+                                            C.c_uint32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+            "orc_num_threads": (C.c_int, []),
+            "orc_set_threads": (None, [C.c_int]),
+        }
+
+        class ref_rng(C.Structure):
+            _fields_ = [("seed", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32)]
+        L.ref_rng = ref_rng
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L.orc_ref_make_random.restype = ref_rng
+        L.orc_ref_make_random.argtypes = [C.c_int32]
+        L.orc_ref_uniform.restype = C.c_double
+        L.orc_ref_uniform.argtypes = [C.POINTER(ref_rng)]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def lattice(lx, ly, lz, periodic=(True, True, True)) -> orc_lattice:
+    L = orc_lattice(lx, ly, lz)
+    for a in range(3):
+        L.periodic[a] = 1 if periodic[a] else 0
+    return L
+
+
+def ham(J=1.0, D=0.0, dmi="bulk", Bz=0.0, Kz=0.0, zeeman_total_scale=1.0) -> orc_ham:
+    return orc_ham(J, D, DMI[dmi], 0, Bz, Kz, zeeman_total_scale)
+
+
+def ham_from(h) -> orc_ham:
+    """Oracle Hamiltonian from an object with the ising_amd.Hamiltonian fields."""
+    return ham(h.J, h.D, h.dmi, h.Bz, h.Kz, h.zeeman_total_scale)
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    k = (C.c_uint32 * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return np.array(list(o), dtype=np.uint32)
+
+
+def metropolis_random(seed, slot, site, sweep) -> np.ndarray:
+    o = (C.c_uint32 * 4)()
+    lib().orc_metropolis_random(seed, slot, site, sweep, o)
+    return np.array(list(o), dtype=np.uint32)
+
+
+def site(L, ix, iy, iz) -> int:
+    return lib().orc_site(C.byref(L), ix, iy, iz)
+
+
+def neighbour(L, s, axis, d) -> int:
+    return lib().orc_neighbour(C.byref(L), s, axis, d)
+
+
+def _spins(s):
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    assert s.ndim == 2 and s.shape[0] == 3
+    return s
+
+
+def local_field(L, h, s, i) -> np.ndarray:
+    s = _spins(s)
+    o = np.zeros(3)
+    lib().orc_local_field(C.byref(L), C.byref(h), _dp(s), i, _dp(o))
+    return o
+
+
+def delta_energy(L, h, s, i, sn) -> float:
+    s = _spins(s)
+    sn = np.ascontiguousarray(sn, dtype=np.float64)
+    return lib().orc_delta_energy(C.byref(L), C.byref(h), _dp(s), i, _dp(sn))
+
+
+def total_energy(L, h, s) -> float:
+    s = _spins(s)
+    return lib().orc_total_energy(C.byref(L), C.byref(h), _dp(s))
+
+
+def magnetisation(L, s) -> np.ndarray:
+    s = _spins(s)
+    m = np.zeros(3)
+    lib().orc_magnetisation(C.byref(L), _dp(s), _dp(m))
+    return m
+
+
+def ref_local_energy(L, J_r, Dz, B, s, i, sp) -> tuple[float, float]:
+    s = _spins(s)
+    sp = np.ascontiguousarray(sp, dtype=np.float64)
+    o, n = C.c_double(), C.c_double()
+    lib().orc_ref_local_energy(C.byref(L), J_r, Dz, B, _dp(s), i, _dp(sp), C.byref(o), C.byref(n))
+    return o.value, n.value
+
+
+def ref_total_energy(L, J_r, Dz, B, s) -> float:
+    s = _spins(s)
+    return lib().orc_ref_total_energy(C.byref(L), J_r, Dz, B, _dp(s))
+
+
+def ref_arc_winding(a, b, c) -> float:
+    a, b, c = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c))
+    return lib().orc_ref_arc_winding(_dp(a), _dp(b), _dp(c))
+
+
+def ref_winding_number(L, s, z1) -> float:
+    s = _spins(s)
+    return lib().orc_ref_winding_number(C.byref(L), _dp(s), z1)
+
+
+def winding_layer(L, s, z) -> tuple[int, float]:
+    s = _spins(s)
+    q = C.c_int64()
+    r = lib().orc_winding_layer(C.byref(L), _dp(s), z, C.byref(q))
+    return q.value, r
+
+
+def propose(sold, radius, rnd) -> tuple[int, np.ndarray, float]:
+    so = np.ascontiguousarray(sold, dtype=np.float64)
+    sn = np.zeros(3)
+    u = C.c_double()
+    r = (C.c_uint32 * 4)(*[int(v) for v in rnd])
+    ok = lib().orc_propose(_dp(so), radius, r, _dp(sn), C.byref(u))
+    return ok, sn, u.value
+
+
+def half_sweep(L, h, s, beta, radius, seed, slot, sweep, colour) -> int:
+    """In place on s (shape (3,N) float64, C-contiguous)."""
+    assert s.flags.c_contiguous and s.dtype == np.float64
+    return lib().orc_half_sweep(C.byref(L), C.byref(h), _dp(s), beta, radius, seed, slot, sweep, colour)
+
+
+def sweeps(L, h, s, beta, radius, seed, slot, sweep0, n) -> int:
+    assert s.flags.c_contiguous and s.dtype == np.float64
+    return lib().orc_sweeps(C.byref(L), C.byref(h), _dp(s), beta, radius, seed, slot, sweep0, n)
+
+
+def init_random(L, seed, slot) -> np.ndarray:
+    s = np.zeros((3, L.lx * L.ly * L.lz))
+    lib().orc_init_random(C.byref(L), _dp(s), seed, slot)
+    return s
+
+
+def init_ferro(L, seed, slot, frac) -> np.ndarray:
+    s = np.zeros((3, L.lx * L.ly * L.lz))
+    lib().orc_init_ferro(C.byref(L), _dp(s), seed, slot, frac)
+    return s
+
+
+def swap_uniform(seed, ladder, pair, rnd) -> float:
+    return lib().orc_swap_uniform(seed, ladder, pair, rnd)
+
+
+def pt_swap_round(beta_of_temp, e_of_slot, slot_of_temp, seed, ladder, rnd, attempts=None, accepts=None) -> int:
+    b = np.ascontiguousarray(beta_of_temp, dtype=np.float64)
+    e = np.ascontiguousarray(e_of_slot, dtype=np.float64)
+    assert slot_of_temp.dtype == np.int32 and slot_of_temp.flags.c_contiguous
+    ap = attempts.ctypes.data_as(C.POINTER(C.c_int64)) if attempts is not None else None
+    cp = accepts.ctypes.data_as(C.POINTER(C.c_int64)) if accepts is not None else None
+    return lib().orc_pt_swap_round(len(b), _dp(b), _dp(e), slot_of_temp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   seed, ladder, rnd, ap, cp)
+
+
+def ref_rng(seed):
+    return lib().orc_ref_make_random(seed)
+
+
+def ref_uniform(rng) -> float:
+    return lib().orc_ref_uniform(C.byref(rng))
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(n)
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
