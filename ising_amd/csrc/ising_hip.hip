@@ -80,15 +80,37 @@ struct Lad {
 
 struct f3 { float x, y, z; };
 
-__device__ __forceinline__ f3 ld3(const float* __restrict__ b, uint32_t i, int64_t n)
+// Element i of a component plane: a 32-bit byte offset from a wavefront-
+// uniform base, so loads use the scalar-base + 32-bit vector-offset form.
+__device__ __forceinline__ float ldf(const float* __restrict__ base, uint32_t i)
 {
-    return f3{ b[i], b[i + n], b[i + 2 * n] };
+    return *(const float*)((const char*)base + (i << 2));
 }
-__device__ __forceinline__ f3 ld3c(const float* __restrict__ b, uint32_t i, int64_t n, bool ok)
+__device__ __forceinline__ void stf(float* __restrict__ base, uint32_t i, float v)
+{
+    *(float*)((char*)base + (i << 2)) = v;
+}
+
+// The three component planes (x, y, z) of one colour of one replica.
+struct Planes { const float* x; const float* y; const float* z; };
+
+__device__ __forceinline__ Planes planes(const float* spins, const struct Rep& R, int colour);
+
+__device__ __forceinline__ f3 ld3(const Planes& P, uint32_t i)
+{
+    return f3{ ldf(P.x, i), ldf(P.y, i), ldf(P.z, i) };
+}
+__device__ __forceinline__ f3 ld3c(const Planes& P, uint32_t i, bool ok)
 {
     f3 v{ 0.f, 0.f, 0.f };
-    if (ok) v = ld3(b, i, n);
+    if (ok) v = ld3(P, i);
     return v;
+}
+
+__device__ __forceinline__ Planes planes(const float* spins, const Rep& R, int colour)
+{
+    const float* b = spins + R.off + (int64_t)colour * R.half;
+    return Planes{ b, b + R.n, b + 2 * R.n };
 }
 
 // Neighbour indices (inside the other colour's arrays) of the site at half
@@ -125,14 +147,14 @@ __device__ __forceinline__ Nbr neighbours(const Rep& R, uint32_t row, uint32_t h
 }
 
 // Local field h = J sum_j S_j - sum_a (S_{+a} - S_{-a}) x d_a + Bz z.
-__device__ __forceinline__ f3 local_field(const Lad& L, const float* __restrict__ oth, int64_t n, const Nbr& nb)
+__device__ __forceinline__ f3 local_field(const Lad& L, const Planes& oth, const Nbr& nb)
 {
-    const f3 xp = ld3c(oth, nb.i[0], n, nb.ok[0]);
-    const f3 xm = ld3c(oth, nb.i[1], n, nb.ok[1]);
-    const f3 yp = ld3c(oth, nb.i[2], n, nb.ok[2]);
-    const f3 ym = ld3c(oth, nb.i[3], n, nb.ok[3]);
-    const f3 zp = ld3c(oth, nb.i[4], n, nb.ok[4]);
-    const f3 zm = ld3c(oth, nb.i[5], n, nb.ok[5]);
+    const f3 xp = ld3c(oth, nb.i[0], nb.ok[0]);
+    const f3 xm = ld3c(oth, nb.i[1], nb.ok[1]);
+    const f3 yp = ld3c(oth, nb.i[2], nb.ok[2]);
+    const f3 ym = ld3c(oth, nb.i[3], nb.ok[3]);
+    const f3 zp = ld3c(oth, nb.i[4], nb.ok[4]);
+    const f3 zm = ld3c(oth, nb.i[5], nb.ok[5]);
     const f3 sum{ (xp.x + xm.x) + (yp.x + ym.x) + (zp.x + zm.x),
                   (xp.y + xm.y) + (yp.y + ym.y) + (zp.y + zm.y),
                   (xp.z + xm.z) + (yp.z + ym.z) + (zp.z + zm.z) };
@@ -180,9 +202,8 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
     const bool active = q < R.half;
     bool accepted = false;
     if (active) {
-        const int64_t n = R.n;
-        float* own = spins + R.off + (int64_t)colour * R.half;
-        const float* oth = spins + R.off + (int64_t)(colour ^ 1) * R.half;
+        const Planes own = planes(spins, R, colour);
+        const Planes oth = planes(spins, R, colour ^ 1);
         const uint32_t row = fdiv(q, R.div_hx);
         const uint32_t h = q - row * (uint32_t)R.hx;
         const uint32_t iz = fdiv(row, R.div_ly);
@@ -190,8 +211,8 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
         const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
         const uint32_t ix = 2u * h + p;
         const Nbr nb = neighbours(R, row, h, iy, iz, p);
-        const f3 s = ld3(own, q, n);
-        const f3 hf = local_field(L, oth, n, nb);
+        const f3 s = ld3(own, q);
+        const f3 hf = local_field(L, oth, nb);
 
         const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
         const u32x4 rnd = philox4x32_10(u32x4{ site, sweep, kStreamMetropolis, seed_hi }, seed_lo, (uint32_t)slot);
@@ -211,16 +232,39 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
             const float de = delta_e(s, sn, hf, L.Kz);
             const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
             if (uacc < w) {
-                own[q] = sn.x;
-                own[q + n] = sn.y;
-                own[q + 2 * n] = sn.z;
+                stf((float*)own.x, q, sn.x);
+                stf((float*)own.y, q, sn.y);
+                stf((float*)own.z, q, sn.z);
                 accepted = true;
             }
         }
     }
+    // Acceptance count: each wavefront owns one counter (stream-ordered
+    // launches never overlap, so a plain read-modify-write is race free).
     const unsigned long long ball = __ballot(accepted);
-    if ((threadIdx.x & (kWave - 1)) == 0 && ball)
-        atomicAdd(&acc_count[slot], (unsigned long long)__popcll(ball));
+    if ((threadIdx.x & (kWave - 1)) == 0 && ball) {
+        unsigned long long* c = acc_count + (size_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+        *c += (unsigned long long)__popcll(ball);
+    }
+}
+
+// Per-slot sum of the per-wavefront acceptance counters.
+__global__ __launch_bounds__(kBlock) void k_acc_finalize(
+    const Rep* __restrict__ reps, const unsigned long long* __restrict__ acc_wave, unsigned long long* __restrict__ acc)
+{
+    __shared__ unsigned long long lds[kBlock];
+    const Rep R = reps[blockIdx.x];
+    unsigned long long v = 0;
+    const int n = R.nblocks * (kBlock / kWave);
+    const unsigned long long* base = acc_wave + (size_t)R.first_block * (kBlock / kWave);
+    for (int j = threadIdx.x; j < n; j += kBlock) v += base[j];
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[blockIdx.x] = lds[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -269,17 +313,16 @@ __global__ __launch_bounds__(kBlock) void k_measure(
     const uint32_t q = (uint32_t)(b - R.first_block) * kBlock + threadIdx.x;
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
     if (q < R.half) {
-        const int64_t n = R.n;
         const uint32_t row = fdiv(q, R.div_hx);
         const uint32_t h = q - row * (uint32_t)R.hx;
         const uint32_t iz = fdiv(row, R.div_ly);
         const uint32_t iy = row - iz * (uint32_t)R.ly;
         for (int colour = 0; colour < 2; ++colour) {
-            const float* own = spins + R.off + (int64_t)colour * R.half;
-            const float* oth = spins + R.off + (int64_t)(colour ^ 1) * R.half;
+            const Planes own = planes(spins, R, colour);
+            const Planes oth = planes(spins, R, colour ^ 1);
             const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
             const Nbr nb = neighbours(R, row, h, iy, iz, p);
-            const f3 sf = ld3(own, q, n);
+            const f3 sf = ld3(own, q);
             const double sx = sf.x, sy = sf.y, sz = sf.z;
             double e = 0.0;
             const int plus[3] = { 0, 2, 4 };
@@ -287,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(
             for (int a = 0; a < 3; ++a) {
                 const int k = plus[a];
                 if (!nb.ok[k]) continue;
-                const f3 tf = ld3(oth, nb.i[k], n);
+                const f3 tf = ld3(oth, nb.i[k]);
                 const double tx = tf.x, ty = tf.y, tz = tf.z;
                 const double cx = sy * tz - sz * ty, cy = sz * tx - sx * tz, cz = sx * ty - sy * tx;
                 e += -L.Jd * (sx * tx + sy * ty + sz * tz)
@@ -336,11 +379,10 @@ __device__ __forceinline__ void renorm3(const f3& v, double o[3])
     o[0] = x / nn; o[1] = y / nn; o[2] = z / nn;
 }
 
-__device__ __noinline__ double triangle_sigma(const f3 af, const f3 bf, const f3 cf)
+// Solid angle of the triangle (a, b, c) of renormalised spins.
+__device__ __forceinline__ double triangle_sigma(const double a[3], const double b[3], const double c[3])
 {
 #pragma clang fp contract(off)
-    double a[3], b[3], c[3];
-    renorm3(af, a); renorm3(bf, b); renorm3(cf, c);
     const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
     const double bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
     const double ca = c[0] * a[0] + c[1] * a[1] + c[2] * a[2];
@@ -356,7 +398,7 @@ __device__ __forceinline__ f3 spin_at(const float* __restrict__ spins, const Rep
 {
     const uint32_t k = (ix + iy + iz) & 1u;
     const uint32_t idx = (iz * (uint32_t)R.ly + iy) * (uint32_t)R.hx + (ix >> 1);
-    return ld3(spins + R.off + (int64_t)k * R.half, idx, R.n);
+    return ld3(planes(spins, R, (int)k), idx);
 }
 
 __global__ __launch_bounds__(kBlock) void k_winding(
@@ -379,10 +421,11 @@ __global__ __launch_bounds__(kBlock) void k_winding(
         if (okx && oky) {
             const uint32_t i2 = (i + 1u == lx) ? 0u : i + 1u;
             const uint32_t j2 = (j + 1u == ly) ? 0u : j + 1u;
-            const f3 s1 = spin_at(spins, R, i, j, z);
-            const f3 s2 = spin_at(spins, R, i2, j, z);
-            const f3 s3 = spin_at(spins, R, i, j2, z);
-            const f3 s4 = spin_at(spins, R, i2, j2, z);
+            double s1[3], s2[3], s3[3], s4[3];
+            renorm3(spin_at(spins, R, i, j, z), s1);
+            renorm3(spin_at(spins, R, i2, j, z), s2);
+            renorm3(spin_at(spins, R, i, j2, z), s3);
+            renorm3(spin_at(spins, R, i2, j2, z), s4);
             v[0] = triangle_sigma(s1, s2, s4) + triangle_sigma(s1, s4, s3);
         }
     }
@@ -508,7 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_init(
     const uint32_t iz = fdiv(row, R.div_ly);
     const uint32_t iy = row - iz * (uint32_t)R.ly;
     for (int colour = 0; colour < 2; ++colour) {
-        float* own = spins + R.off + (int64_t)colour * R.half;
+        const Planes own = planes(spins, R, colour);
         const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
         const uint32_t ix = 2u * h + p;
         const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
@@ -526,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_init(
             x = 0.f; y = 0.f;
             z = ((double)(r.z >> 8) * 0x1p-24 < frac) ? -1.f : 1.f;
         }
-        own[q] = x; own[q + R.n] = y; own[q + 2 * R.n] = z;
+        stf((float*)own.x, q, x); stf((float*)own.y, q, y); stf((float*)own.z, q, z);
     }
 }
 
@@ -547,11 +590,11 @@ __global__ __launch_bounds__(kBlock) void k_delta_energy(
     const uint32_t row = iz * (uint32_t)R.ly + iy;
     const uint32_t h = ix >> 1;
     const uint32_t q = row * (uint32_t)R.hx + h;
-    const float* own = spins + R.off + (int64_t)colour * R.half;
-    const float* oth = spins + R.off + (int64_t)(colour ^ 1u) * R.half;
+    const Planes own = planes(spins, R, (int)colour);
+    const Planes oth = planes(spins, R, (int)(colour ^ 1u));
     const Nbr nb = neighbours(R, row, h, iy, iz, ix & 1u);
-    const f3 s = ld3(own, q, R.n);
-    const f3 hf = local_field(L, oth, R.n, nb);
+    const f3 s = ld3(own, q);
+    const f3 hf = local_field(L, oth, nb);
     const f3 sn{ props[3 * e], props[3 * e + 1], props[3 * e + 2] };
     out[e] = (double)delta_e(s, sn, hf, L.Kz);
 }
@@ -602,7 +645,8 @@ struct ising_ctx {
     double* d_qraw = nullptr;
     long long* d_qint = nullptr;
     uint32_t* d_ctr = nullptr;  // [0] sweep, [1] meas, [2] winding meas, [3] swap round
-    unsigned long long* d_acc = nullptr;
+    unsigned long long* d_acc = nullptr;       // [R] per-slot totals (filled on request)
+    unsigned long long* d_acc_wave = nullptr;  // [n_blocks * waves per block]
     unsigned long long* d_swap_att = nullptr;
     unsigned long long* d_swap_acc = nullptr;
     int32_t* d_sites = nullptr; float* d_props = nullptr; double* d_de = nullptr; int de_cap = 0;
@@ -713,7 +757,7 @@ int launch_half_sweeps(ising_ctx* c, uint32_t off, int colour)
 {
     hipLaunchKernelGGL(k_half_sweep, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
                        c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
-                       c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc);
+                       c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
@@ -839,7 +883,7 @@ void ising_destroy(ising_ctx* c)
     void* ptrs[] = { c->d_spins, c->d_reps, c->d_lads, c->d_lad_override, c->d_block_rep, c->d_qblock_rep,
                      c->d_beta_slot, c->d_beta_temp, c->d_slot_of_temp, c->d_lad_of_pos, c->d_lad_start,
                      c->d_lad_n, c->d_partial, c->d_energy, c->d_mag, c->d_qpartial, c->d_qraw, c->d_qint,
-                     c->d_ctr, c->d_acc, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de };
+                     c->d_ctr, c->d_acc, c->d_acc_wave, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de };
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -947,6 +991,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if ((rc = dalloc(c, &c->d_qint, (size_t)R * zmax))) return fail(rc);
     if ((rc = dalloc(c, &c->d_ctr, 4))) return fail(rc);
     if ((rc = dalloc(c, &c->d_acc, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_acc_wave, (size_t)blk * (kBlock / kWave)))) return fail(rc);
     if ((rc = dalloc(c, &c->d_swap_att, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_swap_acc, R))) return fail(rc);
 
@@ -980,6 +1025,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if (hipMemsetAsync(c->d_spins, 0, (size_t)c->total_floats * sizeof(float), s) != hipSuccess ||
         hipMemsetAsync(c->d_ctr, 0, 4 * sizeof(uint32_t), s) != hipSuccess ||
         hipMemsetAsync(c->d_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
+        hipMemsetAsync(c->d_acc_wave, 0, (size_t)blk * (kBlock / kWave) * sizeof(unsigned long long), s) != hipSuccess ||
         hipMemsetAsync(c->d_swap_att, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
         hipMemsetAsync(c->d_swap_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess)
         return fail(set_err(c, ISING_ERR_HIP, "hipMemsetAsync"));
@@ -1373,6 +1419,9 @@ int ising_accept_counts(ising_ctx* c, int64_t* accepted)
 {
     if (!c || !accepted) return ISING_ERR_INVALID_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_acc_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
+                       (const unsigned long long*)c->d_acc_wave, c->d_acc);
+    LAUNCH_CHECK(c);
     HIP_TRY(c, hipMemcpyAsync(accepted, c->d_acc, c->R * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return ISING_OK;
