@@ -76,6 +76,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-sweeps", type=int, default=0,
+                    help="only run N sweeps plus one 256 MiB copy probe (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the path shards by PT ladder with no data exchange, "
@@ -91,7 +93,18 @@ def main() -> int:
     eng.init_random()
     n_sites = L ** 3 * N_REP
 
-    # Warmup (also instantiates the graph once).
+    if args.profile_sweeps:
+        flags, _ = eng.autotune(5)
+        ms = eng.time_sweeps(args.profile_sweeps)
+        copy_ms = eng.bandwidth_probe(256 << 20)
+        print(json.dumps({"tuning": flags, "ms_per_launch": ms, "copy_probe_bytes": 2 * (256 << 20),
+                          "copy_probe_ms": copy_ms}), flush=True)
+        eng.close()
+        return 0
+
+    # Warmup: launch-tuning flags chosen by timing (real sweeps), then PT
+    # steps (also instantiates the graph once).
+    tune_flags, tune_ms = eng.autotune(10)
     if args.warmup > 0:
         eng.pt_run(args.warmup * SWAP_EVERY, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True)
 
@@ -116,6 +129,7 @@ def main() -> int:
     bytes_launch = BYTES_PER_UPDATE * n_sites / 2
     achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
     traffic = traffic_from_profiles()
+    copy_ms = eng.bandwidth_probe(256 << 20)
 
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
@@ -144,7 +158,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_half_sweep", "ms_per_launch": ms_launch,
-                     "algorithmic_bytes_per_launch": bytes_launch},
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "copy_probe_GBs": 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9,
+                     "tuning_flags": tune_flags, "tuning_ms_per_launch": [float(x) for x in tune_ms]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

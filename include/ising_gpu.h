@@ -173,6 +173,21 @@ int  ising_accept_counts(ising_ctx* ctx, int64_t* accepted);
 int  ising_time_sweeps(ising_ctx* ctx, int n_sweeps, double* ms_per_launch);
 /* hipDeviceSynchronize on the context's device. */
 int  ising_synchronize(ising_ctx* ctx);
+/* Plain device-to-device copy kernel over `bytes` bytes (two scratch buffers
+ * allocated and freed by the call); *ms = device time of one copy. Gives the
+ * achievable-bandwidth reference and the known-size kernel that scales HBM
+ * traffic counters. */
+int  ising_bandwidth_probe(ising_ctx* ctx, int64_t bytes, double* ms);
+
+/* ---- launch tuning (no reference counterpart) ------------------------- */
+#define ISING_TUNE_XCD_REMAP 1u  /* contiguous block ranges per accelerator die */
+#define ISING_TUNE_NT_OWN    2u  /* non-temporal access to the updated colour    */
+int  ising_set_tuning(ising_ctx* ctx, uint32_t flags);
+int  ising_get_tuning(const ising_ctx* ctx, uint32_t* flags);
+/* Times every flag combination over n_sweeps real sweeps each (they advance
+ * the chains like ising_sweep) and keeps the fastest; ms[4] (may be NULL)
+ * receives the time per half-sweep launch of flags 0..3. */
+int  ising_autotune(ising_ctx* ctx, int n_sweeps, uint32_t* best, double* ms);
 
 #ifdef __cplusplus
 }

@@ -85,6 +85,10 @@ SIGNATURES = {
     "ising_accept_counts": (C.c_int, [_P, _i64p]),
     "ising_time_sweeps": (C.c_int, [_P, C.c_int, _dp]),
     "ising_synchronize": (C.c_int, [_P]),
+    "ising_bandwidth_probe": (C.c_int, [_P, C.c_int64, _dp]),
+    "ising_set_tuning": (C.c_int, [_P, C.c_uint32]),
+    "ising_get_tuning": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "ising_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint32), _dp]),
 }
 
 _lib = None

@@ -63,7 +63,9 @@ def flang_available() -> bool:
 
 
 def build_fortran(verbose: bool = False) -> Path | None:
-    """Compile the Fortran drop-in modules and the PT driver (needs flang)."""
+    """Compile the Fortran drop-in modules into ``lib/libising_fortran.so`` and
+    link the PT driver ``lib/pt_gpu`` and the check program ``lib/test_dropin``
+    against it (needs flang)."""
     if not flang_available() or not FSRC.exists():
         return None
     LIB_DIR.mkdir(exist_ok=True)
@@ -72,20 +74,22 @@ def build_fortran(verbose: bool = False) -> Path | None:
     srcs = [FSRC / n for n in ("ising_gpu_c.f90", "ising_chainmesh.f90", "energymin_gpu.f90", "pt_utils_gpu.f90")]
     lib = LIB_DIR / "libising_fortran.so"
     tmp = LIB_DIR / "libising_fortran.so.tmp"
-    cmd = [FLANG, "-O2", "-fPIC", "-shared", f"-J{mod_dir}", *map(str, srcs), f"-L{LIB_DIR}", "-lising_hip",
-           f"-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    flags = ["-O2", "-fopenmp", f"-J{mod_dir}"]
+    cmd = [FLANG, *flags, "-fPIC", "-shared", *map(str, srcs), f"-L{LIB_DIR}", "-lising_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     _run(cmd)
     _atomic_replace(tmp, lib)
-    exe = LIB_DIR / "pt_gpu"
-    tmp = LIB_DIR / "pt_gpu.tmp"
-    cmd = [FLANG, "-O2", f"-J{mod_dir}", f"-I{mod_dir}", str(FSRC / "pt_gpu.f90"), f"-L{LIB_DIR}",
-           "-lising_fortran", "-lising_hip", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    _run(cmd)
-    _atomic_replace(tmp, exe)
+    for prog in ("pt_gpu", "test_dropin"):
+        exe = LIB_DIR / prog
+        tmp = LIB_DIR / f"{prog}.tmp"
+        cmd = [FLANG, *flags, f"-I{mod_dir}", str(FSRC / f"{prog}.f90"), f"-L{LIB_DIR}",
+               "-lising_fortran", "-lising_hip", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+        _atomic_replace(tmp, exe)
     return lib
 
 

@@ -33,6 +33,21 @@ namespace ising {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
+constexpr int kXcds = 8;          // MI355X: 8 accelerator dies, workgroups dealt round robin
+
+// Sweep tuning flags (bit mask, chosen by ising_autotune).
+constexpr uint32_t kTuneXcdRemap = 1u;     // give each die a contiguous range of blocks
+constexpr uint32_t kTuneNtOwn = 2u;        // non-temporal loads/stores of the updated colour
+
+// Logical workgroup index: with kTuneXcdRemap the blocks a die receives
+// (b mod 8 == die) map to one contiguous range, so neighbouring rows of a
+// lattice stay in that die's L2. A bijection on [0, n).
+__device__ __forceinline__ uint32_t logical_block(uint32_t b, uint32_t n, uint32_t tune)
+{
+    if (!(tune & kTuneXcdRemap)) return b;
+    const uint32_t die = b % kXcds, q = n / kXcds, r = n % kXcds;
+    return die * q + min(die, r) + b / kXcds;
+}
 
 // ---------------------------------------------------------------------------
 // Division by an invariant 32-bit divisor (Granlund & Montgomery, PLDI'94,
@@ -72,6 +87,7 @@ struct Rep {
 // Per ladder Hamiltonian, FP32 copy for sweeps and FP64 copy for energies.
 struct Lad {
     float J, Bz, Kz, r;
+    float zsf;              // Zeeman coefficient of total energies
     float d[9];
     int32_t dmi;
     double Jd, Bzd, Kzd, zs;
@@ -89,6 +105,15 @@ __device__ __forceinline__ float ldf(const float* __restrict__ base, uint32_t i)
 __device__ __forceinline__ void stf(float* __restrict__ base, uint32_t i, float v)
 {
     *(float*)((char*)base + (i << 2)) = v;
+}
+
+__device__ __forceinline__ float ldf_nt(const float* __restrict__ base, uint32_t i)
+{
+    return __builtin_nontemporal_load((const float*)((const char*)base + (i << 2)));
+}
+__device__ __forceinline__ void stf_nt(float* __restrict__ base, uint32_t i, float v)
+{
+    __builtin_nontemporal_store(v, (float*)((char*)base + (i << 2)));
 }
 
 // The three component planes (x, y, z) of one colour of one replica.
@@ -115,13 +140,15 @@ __device__ __forceinline__ Planes planes(const float* spins, const Rep& R, int c
 
 // Neighbour indices (inside the other colour's arrays) of the site at half
 // index h of row (iy, iz), parity p = ix & 1. ok[] false across open faces.
+// kPer: every axis periodic (wraps become selects, every neighbour exists).
 struct Nbr { uint32_t i[6]; bool ok[6]; };
 
+template <bool kPer>
 __device__ __forceinline__ Nbr neighbours(const Rep& R, uint32_t row, uint32_t h, uint32_t iy, uint32_t iz, uint32_t p)
 {
     Nbr nb;
     const uint32_t hx = (uint32_t)R.hx, ly = (uint32_t)R.ly, lz = (uint32_t)R.lz;
-    const bool px = R.per & 1, py = (R.per >> 1) & 1, pz = (R.per >> 2) & 1;
+    const bool px = kPer || (R.per & 1), py = kPer || ((R.per >> 1) & 1), pz = kPer || ((R.per >> 2) & 1);
     const uint32_t rb = row * hx;
     // +x: ix+1 has half index h+p unless it wraps (p=1, h=hx-1).
     const bool xwrap_p = (p == 1u) && (h == hx - 1u);
@@ -147,14 +174,15 @@ __device__ __forceinline__ Nbr neighbours(const Rep& R, uint32_t row, uint32_t h
 }
 
 // Local field h = J sum_j S_j - sum_a (S_{+a} - S_{-a}) x d_a + Bz z.
+template <bool kPer>
 __device__ __forceinline__ f3 local_field(const Lad& L, const Planes& oth, const Nbr& nb)
 {
-    const f3 xp = ld3c(oth, nb.i[0], nb.ok[0]);
-    const f3 xm = ld3c(oth, nb.i[1], nb.ok[1]);
-    const f3 yp = ld3c(oth, nb.i[2], nb.ok[2]);
-    const f3 ym = ld3c(oth, nb.i[3], nb.ok[3]);
-    const f3 zp = ld3c(oth, nb.i[4], nb.ok[4]);
-    const f3 zm = ld3c(oth, nb.i[5], nb.ok[5]);
+    const f3 xp = kPer ? ld3(oth, nb.i[0]) : ld3c(oth, nb.i[0], nb.ok[0]);
+    const f3 xm = kPer ? ld3(oth, nb.i[1]) : ld3c(oth, nb.i[1], nb.ok[1]);
+    const f3 yp = kPer ? ld3(oth, nb.i[2]) : ld3c(oth, nb.i[2], nb.ok[2]);
+    const f3 ym = kPer ? ld3(oth, nb.i[3]) : ld3c(oth, nb.i[3], nb.ok[3]);
+    const f3 zp = kPer ? ld3(oth, nb.i[4]) : ld3c(oth, nb.i[4], nb.ok[4]);
+    const f3 zm = kPer ? ld3(oth, nb.i[5]) : ld3c(oth, nb.i[5], nb.ok[5]);
     const f3 sum{ (xp.x + xm.x) + (yp.x + ym.x) + (zp.x + zm.x),
                   (xp.y + xm.y) + (yp.y + ym.y) + (zp.y + zm.y),
                   (xp.z + xm.z) + (yp.z + ym.z) + (zp.z + zm.z) };
@@ -174,6 +202,13 @@ __device__ __forceinline__ f3 local_field(const Lad& L, const Planes& oth, const
     return f3{ L.J * sum.x - dm.x, L.J * sum.y - dm.y, L.J * sum.z - dm.z + L.Bz };
 }
 
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
 // dE = -(S' - S).h - K (S'_z - S_z)(S'_z + S_z): the difference is formed
 // first so small moves keep their relative accuracy.
 __device__ __forceinline__ float delta_e(const f3& s, const f3& sn, const f3& h, float K)
@@ -187,20 +222,31 @@ __device__ __forceinline__ float delta_e(const f3& s, const f3& sn, const f3& h,
 // replica. Proposal: u uniform in a ball of radius r (cos(theta) = 2U-1,
 // phi = 2 pi U', radius r U''^(1/3)), S' = (S+u)/|S+u|; accept iff
 // U''' < exp(-beta dE). One Philox block supplies the four uniforms.
+//
+// kMeasure (colour-1 launches only): after its update every colour-1 site adds
+// its six bonds, -S.(h - Bz z) with the field h already formed (each bond has
+// exactly one colour-1 end), plus the on-site terms and spins of itself and of
+// the colour-0 site at the same half index; per-site FP32, summed in FP64 per
+// wavefront into wave_part[wave][4] = (E, Mx, My, Mz).
+template <bool kMeasure>
 __global__ __launch_bounds__(kBlock) void k_half_sweep(
     float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ block_rep,
     const uint32_t* __restrict__ sweep_ctr, uint32_t sweep_off, int colour,
-    uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count)
+    uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count,
+    double* __restrict__ wave_part, uint32_t tune)
 {
-    const int slot = block_rep[blockIdx.x];
+    const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
+    const int slot = block_rep[blk];
     const Rep R = reps[slot];
     const Lad L = lads[R.ladder];
     const float beta = beta_slot[slot];
     const uint32_t sweep = *sweep_ctr + sweep_off;
-    const uint32_t q = (uint32_t)(blockIdx.x - R.first_block) * kBlock + threadIdx.x;
+    const uint32_t q = (blk - (uint32_t)R.first_block) * kBlock + threadIdx.x;
     const bool active = q < R.half;
+    const bool nt = tune & kTuneNtOwn;
     bool accepted = false;
+    float me = 0.f, mx = 0.f, my = 0.f, mz = 0.f;   // measurement of this lane
     if (active) {
         const Planes own = planes(spins, R, colour);
         const Planes oth = planes(spins, R, colour ^ 1);
@@ -210,9 +256,14 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
         const uint32_t iy = row - iz * (uint32_t)R.ly;
         const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
         const uint32_t ix = 2u * h + p;
-        const Nbr nb = neighbours(R, row, h, iy, iz, p);
-        const f3 s = ld3(own, q);
-        const f3 hf = local_field(L, oth, nb);
+        const f3 s = nt ? f3{ ldf_nt(own.x, q), ldf_nt(own.y, q), ldf_nt(own.z, q) } : ld3(own, q);
+        f3 fs = s;                 // spin after the decision
+        f3 hf;
+        if (R.per == 7) {
+            hf = local_field<true>(L, oth, neighbours<true>(R, row, h, iy, iz, p));
+        } else {
+            hf = local_field<false>(L, oth, neighbours<false>(R, row, h, iy, iz, p));
+        }
 
         const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
         const u32x4 rnd = philox4x32_10(u32x4{ site, sweep, kStreamMetropolis, seed_hi }, seed_lo, (uint32_t)slot);
@@ -232,18 +283,40 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
             const float de = delta_e(s, sn, hf, L.Kz);
             const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
             if (uacc < w) {
-                stf((float*)own.x, q, sn.x);
-                stf((float*)own.y, q, sn.y);
-                stf((float*)own.z, q, sn.z);
+                if (nt) {
+                    stf_nt((float*)own.x, q, sn.x);
+                    stf_nt((float*)own.y, q, sn.y);
+                    stf_nt((float*)own.z, q, sn.z);
+                } else {
+                    stf((float*)own.x, q, sn.x);
+                    stf((float*)own.y, q, sn.y);
+                    stf((float*)own.z, q, sn.z);
+                }
                 accepted = true;
+                fs = sn;
             }
+        }
+        if (kMeasure) {
+            const f3 c0 = ld3(planes(spins, R, 0), q);
+            me = -(fs.x * hf.x + fs.y * hf.y + fs.z * (hf.z - L.Bz))
+                 - L.zsf * L.Bz * (fs.z + c0.z) - L.Kz * (fs.z * fs.z + c0.z * c0.z);
+            mx = fs.x + c0.x; my = fs.y + c0.y; mz = fs.z + c0.z;
+        }
+    }
+    if (kMeasure) {
+        double v[4] = { me, mx, my, mz };
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            double* o = wave_part + 4 * ((size_t)blk * (kBlock / kWave) + threadIdx.x / kWave);
+            o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
         }
     }
     // Acceptance count: each wavefront owns one counter (stream-ordered
     // launches never overlap, so a plain read-modify-write is race free).
     const unsigned long long ball = __ballot(accepted);
     if ((threadIdx.x & (kWave - 1)) == 0 && ball) {
-        unsigned long long* c = acc_count + (size_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+        unsigned long long* c = acc_count + (size_t)blk * (kBlock / kWave) + threadIdx.x / kWave;
         *c += (unsigned long long)__popcll(ball);
     }
 }
@@ -270,12 +343,6 @@ __global__ __launch_bounds__(kBlock) void k_acc_finalize(
 // ---------------------------------------------------------------------------
 // FP64 reductions.
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
 
 // Sum of 4 values over a workgroup of kBlock threads; result valid in thread 0.
 __device__ __forceinline__ void block_sum4(double v[4], double* lds /* [4][kBlock/kWave] */)
@@ -321,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(
             const Planes own = planes(spins, R, colour);
             const Planes oth = planes(spins, R, colour ^ 1);
             const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
-            const Nbr nb = neighbours(R, row, h, iy, iz, p);
+            const Nbr nb = neighbours<false>(R, row, h, iy, iz, p);
             const f3 sf = ld3(own, q);
             const double sx = sf.x, sy = sf.y, sz = sf.z;
             double e = 0.0;
@@ -344,6 +411,29 @@ __global__ __launch_bounds__(kBlock) void k_measure(
     if (threadIdx.x == 0) {
         double* o = partial + 4 * (int64_t)b;
         o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    }
+}
+
+// One workgroup per slot: fixed-order FP64 sum of the per-wavefront
+// measurement partials of the fused colour-1 half-sweep (deterministic).
+__global__ __launch_bounds__(kBlock) void k_wave_finalize(
+    const Rep* __restrict__ reps, const double* __restrict__ wave_part,
+    double* __restrict__ energy, double* __restrict__ mag)
+{
+    __shared__ double lds[4 * (kBlock / kWave)];
+    const int slot = blockIdx.x;
+    const Rep R = reps[slot];
+    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+    const int nw = R.nblocks * (kBlock / kWave);
+    const double* base = wave_part + 4 * (size_t)R.first_block * (kBlock / kWave);
+    for (int j = threadIdx.x; j < nw; j += kBlock) {
+        v[0] += base[4 * j]; v[1] += base[4 * j + 1]; v[2] += base[4 * j + 2]; v[3] += base[4 * j + 3];
+    }
+    block_sum4(v, lds);
+    if (threadIdx.x == 0) {
+        energy[slot] = v[0];
+        const double inv = 1.0 / (double)R.n;
+        mag[3 * slot] = v[1] * inv; mag[3 * slot + 1] = v[2] * inv; mag[3 * slot + 2] = v[3] * inv;
     }
 }
 
@@ -529,6 +619,13 @@ __global__ __launch_bounds__(kBlock) void k_record_q(
     if (threadIdx.x == 0) *q_ctr = m + 1u;
 }
 
+// Plain copy (16-byte vectors, grid-stride): the bandwidth and traffic probe.
+__global__ __launch_bounds__(kBlock) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        dst[i] = src[i];
+}
+
 __global__ void k_add_u32(uint32_t* __restrict__ p, uint32_t v) { if (threadIdx.x == 0) *p += v; }
 __global__ void k_set_u32(uint32_t* __restrict__ p, int n, uint32_t v)
 {
@@ -592,9 +689,9 @@ __global__ __launch_bounds__(kBlock) void k_delta_energy(
     const uint32_t q = row * (uint32_t)R.hx + h;
     const Planes own = planes(spins, R, (int)colour);
     const Planes oth = planes(spins, R, (int)(colour ^ 1u));
-    const Nbr nb = neighbours(R, row, h, iy, iz, ix & 1u);
+    const Nbr nb = neighbours<false>(R, row, h, iy, iz, ix & 1u);
     const f3 s = ld3(own, q);
-    const f3 hf = local_field(L, oth, nb);
+    const f3 hf = local_field<false>(L, oth, nb);
     const f3 sn{ props[3 * e], props[3 * e + 1], props[3 * e + 2] };
     out[e] = (double)delta_e(s, sn, hf, L.Kz);
 }
@@ -625,6 +722,7 @@ struct ising_ctx {
     std::vector<int32_t> block_rep, qblock_rep;
     int64_t total_floats = 0;
     uint32_t host_sweep = 0;
+    uint32_t tune = 0;           // sweep tuning flags (kTune*)
 
     float* d_spins = nullptr;
     Rep* d_reps = nullptr;
@@ -647,6 +745,7 @@ struct ising_ctx {
     uint32_t* d_ctr = nullptr;  // [0] sweep, [1] meas, [2] winding meas, [3] swap round
     unsigned long long* d_acc = nullptr;       // [R] per-slot totals (filled on request)
     unsigned long long* d_acc_wave = nullptr;  // [n_blocks * waves per block]
+    double* d_wave_part = nullptr;             // [n_blocks * waves per block][4]
     unsigned long long* d_swap_att = nullptr;
     unsigned long long* d_swap_acc = nullptr;
     int32_t* d_sites = nullptr; float* d_props = nullptr; double* d_de = nullptr; int de_cap = 0;
@@ -704,6 +803,7 @@ Lad make_lad(const ising_hamiltonian& h, double cone)
     if (h.dmi_type == ISING_DMI_BULK) { d[0] = h.D; d[4] = h.D; d[8] = h.D; }
     else if (h.dmi_type == ISING_DMI_INTERFACIAL_REF) { d[1] = -h.D; d[3] = h.D; }
     L.J = (float)h.J; L.Bz = (float)h.Bz; L.Kz = (float)h.Kz; L.r = (float)cone;
+    L.zsf = (float)h.zeeman_total_scale;
     for (int k = 0; k < 9; ++k) { L.d[k] = (float)d[k]; L.dd[k] = d[k]; }
     L.dmi = h.dmi_type;
     L.Jd = h.J; L.Bzd = h.Bz; L.Kzd = h.Kz; L.zs = h.zeeman_total_scale;
@@ -753,11 +853,22 @@ int check_replica(ising_ctx* c, int r)
 
 // ---- launch helpers --------------------------------------------------------
 
-int launch_half_sweeps(ising_ctx* c, uint32_t off, int colour)
+int launch_half_sweeps(ising_ctx* c, uint32_t off, int colour, bool measure = false)
 {
-    hipLaunchKernelGGL(k_half_sweep, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
-                       c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
-                       c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave);
+    if (measure) {
+        hipLaunchKernelGGL(k_half_sweep<true>, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
+                           c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
+                           c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
+                           c->d_wave_part, c->tune);
+        LAUNCH_CHECK(c);
+        hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
+                           (const double*)c->d_wave_part, c->d_energy, c->d_mag);
+    } else {
+        hipLaunchKernelGGL(k_half_sweep<false>, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
+                           c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
+                           c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
+                           (double*)nullptr, c->tune);
+    }
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
@@ -814,13 +925,16 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
 {
     int rc;
     for (int k = 0; k < count; ++k) {
-        if ((rc = launch_half_sweeps(c, (uint32_t)k, 0))) return rc;
-        if ((rc = launch_half_sweeps(c, (uint32_t)k, 1))) return rc;
         const int done = k + 1;
         const bool do_swap = s.swap_every > 0 && done % s.swap_every == 0;
         const bool do_meas = s.measure_every > 0 && done % s.measure_every == 0;
         const bool do_q = s.winding_every > 0 && done % s.winding_every == 0;
-        if (do_swap || do_meas)
+        // Measurements ride on the colour-1 launch, except before a swap, whose
+        // exponent takes the per-site FP64 energies of the measure kernel.
+        const bool fused = do_meas && !do_swap;
+        if ((rc = launch_half_sweeps(c, (uint32_t)k, 0))) return rc;
+        if ((rc = launch_half_sweeps(c, (uint32_t)k, 1, fused))) return rc;
+        if (do_swap)
             if ((rc = launch_measure_all(c))) return rc;
         if (do_swap)
             if ((rc = launch_swap(c))) return rc;
@@ -878,15 +992,15 @@ int ising_replica_sites(const ising_ctx* ctx, int replica, int64_t* n)
 void ising_destroy(ising_ctx* c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = { c->d_spins, c->d_reps, c->d_lads, c->d_lad_override, c->d_block_rep, c->d_qblock_rep,
                      c->d_beta_slot, c->d_beta_temp, c->d_slot_of_temp, c->d_lad_of_pos, c->d_lad_start,
                      c->d_lad_n, c->d_partial, c->d_energy, c->d_mag, c->d_qpartial, c->d_qraw, c->d_qint,
-                     c->d_ctr, c->d_acc, c->d_acc_wave, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de };
+                     c->d_ctr, c->d_acc, c->d_acc_wave, c->d_wave_part, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de };
     for (void* p : ptrs)
-        if (p) hipFree(p);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -992,6 +1106,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if ((rc = dalloc(c, &c->d_ctr, 4))) return fail(rc);
     if ((rc = dalloc(c, &c->d_acc, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_acc_wave, (size_t)blk * (kBlock / kWave)))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_wave_part, 4 * (size_t)blk * (kBlock / kWave)))) return fail(rc);
     if ((rc = dalloc(c, &c->d_swap_att, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_swap_acc, R))) return fail(rc);
 
@@ -1258,9 +1373,9 @@ int ising_delta_energy(ising_ctx* c, int replica, const int32_t* sites, const do
     }
     HIP_TRY(c, hipSetDevice(c->device));
     if (n > c->de_cap) {
-        if (c->d_sites) hipFree(c->d_sites);
-        if (c->d_props) hipFree(c->d_props);
-        if (c->d_de) hipFree(c->d_de);
+        if (c->d_sites) (void)hipFree(c->d_sites);
+        if (c->d_props) (void)hipFree(c->d_props);
+        if (c->d_de) (void)hipFree(c->d_de);
         c->d_sites = nullptr; c->d_props = nullptr; c->d_de = nullptr; c->de_cap = 0;
         if ((rc = dalloc(c, &c->d_sites, n)) || (rc = dalloc(c, &c->d_props, 3 * (size_t)n)) || (rc = dalloc(c, &c->d_de, n)))
             return rc;
@@ -1290,10 +1405,10 @@ int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* 
     sd.n_q = s.winding_every ? (uint32_t)(s.n_sweeps / s.winding_every) : 0u;
     const size_t R = (size_t)c->R, Z = (size_t)c->zmax;
     auto cleanup = [&]() {
-        if (sd.e) hipFree(sd.e);
-        if (sd.m) hipFree(sd.m);
-        if (sd.qr) hipFree(sd.qr);
-        if (sd.qi) hipFree(sd.qi);
+        if (sd.e) (void)hipFree(sd.e);
+        if (sd.m) (void)hipFree(sd.m);
+        if (sd.qr) (void)hipFree(sd.qr);
+        if (sd.qi) (void)hipFree(sd.qi);
     };
     if (sd.n_meas) {
         if ((rc = dalloc(c, &sd.e, sd.n_meas * R)) || (rc = dalloc(c, &sd.m, sd.n_meas * R * 3))) { cleanup(); return rc; }
@@ -1319,22 +1434,22 @@ int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* 
         }
         rc = enqueue_period(c, s, (int)period, sd);
         hipError_t ec = hipStreamEndCapture(c->stream, &g);
-        if (rc || ec != hipSuccess) { if (g) hipGraphDestroy(g); cleanup(); return rc ? rc : set_err(c, ISING_ERR_HIP, "capture"); }
+        if (rc || ec != hipSuccess) { if (g) (void)hipGraphDestroy(g); cleanup(); return rc ? rc : set_err(c, ISING_ERR_HIP, "capture"); }
         if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
-            hipGraphDestroy(g); cleanup();
+            (void)hipGraphDestroy(g); cleanup();
             return set_err(c, ISING_ERR_HIP, "hipGraphInstantiate");
         }
         const int reps = s.n_sweeps / (int)period;
         for (int k = 0; k < reps; ++k) {
             if (hipGraphLaunch(ge, c->stream) != hipSuccess) {
-                hipGraphExecDestroy(ge); hipGraphDestroy(g); cleanup();
+                (void)hipGraphExecDestroy(ge); (void)hipGraphDestroy(g); cleanup();
                 return set_err(c, ISING_ERR_HIP, "hipGraphLaunch");
             }
         }
         done = reps * (int)period;
         hipError_t es = hipStreamSynchronize(c->stream);
-        hipGraphExecDestroy(ge);
-        hipGraphDestroy(g);
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
         if (es != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "graph replay: %s", hipGetErrorString(es)); }
     }
     // Remaining sweeps eagerly, period by period (records keep their order).
@@ -1387,7 +1502,7 @@ int ising_time_sweeps(ising_ctx* c, int n_sweeps, double* ms_per_launch)
     HIP_TRY(c, hipSetDevice(c->device));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(c, hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
     int rc = ISING_OK;
     if (hipEventRecord(e0, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
     for (int k = 0; k < n_sweeps && !rc; ++k) {
@@ -1399,8 +1514,8 @@ int ising_time_sweeps(ising_ctx* c, int n_sweeps, double* ms_per_launch)
     float ms = 0.f;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "sync");
     if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventElapsedTime");
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     c->host_sweep += (uint32_t)n_sweeps;
     *ms_per_launch = (double)ms / (2.0 * n_sweeps);
@@ -1412,6 +1527,68 @@ int ising_synchronize(ising_ctx* c)
     if (!c) return ISING_ERR_INVALID_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipDeviceSynchronize());
+    return ISING_OK;
+}
+
+int ising_bandwidth_probe(ising_ctx* c, int64_t bytes, double* ms)
+{
+    if (!c || !ms || bytes < 16) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t n = (size_t)bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &a, n))) return rc;
+    if ((rc = dalloc(c, &b, n))) { (void)hipFree(a); return rc; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = ISING_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventCreate");
+    const int grid = 256 * 16;
+    if (!rc && hipMemsetAsync(a, 0, n * sizeof(float4), c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "memset");
+    if (!rc) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(kBlock), 0, c->stream, (const float4*)a, b, n);
+    if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
+    if (!rc) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(kBlock), 0, c->stream, (const float4*)b, a, n);
+    if (!rc && hipEventRecord(e1, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
+    float t = 0.f;
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "sync");
+    if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "elapsed");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (rc) return rc;
+    *ms = t;
+    return ISING_OK;
+}
+
+int ising_set_tuning(ising_ctx* c, uint32_t flags)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (flags & ~(kTuneXcdRemap | kTuneNtOwn)) return set_err(c, ISING_ERR_INVALID_ARG, "unknown tuning flags %u", flags);
+    c->tune = flags;
+    return ISING_OK;
+}
+
+int ising_get_tuning(const ising_ctx* c, uint32_t* flags)
+{
+    if (!c || !flags) return ISING_ERR_INVALID_ARG;
+    *flags = c->tune;
+    return ISING_OK;
+}
+
+int ising_autotune(ising_ctx* c, int n_sweeps, uint32_t* best, double* ms)
+{
+    if (!c || n_sweeps < 1) return ISING_ERR_INVALID_ARG;
+    double t[4];
+    uint32_t arg = 0;
+    for (uint32_t f = 0; f < 4; ++f) {
+        c->tune = f;
+        int rc = ising_time_sweeps(c, n_sweeps, &t[f]);
+        if (rc) return rc;
+        if (t[f] < t[arg]) arg = f;
+    }
+    c->tune = arg;
+    if (best) *best = arg;
+    if (ms) for (int f = 0; f < 4; ++f) ms[f] = t[f];
     return ISING_OK;
 }
 

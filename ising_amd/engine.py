@@ -280,6 +280,29 @@ class Engine:
         self._check(self._lib.ising_time_sweeps(self._h, int(n_sweeps), C.byref(ms)))
         return ms.value
 
+    def bandwidth_probe(self, nbytes: int = 256 << 20) -> float:
+        """Device time (ms) of one plain copy of nbytes (read nbytes + write nbytes)."""
+        ms = C.c_double()
+        self._check(self._lib.ising_bandwidth_probe(self._h, int(nbytes), C.byref(ms)))
+        return ms.value
+
+    def set_tuning(self, flags: int) -> None:
+        self._check(self._lib.ising_set_tuning(self._h, int(flags)))
+
+    def tuning(self) -> int:
+        f = C.c_uint32()
+        self._check(self._lib.ising_get_tuning(self._h, C.byref(f)))
+        return f.value
+
+    def autotune(self, n_sweeps: int = 20) -> tuple[int, np.ndarray]:
+        """Times each sweep-tuning flag set over n_sweeps real sweeps and keeps
+        the fastest; returns (flags, ms per half-sweep launch of flags 0..3)."""
+        best = C.c_uint32()
+        ms = np.zeros(4)
+        self._check(self._lib.ising_autotune(self._h, int(n_sweeps), C.byref(best),
+                                             ms.ctypes.data_as(C.POINTER(C.c_double))))
+        return best.value, ms
+
     def synchronize(self) -> None:
         self._check(self._lib.ising_synchronize(self._h))
 

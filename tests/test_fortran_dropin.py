@@ -1,0 +1,95 @@
+"""The Fortran drop-in layer (ising_amd/fortran): the reference's entry points
+called from Fortran through iso_c_binding.
+
+CPU tests: the library and programs are built and stop with the library's
+error text when no device is present. GPU tests: the check program's values
+against closed forms / the oracle's identities, and the batched PT driver
+with the reference's command line and CSV format."""
+import csv
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+LIB = Path(__file__).resolve().parents[1] / "ising_amd" / "lib"
+needs_fortran = pytest.mark.skipif(not (LIB / "test_dropin").exists(), reason="Fortran layer not built (no flang)")
+
+
+def run(cmd, env=None, timeout=120):
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    return subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=e)
+
+
+@needs_fortran
+def test_driver_rejects_bad_cli():
+    p = run([str(LIB / "pt_gpu"), "1", "2"])
+    assert p.returncode != 0
+    assert "Jmin Jmax NJ Dmin Dmax ND Bmin Bmax NB outputFilePath" in (p.stdout + p.stderr)
+
+
+@needs_fortran
+def test_dropin_reports_missing_device_or_runs():
+    import ctypes as C
+    from ising_amd import _capi
+    n = C.c_int()
+    _capi.load().ising_device_count(C.byref(n))
+    if n.value > 0:
+        pytest.skip("device present: covered by the gpu tests")
+    p = run([str(LIB / "test_dropin")])
+    assert p.returncode != 0
+    assert "no HIP device" in (p.stdout + p.stderr)
+
+
+def parse(out):
+    vals = {}
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) >= 2:
+            try:
+                vals[parts[0]] = [float(x) for x in parts[1:]]
+            except ValueError:
+                pass
+    return vals
+
+
+@pytest.mark.gpu
+@needs_fortran
+def test_dropin_entry_points_on_gpu():
+    p = run([str(LIB / "test_dropin")])
+    assert p.returncode == 0, p.stdout + p.stderr
+    v = parse(p.stdout)
+    assert "done" in p.stdout
+    # spins are stored in FP32 on the device: the closed form holds to FP32 rounding
+    assert abs(v["spiral_energy"][0] - v["spiral_expect"][0]) < 2e-7 * abs(v["spiral_expect"][0])
+    assert abs(v["local_delta"][0] - v["total_delta"][0]) < 1e-5 * (1 + abs(v["total_delta"][0]))
+    assert v["max_norm_dev"][0] < 1e-6
+    assert v["energy_after"][0] < v["spiral_energy"][0] + 200        # sweeps ran and stayed finite
+    assert np.allclose(v["fm_mag"], [0, 0, 1]) and v["fm_winding"][0] == 0.0
+    assert abs(abs(v["sk_winding_middle"][0]) - 1) < 1e-9 and v["sk_number"][0] == 1
+    assert abs(v["sk_layer1"][0] - v["sk_winding_middle"][0]) < 1e-12
+
+
+@pytest.mark.gpu
+@needs_fortran
+def test_pt_driver_reference_cli_and_csv(tmp_path):
+    out = tmp_path / "run"
+    env = {"ISING_PT_ITERATIONS": "4", "ISING_PT_SWEEPS_PER_SWAP": "10", "ISING_PT_RELAX": "20"}
+    p = run([str(LIB / "pt_gpu"), "-1.0", "1.0", "2", "0.5", "0.5", "1", "0.1", "0.1", "1", str(out)], env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    files = sorted(out.glob("data_*.csv"))
+    assert len(files) == 2 * 10                      # 2 parameter slots x 10 temperatures
+    rows = list(csv.reader(files[0].open()))
+    assert rows[0] == ["J", "D", "B", "T", "winding_number_middle", "skyrmion_number_middle",
+                       "winding_number_spread", "sx", "sy", "sz"]
+    vals = [float(x) for x in rows[1]]
+    assert vals[0] in (-1.0, 0.0) and vals[1] == 0.5 and vals[2] == 0.1
+    assert 400.0 <= vals[3] <= 20000.0 and abs(vals[7] ** 2 + vals[8] ** 2 + vals[9] ** 2) <= 1.0001
+    spins = sorted((tmp_path / "run_spins").glob("spins_*.csv"))
+    assert len(spins) == 20
+    data = np.loadtxt(spins[0], delimiter=",", skiprows=1)
+    assert data.shape == (30 * 30 * 6, 6)
+    assert np.allclose(np.linalg.norm(data[:, 3:], axis=1), 1.0, atol=1e-6)

@@ -300,3 +300,22 @@ def test_errors_are_reported():
         assert "BAD_SPIN" in str(ei.value)
         with pytest.raises(Exception):
             e.winding(0, 99)
+
+
+@pytest.mark.parametrize("lname", sorted(LATS))
+@pytest.mark.parametrize("hname", ["bulk_K", "interfacial_ref"])
+def test_fused_measurement_matches_exact(lname, hname):
+    """E and M recorded by the colour-1 launch (per-site FP32, FP64 sums)
+    against the FP64 measure kernel on the same configuration:
+    relative 1e-6 for E (FP32 per-site terms), 1e-6 absolute for M."""
+    lat, ham = LATS[lname], HAMS[hname]
+    with engine(lat, ham, [1.5, 0.6, 0.3]) as e:
+        e.init_random()
+        for _ in range(3):
+            o = e.pt_run(n_sweeps=2, swap_every=0, measure_every=2, winding_every=0)
+            sot = e.temperature_map()
+            exact = e.energy()[sot]
+            mag = e.magnetisation()[sot]
+            scale = np.maximum(np.abs(exact), lat.sites)
+            assert np.all(np.abs(o["energy"][-1] - exact) <= 1e-6 * scale)
+            assert np.allclose(o["magnetisation"][-1], mag, atol=1e-6)
