@@ -4,13 +4,18 @@ Workload (one GPU): 64^3 periodic simple-cubic lattice, 64 replicas on a
 geometric temperature ladder 0.2..2.0 J, J = 1, bulk D = 0.5 J, Bz = 0.2 J,
 fixed cone radius 0.5, seeded random-sphere start (synthetic data).
 One step = 10 exact checkerboard sweeps of all 64 replicas followed by one
-replica-exchange round; E and M are recorded every 5 sweeps and the per-layer
+replica-exchange round; E and M are recorded every 2 sweeps and the per-layer
 topological charge every 10, all on device inside one replayed hipGraph.
 
-value  = site updates/s (every replica's sites x sweeps / wall time),
-also reported: tau_int-corrected independent samples/s of E and of Q
-(middle layer), the half-sweep kernel's HBM roofline fraction from hipEvent
-timing, and the FP64 CPU oracle's rate on this host's cores.
+Before the timed runs (untimed): 2000 plain sweeps equilibrate and collect
+training records, and the cone-width network is trained on them. Then the
+same schedule is timed twice: fixed cone r = 0.5, and learned cone.
+
+value  = site updates/s of the fixed-cone run (every replica's sites x sweeps
+/ wall time); also reported per sampler: tau_int-corrected independent
+samples/s of E and of Q (middle layer) and their learned/fixed ratio, the
+half-sweep kernel's HBM roofline fraction from hipEvent timing, and the FP64
+CPU oracle's rate on this host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -32,8 +37,15 @@ L = 64
 N_REP = 64
 T_MIN, T_MAX = 0.2, 2.0
 SWAP_EVERY = 10
-MEASURE_EVERY = 5
+MEASURE_EVERY = 2
 WINDING_EVERY = 10
+TRAIN_SWEEPS = 2000          # plain sweeps that generate the training records
+COLLECT_EVERY = 20
+RECORDS_PER_REPLICA = 2048
+TRAIN_STEPS = 1000
+TRAIN_BATCH = 16384
+TRAIN_LR = 3e-3
+MLP_FLOPS_PER_UPDATE = 2 * (21 * 32 + 32 * 32 + 32)
 CONE = 0.5
 SEED = 20261203
 BYTES_PER_UPDATE = 24  # FP32 (x,y,z) read once + written once per site update
@@ -86,6 +98,7 @@ def main() -> int:
 
     from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
     from ising_amd.analysis import independent_samples
+    from ising_amd.learned import generate_data, train
 
     T = geometric_temperatures(T_MIN, T_MAX, N_REP)
     ham = Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2, Kz=0.0)
@@ -102,65 +115,94 @@ def main() -> int:
         eng.close()
         return 0
 
-    # Warmup: launch-tuning flags chosen by timing (real sweeps), then PT
-    # steps (also instantiates the graph once).
+    # Untimed preparation: launch-tuning flags chosen by timing (real sweeps);
+    # 2000 plain sweeps that equilibrate and collect training records; Adam
+    # training of the cone-width network on them.
     tune_flags, tune_ms = eng.autotune(10)
-    if args.warmup > 0:
-        eng.pt_run(args.warmup * SWAP_EVERY, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True)
+    t0 = time.perf_counter()
+    n_records = generate_data(eng, sweeps=TRAIN_SWEEPS, collect_every=COLLECT_EVERY, per_replica=RECORDS_PER_REPLICA,
+                              swap_every=SWAP_EVERY)
+    loss = train(eng, steps=TRAIN_STEPS, batch=TRAIN_BATCH, lr=TRAIN_LR)
+    prep_s = time.perf_counter() - t0
 
     n_sweeps = args.steps * SWAP_EVERY
-    # Device-wide synchronisation around the timed region goes through the
-    # library (torch is not imported: it bundles its own HIP runtime).
-    eng.synchronize()
-    t0 = time.perf_counter()
-    out = eng.pt_run(n_sweeps, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True)
-    eng.synchronize()
-    wall = time.perf_counter() - t0
 
-    updates_per_s = n_sites * n_sweeps / wall
-    e_series = out["energy"]                                  # [n_meas, R]
-    q_series = out["winding_raw"][:, :, L // 2]               # middle layer, [n_q, R]
-    n_e, tau_e = independent_samples(e_series)
-    n_q, tau_q = independent_samples(q_series)
+    def timed_run(learned: bool) -> dict:
+        eng.mlp_enable(learned)
+        if args.warmup > 0:
+            eng.pt_run(args.warmup * SWAP_EVERY, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True)
+        # Device-wide synchronisation around the timed region goes through the
+        # library (torch is not imported: it bundles its own HIP runtime).
+        eng.synchronize()
+        t0 = time.perf_counter()
+        out = eng.pt_run(n_sweeps, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True)
+        eng.synchronize()
+        wall = time.perf_counter() - t0
+        e_series = out["energy"]                                  # [n_meas, R]
+        q_series = out["winding_raw"][:, :, L // 2]               # middle layer, [n_q, R]
+        n_e, tau_e = independent_samples(e_series)
+        n_q, tau_q = independent_samples(q_series)
+        ms_launch = eng.time_sweeps(50)
+        return {"wall": wall, "updates_per_s": n_sites * n_sweeps / wall, "ms_per_launch": ms_launch,
+                "samples_per_s_E": n_e / wall, "samples_per_s_Q": n_q / wall,
+                "tau_int_E_sweeps": (tau_e * MEASURE_EVERY).tolist(), "tau_int_Q_sweeps": (tau_q * WINDING_EVERY).tolist(),
+                "mean_E_per_site": (e_series.mean(axis=0) / L ** 3).tolist(),
+                "mean_absM": np.linalg.norm(out["magnetisation"], axis=2).mean(axis=0).tolist()}
+
+    fixed = timed_run(False)
+    learned = timed_run(True)
     att, acc = eng.swap_stats()
 
-    # Kernel timing: hipEvents around back-to-back half-sweep launches.
-    ms_launch = eng.time_sweeps(50)
     bytes_launch = BYTES_PER_UPDATE * n_sites / 2
-    achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
+    achieved = bytes_launch / (fixed["ms_per_launch"] * 1e-3) / 1e9
     traffic = traffic_from_profiles()
     copy_ms = eng.bandwidth_probe(256 << 20)
+    mlp_tflops = MLP_FLOPS_PER_UPDATE * n_sites / 2 / (learned["ms_per_launch"] * 1e-3) / 1e12
 
     cpu = None if args.no_cpu else cpu_baseline()
+
+    def summary(r: dict) -> dict:
+        te, tq = np.array(r["tau_int_E_sweeps"]), np.array(r["tau_int_Q_sweeps"])
+        return {"site_updates_per_s": r["updates_per_s"], "ms_per_step": r["wall"] * 1e3 / args.steps,
+                "ms_per_half_sweep_launch": r["ms_per_launch"],
+                "samples_per_s_E": r["samples_per_s_E"], "samples_per_s_Q_middle_layer": r["samples_per_s_Q"],
+                "tau_int_E_sweeps": {"min": float(te.min()), "median": float(np.median(te)), "max": float(te.max())},
+                "tau_int_Q_sweeps": {"min": float(tq.min()), "median": float(np.median(tq)), "max": float(tq.max())}}
+
     line = {
         "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
-        "value": updates_per_s,
+        "value": fixed["updates_per_s"],
         "unit": "site updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": fixed["wall"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 spins, fp64 reductions",
+        "dtype": "fp32 spins, fp64 reductions, bf16 MFMA network",
         "data": "synthetic (seeded random-sphere start)",
-        "config": {"workload": "PT 64^3 x 64 replicas, T 0.2-2.0 geometric, J=1 D=0.5 (bulk) Bz=0.2, "
-                               "fixed cone r=0.5, swap every 10 sweeps, E/M every 5, Q every 10",
-                   "sweeps_per_step": SWAP_EVERY, "sampler": "fixed"},
-        "samples_per_s": {"E": n_e / wall, "Q_middle_layer": n_q / wall,
-                          "tau_int_E_sweeps": {"min": float(tau_e.min() * MEASURE_EVERY),
-                                               "max": float(tau_e.max() * MEASURE_EVERY)},
-                          "tau_int_Q_sweeps": {"min": float(tau_q.min() * WINDING_EVERY),
-                                               "max": float(tau_q.max() * WINDING_EVERY)}},
+        "config": {"workload": "PT 64^3 x 64 replicas, T 0.2-2.0 geometric, J=1 D=0.5 (bulk) Bz=0.2; "
+                               "swap every 10 sweeps, E/M every 2, Q every 10; value = fixed cone r=0.5",
+                   "sweeps_per_step": SWAP_EVERY},
+        "fixed": summary(fixed),
+        "learned": summary(learned),
+        "learned_over_fixed": {"samples_per_s_E": learned["samples_per_s_E"] / fixed["samples_per_s_E"],
+                               "samples_per_s_Q": learned["samples_per_s_Q"] / fixed["samples_per_s_Q"],
+                               "site_updates_per_s": learned["updates_per_s"] / fixed["updates_per_s"]},
+        "training": {"records": n_records, "steps": TRAIN_STEPS, "batch": TRAIN_BATCH,
+                     "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-10:].mean()),
+                     "prep_seconds": prep_s},
+        "observables": {"fixed_E_per_site": fixed["mean_E_per_site"][::8], "learned_E_per_site": learned["mean_E_per_site"][::8]},
         "swap_acceptance": {"min": float((acc[:-1] / np.maximum(att[:-1], 1)).min()),
                             "mean": float((acc[:-1] / np.maximum(att[:-1], 1)).mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_half_sweep", "ms_per_launch": ms_launch,
+                     "kernel": "k_half_sweep", "ms_per_launch": fixed["ms_per_launch"],
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "copy_probe_GBs": 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9,
-                     "tuning_flags": tune_flags, "tuning_ms_per_launch": [float(x) for x in tune_ms]},
+                     "tuning_flags": tune_flags, "tuning_ms_per_launch": [float(x) for x in tune_ms],
+                     "learned_kernel_mfma_TFLOPs": mlp_tflops, "mfma_bf16_peak_TFLOPs": 2500.0},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

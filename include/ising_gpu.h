@@ -189,6 +189,33 @@ int  ising_get_tuning(const ising_ctx* ctx, uint32_t* flags);
  * receives the time per half-sweep launch of flags 0..3. */
 int  ising_autotune(ising_ctx* ctx, int n_sweeps, uint32_t* best, double* ms);
 
+/* ---- learned proposal (no reference counterpart) ---------------------- */
+/* A per-site network sets the proposal radius from the six neighbour spins
+ * and beta*h (never the site's own spin, so the proposal stays symmetric):
+ * x[21] -> 32 -> 32 -> 1 with SiLU, bfloat16 weights, FP32 accumulation,
+ * output log r clamped to [ln 0.01, ln 8]. Parameters (FP32 master copy):
+ * W1[32][21], b1[32], W2[32][32], b2[32], w3[32], b3. */
+#define ISING_MLP_NPARAM 1793
+#define ISING_MLP_RECORD 32  /* floats per training record: x[21], S[3], h[3], beta, K, 3 pad */
+int  ising_mlp_set_params(ising_ctx* ctx, const float* params, int n);
+int  ising_mlp_get_params(ising_ctx* ctx, float* params, int n);
+/* Sweeps of `ladder` (< 0: all) take their radius from the network. */
+int  ising_mlp_enable(ising_ctx* ctx, int ladder, int on);
+/* Appends per_replica records per replica from the current configurations. */
+int  ising_mlp_collect(ising_ctx* ctx, int per_replica);
+int  ising_mlp_num_records(const ising_ctx* ctx, int64_t* n);
+int  ising_mlp_clear_records(ising_ctx* ctx);
+int  ising_mlp_get_records(ising_ctx* ctx, float* out, int64_t n);
+/* Adam steps on the collected records maximising the expected squared jump
+ * distance of the proposal; loss[steps] (may be NULL) = mean batch loss. */
+int  ising_mlp_train(ising_ctx* ctx, int steps, int batch, double lr, double* loss);
+/* Test hooks: the device forward pass on rows x[n][21]; the gradient of the
+ * mean loss over records[n] at optimiser step `step` (no update); one
+ * 32x32x16 bfloat16 MFMA D = A B with the operand layout the kernels use. */
+int  ising_mlp_forward(ising_ctx* ctx, const float* x, int n, float* f);
+int  ising_mlp_grad(ising_ctx* ctx, const float* records, int n, uint32_t step, double* loss, float* grad);
+int  ising_mfma_probe(ising_ctx* ctx, const float* A, const float* B, float* D);
+
 #ifdef __cplusplus
 }
 #endif

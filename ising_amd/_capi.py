@@ -24,6 +24,8 @@ ERRORS = {
     -8: "ISING_ERR_STATE",
 }
 DMI_NONE, DMI_BULK, DMI_INTERFACIAL_REF = 0, 1, 2
+MLP_NPARAM = 1793
+MLP_RECORD = 32
 
 
 class ising_lattice(C.Structure):
@@ -54,6 +56,7 @@ _P = C.c_void_p
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 
 # name -> (restype, argtypes); exactly the functions the header declares.
 SIGNATURES = {
@@ -89,6 +92,17 @@ SIGNATURES = {
     "ising_set_tuning": (C.c_int, [_P, C.c_uint32]),
     "ising_get_tuning": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "ising_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint32), _dp]),
+    "ising_mlp_set_params": (C.c_int, [_P, _fp, C.c_int]),
+    "ising_mlp_get_params": (C.c_int, [_P, _fp, C.c_int]),
+    "ising_mlp_enable": (C.c_int, [_P, C.c_int, C.c_int]),
+    "ising_mlp_collect": (C.c_int, [_P, C.c_int]),
+    "ising_mlp_num_records": (C.c_int, [_P, _i64p]),
+    "ising_mlp_clear_records": (C.c_int, [_P]),
+    "ising_mlp_get_records": (C.c_int, [_P, _fp, C.c_int64]),
+    "ising_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, _dp]),
+    "ising_mlp_forward": (C.c_int, [_P, _fp, C.c_int, _fp]),
+    "ising_mlp_grad": (C.c_int, [_P, _fp, C.c_int, C.c_uint32, _dp, _fp]),
+    "ising_mfma_probe": (C.c_int, [_P, _fp, _fp, _fp]),
 }
 
 _lib = None

@@ -28,6 +28,7 @@
 
 #include "ising_gpu.h"
 #include "philox.hpp"
+#include "mlp.hpp"
 
 namespace ising {
 
@@ -90,6 +91,7 @@ struct Lad {
     float zsf;              // Zeeman coefficient of total energies
     float d[9];
     int32_t dmi;
+    int32_t learned;        // cone radius from the network (mlp.hpp)
     double Jd, Bzd, Kzd, zs;
     double dd[9];
 };
@@ -173,16 +175,18 @@ __device__ __forceinline__ Nbr neighbours(const Rep& R, uint32_t row, uint32_t h
     return nb;
 }
 
-// Local field h = J sum_j S_j - sum_a (S_{+a} - S_{-a}) x d_a + Bz z.
+// The six neighbour spins (+x, -x, +y, -y, +z, -z); zero across open faces.
 template <bool kPer>
-__device__ __forceinline__ f3 local_field(const Lad& L, const Planes& oth, const Nbr& nb)
+__device__ __forceinline__ void load_nbrs(const Planes& oth, const Nbr& nb, f3 (&v)[6])
 {
-    const f3 xp = kPer ? ld3(oth, nb.i[0]) : ld3c(oth, nb.i[0], nb.ok[0]);
-    const f3 xm = kPer ? ld3(oth, nb.i[1]) : ld3c(oth, nb.i[1], nb.ok[1]);
-    const f3 yp = kPer ? ld3(oth, nb.i[2]) : ld3c(oth, nb.i[2], nb.ok[2]);
-    const f3 ym = kPer ? ld3(oth, nb.i[3]) : ld3c(oth, nb.i[3], nb.ok[3]);
-    const f3 zp = kPer ? ld3(oth, nb.i[4]) : ld3c(oth, nb.i[4], nb.ok[4]);
-    const f3 zm = kPer ? ld3(oth, nb.i[5]) : ld3c(oth, nb.i[5], nb.ok[5]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = kPer ? ld3(oth, nb.i[k]) : ld3c(oth, nb.i[k], nb.ok[k]);
+}
+
+// Local field h = J sum_j S_j - sum_a (S_{+a} - S_{-a}) x d_a + Bz z.
+__device__ __forceinline__ f3 field_from(const Lad& L, const f3 (&v)[6])
+{
+    const f3 &xp = v[0], &xm = v[1], &yp = v[2], &ym = v[3], &zp = v[4], &zm = v[5];
     const f3 sum{ (xp.x + xm.x) + (yp.x + ym.x) + (zp.x + zm.x),
                   (xp.y + xm.y) + (yp.y + ym.y) + (zp.y + zm.y),
                   (xp.z + xm.z) + (yp.z + ym.z) + (zp.z + zm.z) };
@@ -202,11 +206,12 @@ __device__ __forceinline__ f3 local_field(const Lad& L, const Planes& oth, const
     return f3{ L.J * sum.x - dm.x, L.J * sum.y - dm.y, L.J * sum.z - dm.z + L.Bz };
 }
 
-__device__ __forceinline__ double wave_sum(double v)
+template <bool kPer>
+__device__ __forceinline__ f3 local_field(const Lad& L, const Planes& oth, const Nbr& nb)
 {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    f3 v[6];
+    load_nbrs<kPer>(oth, nb, v);
+    return field_from(L, v);
 }
 
 // dE = -(S' - S).h - K (S'_z - S_z)(S'_z + S_z): the difference is formed
@@ -217,24 +222,39 @@ __device__ __forceinline__ float delta_e(const f3& s, const f3& sn, const f3& h,
     return -(ddx * h.x + ddy * h.y + ddz * h.z) - K * ddz * (sn.z + s.z);
 }
 
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+
 // ---------------------------------------------------------------------------
 // Half-sweep: one Metropolis proposal for every site of one colour of every
 // replica. Proposal: u uniform in a ball of radius r (cos(theta) = 2U-1,
 // phi = 2 pi U', radius r U''^(1/3)), S' = (S+u)/|S+u|; accept iff
 // U''' < exp(-beta dE). One Philox block supplies the four uniforms.
 //
+// kLearned: ladders with the learned flag take r from the network of
+// mlp.hpp evaluated on the six neighbour spins and beta*h (never the site's
+// own spin), so r is the same for S -> S' and S' -> S. Every lane of the
+// wavefront computes (lanes past the end of a replica on a clamped valid
+// site) because the network exchanges data across lanes; only in-range
+// lanes store or count.
+//
 // kMeasure (colour-1 launches only): after its update every colour-1 site adds
 // its six bonds, -S.(h - Bz z) with the field h already formed (each bond has
 // exactly one colour-1 end), plus the on-site terms and spins of itself and of
 // the colour-0 site at the same half index; per-site FP32, summed in FP64 per
 // wavefront into wave_part[wave][4] = (E, Mx, My, Mz).
-template <bool kMeasure>
+template <bool kMeasure, bool kLearned>
 __global__ __launch_bounds__(kBlock) void k_half_sweep(
     float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ block_rep,
     const uint32_t* __restrict__ sweep_ctr, uint32_t sweep_off, int colour,
     uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count,
-    double* __restrict__ wave_part, uint32_t tune)
+    double* __restrict__ wave_part, uint32_t tune, const MlpDev* __restrict__ mlp)
 {
     const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
     const int slot = block_rep[blk];
@@ -242,68 +262,78 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
     const Lad L = lads[R.ladder];
     const float beta = beta_slot[slot];
     const uint32_t sweep = *sweep_ctr + sweep_off;
-    const uint32_t q = (blk - (uint32_t)R.first_block) * kBlock + threadIdx.x;
-    const bool active = q < R.half;
+    const uint32_t q0 = (blk - (uint32_t)R.first_block) * kBlock + threadIdx.x;
+    const bool active = q0 < R.half;
+    const uint32_t q = active ? q0 : R.half - 1u;
     const bool nt = tune & kTuneNtOwn;
-    bool accepted = false;
-    float me = 0.f, mx = 0.f, my = 0.f, mz = 0.f;   // measurement of this lane
-    if (active) {
-        const Planes own = planes(spins, R, colour);
-        const Planes oth = planes(spins, R, colour ^ 1);
-        const uint32_t row = fdiv(q, R.div_hx);
-        const uint32_t h = q - row * (uint32_t)R.hx;
-        const uint32_t iz = fdiv(row, R.div_ly);
-        const uint32_t iy = row - iz * (uint32_t)R.ly;
-        const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
-        const uint32_t ix = 2u * h + p;
-        const f3 s = nt ? f3{ ldf_nt(own.x, q), ldf_nt(own.y, q), ldf_nt(own.z, q) } : ld3(own, q);
-        f3 fs = s;                 // spin after the decision
-        f3 hf;
-        if (R.per == 7) {
-            hf = local_field<true>(L, oth, neighbours<true>(R, row, h, iy, iz, p));
-        } else {
-            hf = local_field<false>(L, oth, neighbours<false>(R, row, h, iy, iz, p));
-        }
 
-        const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
-        const u32x4 rnd = philox4x32_10(u32x4{ site, sweep, kStreamMetropolis, seed_hi }, seed_lo, (uint32_t)slot);
-        const float ct = s24f(rnd.x);
-        const float uphi = u24f(rnd.y);
-        const float urad = u24f(rnd.z);
-        const float uacc = u24f(rnd.w);
-        const float st = sqrtf(fmaxf(0.f, (1.f - ct) * (1.f + ct)));
-        const float sp = __builtin_amdgcn_sinf(uphi);   // sin(2 pi uphi)
-        const float cp = __builtin_amdgcn_cosf(uphi);   // cos(2 pi uphi)
-        const float rr = L.r * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(urad) * (1.f / 3.f));
-        const f3 t{ s.x + rr * st * cp, s.y + rr * st * sp, s.z + rr * ct };
-        const float n2 = t.x * t.x + t.y * t.y + t.z * t.z;
-        if (n2 > 1e-30f) {
-            const float inv = __builtin_amdgcn_rsqf(n2);
-            const f3 sn{ t.x * inv, t.y * inv, t.z * inv };
-            const float de = delta_e(s, sn, hf, L.Kz);
-            const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
-            if (uacc < w) {
-                if (nt) {
-                    stf_nt((float*)own.x, q, sn.x);
-                    stf_nt((float*)own.y, q, sn.y);
-                    stf_nt((float*)own.z, q, sn.z);
-                } else {
-                    stf((float*)own.x, q, sn.x);
-                    stf((float*)own.y, q, sn.y);
-                    stf((float*)own.z, q, sn.z);
-                }
-                accepted = true;
-                fs = sn;
-            }
+    const Planes own = planes(spins, R, colour);
+    const Planes oth = planes(spins, R, colour ^ 1);
+    const uint32_t row = fdiv(q, R.div_hx);
+    const uint32_t h = q - row * (uint32_t)R.hx;
+    const uint32_t iz = fdiv(row, R.div_ly);
+    const uint32_t iy = row - iz * (uint32_t)R.ly;
+    const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
+    const uint32_t ix = 2u * h + p;
+    const f3 s = nt ? f3{ ldf_nt(own.x, q), ldf_nt(own.y, q), ldf_nt(own.z, q) } : ld3(own, q);
+    f3 nbr[6];
+    if (R.per == 7) load_nbrs<true>(oth, neighbours<true>(R, row, h, iy, iz, p), nbr);
+    else load_nbrs<false>(oth, neighbours<false>(R, row, h, iy, iz, p), nbr);
+    const f3 hf = field_from(L, nbr);
+
+    float radius = L.r;
+    if constexpr (kLearned) {
+        if (L.learned) {      // uniform per workgroup (one replica)
+            const float x[kMlpIn] = { nbr[0].x, nbr[0].y, nbr[0].z, nbr[1].x, nbr[1].y, nbr[1].z,
+                                      nbr[2].x, nbr[2].y, nbr[2].z, nbr[3].x, nbr[3].y, nbr[3].z,
+                                      nbr[4].x, nbr[4].y, nbr[4].z, nbr[5].x, nbr[5].y, nbr[5].z,
+                                      beta * hf.x, beta * hf.y, beta * hf.z };
+            const float f = mlp_forward<false>(*mlp, x, nullptr);
+            radius = __builtin_amdgcn_exp2f(clamp_log_radius(f) * 1.4426950408889634f);
         }
-        if (kMeasure) {
+    }
+
+    const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
+    const u32x4 rnd = philox4x32_10(u32x4{ site, sweep, kStreamMetropolis, seed_hi }, seed_lo, (uint32_t)slot);
+    const float ct = s24f(rnd.x);
+    const float uphi = u24f(rnd.y);
+    const float urad = u24f(rnd.z);
+    const float uacc = u24f(rnd.w);
+    const float st = sqrtf(fmaxf(0.f, (1.f - ct) * (1.f + ct)));
+    const float sp = __builtin_amdgcn_sinf(uphi);   // sin(2 pi uphi)
+    const float cp = __builtin_amdgcn_cosf(uphi);   // cos(2 pi uphi)
+    const float rr = radius * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(urad) * (1.f / 3.f));
+    const f3 t{ s.x + rr * st * cp, s.y + rr * st * sp, s.z + rr * ct };
+    const float n2 = t.x * t.x + t.y * t.y + t.z * t.z;
+    bool accepted = false;
+    f3 fs = s;                 // spin after the decision
+    if (active && n2 > 1e-30f) {
+        const float inv = __builtin_amdgcn_rsqf(n2);
+        const f3 sn{ t.x * inv, t.y * inv, t.z * inv };
+        const float de = delta_e(s, sn, hf, L.Kz);
+        const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
+        if (uacc < w) {
+            if (nt) {
+                stf_nt((float*)own.x, q, sn.x);
+                stf_nt((float*)own.y, q, sn.y);
+                stf_nt((float*)own.z, q, sn.z);
+            } else {
+                stf((float*)own.x, q, sn.x);
+                stf((float*)own.y, q, sn.y);
+                stf((float*)own.z, q, sn.z);
+            }
+            accepted = true;
+            fs = sn;
+        }
+    }
+    if constexpr (kMeasure) {
+        float me = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+        if (active) {
             const f3 c0 = ld3(planes(spins, R, 0), q);
             me = -(fs.x * hf.x + fs.y * hf.y + fs.z * (hf.z - L.Bz))
                  - L.zsf * L.Bz * (fs.z + c0.z) - L.Kz * (fs.z * fs.z + c0.z * c0.z);
             mx = fs.x + c0.x; my = fs.y + c0.y; mz = fs.z + c0.z;
         }
-    }
-    if (kMeasure) {
         double v[4] = { me, mx, my, mz };
 #pragma unroll
         for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
@@ -319,6 +349,44 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
         unsigned long long* c = acc_count + (size_t)blk * (kBlock / kWave) + threadIdx.x / kWave;
         *c += (unsigned long long)__popcll(ball);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Training records from the current configurations: per replica slot,
+// `per` sites drawn with the collect stream; record = the six neighbour
+// spins, beta*h, S, h, beta, K (mlp.hpp layout).
+__global__ __launch_bounds__(kBlock) void k_mlp_collect(
+    const float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const float* __restrict__ beta_slot, int R, int per, uint32_t call, uint32_t seed_lo, uint32_t seed_hi,
+    float* __restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= R * per) return;
+    const int slot = e / per, k = e - slot * per;
+    const Rep Rr = reps[slot];
+    const Lad L = lads[Rr.ladder];
+    const float beta = beta_slot[slot];
+    const u32x4 r = philox4x32_10(u32x4{ (uint32_t)k, call, kStreamCollect, seed_hi }, seed_lo, (uint32_t)slot);
+    const uint32_t q = r.x % Rr.half;
+    const int colour = (int)(r.y & 1u);
+    const uint32_t row = fdiv(q, Rr.div_hx);
+    const uint32_t h = q - row * (uint32_t)Rr.hx;
+    const uint32_t iz = fdiv(row, Rr.div_ly);
+    const uint32_t iy = row - iz * (uint32_t)Rr.ly;
+    const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
+    const Planes own = planes(spins, Rr, colour);
+    const Planes oth = planes(spins, Rr, colour ^ 1);
+    f3 nbr[6];
+    load_nbrs<false>(oth, neighbours<false>(Rr, row, h, iy, iz, p), nbr);
+    const f3 hf = field_from(L, nbr);
+    const f3 s = ld3(own, q);
+    float* o = out + (size_t)e * kRecord;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) { o[3 * j] = nbr[j].x; o[3 * j + 1] = nbr[j].y; o[3 * j + 2] = nbr[j].z; }
+    o[18] = beta * hf.x; o[19] = beta * hf.y; o[20] = beta * hf.z;
+    o[21] = s.x; o[22] = s.y; o[23] = s.z;
+    o[24] = hf.x; o[25] = hf.y; o[26] = hf.z;
+    o[27] = beta; o[28] = L.Kz; o[29] = 0.f; o[30] = 0.f; o[31] = 0.f;
 }
 
 // Per-slot sum of the per-wavefront acceptance counters.
@@ -723,6 +791,21 @@ struct ising_ctx {
     int64_t total_floats = 0;
     uint32_t host_sweep = 0;
     uint32_t tune = 0;           // sweep tuning flags (kTune*)
+    bool any_learned = false;    // some ladder takes its cone radius from the network
+    std::vector<int> learned;    // per ladder
+
+    // learned proposal
+    MlpDev* d_mlp = nullptr;
+    float* d_mlp_prm = nullptr;  // FP32 master parameters
+    float* d_mlp_m1 = nullptr;   // Adam moments
+    float* d_mlp_m2 = nullptr;
+    float* d_mlp_grad = nullptr; // [kMlpParams + 1], last = loss sum
+    float* d_wave_grad = nullptr;
+    int wave_grad_waves = 0;
+    float* d_recs = nullptr;     // training records [cap][kRecord]
+    int64_t n_recs = 0, cap_recs = 0;
+    uint32_t collect_calls = 0;
+    int adam_step = 0;
 
     float* d_spins = nullptr;
     Rep* d_reps = nullptr;
@@ -838,7 +921,10 @@ int check_ham(ising_ctx* c, const ising_hamiltonian& h)
 int upload_lads(ising_ctx* c)
 {
     std::vector<Lad> l;
-    for (auto& L : c->ladders) l.push_back(make_lad(L.ham, L.cone));
+    for (size_t k = 0; k < c->ladders.size(); ++k) {
+        l.push_back(make_lad(c->ladders[k].ham, c->ladders[k].cone));
+        l.back().learned = k < c->learned.size() ? c->learned[k] : 0;
+    }
     HIP_TRY(c, hipMemcpyAsync(c->d_lads, l.data(), l.size() * sizeof(Lad), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return ISING_OK;
@@ -853,21 +939,27 @@ int check_replica(ising_ctx* c, int r)
 
 // ---- launch helpers --------------------------------------------------------
 
+template <bool kMeasure, bool kLearned>
+void launch_sweep_kernel(ising_ctx* c, uint32_t off, int colour)
+{
+    hipLaunchKernelGGL((k_half_sweep<kMeasure, kLearned>), dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
+                       c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
+                       c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
+                       kMeasure ? c->d_wave_part : (double*)nullptr, c->tune, (const MlpDev*)c->d_mlp);
+}
+
 int launch_half_sweeps(ising_ctx* c, uint32_t off, int colour, bool measure = false)
 {
+    const bool learned = c->any_learned;
     if (measure) {
-        hipLaunchKernelGGL(k_half_sweep<true>, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
-                           c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
-                           c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
-                           c->d_wave_part, c->tune);
+        if (learned) launch_sweep_kernel<true, true>(c, off, colour);
+        else launch_sweep_kernel<true, false>(c, off, colour);
         LAUNCH_CHECK(c);
         hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
                            (const double*)c->d_wave_part, c->d_energy, c->d_mag);
     } else {
-        hipLaunchKernelGGL(k_half_sweep<false>, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
-                           c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
-                           c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
-                           (double*)nullptr, c->tune);
+        if (learned) launch_sweep_kernel<false, true>(c, off, colour);
+        else launch_sweep_kernel<false, false>(c, off, colour);
     }
     LAUNCH_CHECK(c);
     return ISING_OK;
@@ -997,7 +1089,8 @@ void ising_destroy(ising_ctx* c)
     void* ptrs[] = { c->d_spins, c->d_reps, c->d_lads, c->d_lad_override, c->d_block_rep, c->d_qblock_rep,
                      c->d_beta_slot, c->d_beta_temp, c->d_slot_of_temp, c->d_lad_of_pos, c->d_lad_start,
                      c->d_lad_n, c->d_partial, c->d_energy, c->d_mag, c->d_qpartial, c->d_qraw, c->d_qint,
-                     c->d_ctr, c->d_acc, c->d_acc_wave, c->d_wave_part, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de };
+                     c->d_ctr, c->d_acc, c->d_acc_wave, c->d_wave_part, c->d_mlp, c->d_mlp_prm, c->d_mlp_m1,
+                     c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1107,6 +1200,12 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if ((rc = dalloc(c, &c->d_acc, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_acc_wave, (size_t)blk * (kBlock / kWave)))) return fail(rc);
     if ((rc = dalloc(c, &c->d_wave_part, 4 * (size_t)blk * (kBlock / kWave)))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_mlp, 1))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_mlp_prm, kMlpParams))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_mlp_m1, kMlpParams))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_mlp_m2, kMlpParams))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_mlp_grad, kMlpParams + 1))) return fail(rc);
+    c->learned.assign(n_ladders, 0);
     if ((rc = dalloc(c, &c->d_swap_att, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_swap_acc, R))) return fail(rc);
 
@@ -1141,10 +1240,16 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
         hipMemsetAsync(c->d_ctr, 0, 4 * sizeof(uint32_t), s) != hipSuccess ||
         hipMemsetAsync(c->d_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
         hipMemsetAsync(c->d_acc_wave, 0, (size_t)blk * (kBlock / kWave) * sizeof(unsigned long long), s) != hipSuccess ||
+        hipMemsetAsync(c->d_mlp_prm, 0, kMlpParams * sizeof(float), s) != hipSuccess ||
+        hipMemsetAsync(c->d_mlp_m1, 0, kMlpParams * sizeof(float), s) != hipSuccess ||
+        hipMemsetAsync(c->d_mlp_m2, 0, kMlpParams * sizeof(float), s) != hipSuccess ||
         hipMemsetAsync(c->d_swap_att, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
         hipMemsetAsync(c->d_swap_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess)
         return fail(set_err(c, ISING_ERR_HIP, "hipMemsetAsync"));
     if ((rc = upload_lads(c))) return fail(rc);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, s, (const float*)c->d_mlp_prm, c->d_mlp);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return fail(set_err(c, ISING_ERR_HIP, "network initialisation"));
     *out = c;
     return ISING_OK;
 }
@@ -1590,6 +1695,219 @@ int ising_autotune(ising_ctx* c, int n_sweeps, uint32_t* best, double* ms)
     if (best) *best = arg;
     if (ms) for (int f = 0; f < 4; ++f) ms[f] = t[f];
     return ISING_OK;
+}
+
+// ---- learned proposal ------------------------------------------------------
+
+static int mlp_pack(ising_ctx* c)
+{
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, c->stream, (const float*)c->d_mlp_prm, c->d_mlp);
+    LAUNCH_CHECK(c);
+    return ISING_OK;
+}
+
+int ising_mlp_set_params(ising_ctx* c, const float* prm, int n)
+{
+    if (!c || !prm) return ISING_ERR_INVALID_ARG;
+    if (n != kMlpParams) return set_err(c, ISING_ERR_INVALID_ARG, "network has %d parameters, got %d", kMlpParams, n);
+    for (int k = 0; k < n; ++k)
+        if (!std::isfinite(prm[k])) return set_err(c, ISING_ERR_INVALID_ARG, "parameter %d is not finite", k);
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->d_mlp_prm, prm, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_mlp_m1, 0, n * sizeof(float), c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_mlp_m2, 0, n * sizeof(float), c->stream));
+    c->adam_step = 0;
+    int rc = mlp_pack(c);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_mlp_get_params(ising_ctx* c, float* prm, int n)
+{
+    if (!c || !prm) return ISING_ERR_INVALID_ARG;
+    if (n != kMlpParams) return set_err(c, ISING_ERR_INVALID_ARG, "network has %d parameters, got %d", kMlpParams, n);
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(prm, c->d_mlp_prm, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_mlp_enable(ising_ctx* c, int ladder, int on)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (ladder >= (int)c->ladders.size()) return set_err(c, ISING_ERR_INVALID_ARG, "ladder %d", ladder);
+    HIP_TRY(c, hipSetDevice(c->device));
+    for (int k = 0; k < (int)c->learned.size(); ++k)
+        if (ladder < 0 || k == ladder) c->learned[k] = on ? 1 : 0;
+    c->any_learned = false;
+    for (int v : c->learned) c->any_learned = c->any_learned || v;
+    return upload_lads(c);
+}
+
+int ising_mlp_collect(ising_ctx* c, int per_replica)
+{
+    if (!c || per_replica < 1) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t add = (int64_t)c->R * per_replica;
+    if (c->n_recs + add > (int64_t)UINT32_MAX) return set_err(c, ISING_ERR_INVALID_ARG, "too many training records");
+    if (c->n_recs + add > c->cap_recs) {
+        int64_t cap = std::max<int64_t>(2 * c->cap_recs, c->n_recs + add);
+        float* nb = nullptr;
+        int rc = dalloc(c, &nb, (size_t)cap * kRecord);
+        if (rc) return rc;
+        if (c->n_recs)
+            HIP_TRY(c, hipMemcpyAsync(nb, c->d_recs, (size_t)c->n_recs * kRecord * sizeof(float),
+                                      hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (c->d_recs) (void)hipFree(c->d_recs);
+        c->d_recs = nb;
+        c->cap_recs = cap;
+    }
+    hipLaunchKernelGGL(k_mlp_collect, dim3((unsigned)((add + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                       (const float*)c->d_spins, c->d_reps, c->d_lads, (const float*)c->d_beta_slot, c->R, per_replica,
+                       c->collect_calls, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
+                       c->d_recs + (size_t)c->n_recs * kRecord);
+    LAUNCH_CHECK(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->n_recs += add;
+    c->collect_calls++;
+    return ISING_OK;
+}
+
+int ising_mlp_num_records(const ising_ctx* c, int64_t* n)
+{
+    if (!c || !n) return ISING_ERR_INVALID_ARG;
+    *n = c->n_recs;
+    return ISING_OK;
+}
+
+int ising_mlp_clear_records(ising_ctx* c)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    c->n_recs = 0;
+    return ISING_OK;
+}
+
+int ising_mlp_get_records(ising_ctx* c, float* out, int64_t n)
+{
+    if (!c || !out || n < 0 || n > c->n_recs) return ISING_ERR_INVALID_ARG;
+    if (n == 0) return ISING_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(out, c->d_recs, (size_t)n * kRecord * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+static int mlp_grad_launch(ising_ctx* c, const float* recs, uint32_t n_recs, uint32_t batch, uint32_t step, int hashed)
+{
+    const int tiles = (int)((batch + 63) / 64);
+    const int blocks = std::max(1, std::min(256, (tiles + 3) / 4));
+    const int waves = blocks * 4;
+    if (waves > c->wave_grad_waves) {
+        if (c->d_wave_grad) (void)hipFree(c->d_wave_grad);
+        c->d_wave_grad = nullptr;
+        c->wave_grad_waves = 0;
+        int rc = dalloc(c, &c->d_wave_grad, (size_t)waves * (kMlpParams + 1));
+        if (rc) return rc;
+        c->wave_grad_waves = waves;
+    }
+    hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(256), 0, c->stream, recs, n_recs, batch, step, hashed,
+                       (const MlpDev*)c->d_mlp, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_wave_grad);
+    LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, c->stream,
+                       (const float*)c->d_wave_grad, waves, c->d_mlp_grad);
+    LAUNCH_CHECK(c);
+    return ISING_OK;
+}
+
+int ising_mlp_train(ising_ctx* c, int steps, int batch, double lr, double* loss)
+{
+    if (!c || steps < 0 || batch < 1 || !(lr > 0.0)) return ISING_ERR_INVALID_ARG;
+    if (c->n_recs < 1) return set_err(c, ISING_ERR_STATE, "no training records: call ising_mlp_collect first");
+    HIP_TRY(c, hipSetDevice(c->device));
+    std::vector<float> lsum(loss ? steps : 0);
+    for (int t = 0; t < steps; ++t) {
+        int rc = mlp_grad_launch(c, c->d_recs, (uint32_t)c->n_recs, (uint32_t)batch, (uint32_t)c->adam_step, 1);
+        if (rc) return rc;
+        c->adam_step++;
+        const float bc1 = 1.f - powf(0.9f, (float)c->adam_step), bc2 = 1.f - powf(0.999f, (float)c->adam_step);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, c->stream, c->d_mlp_prm,
+                           c->d_mlp_m1, c->d_mlp_m2, (const float*)c->d_mlp_grad, (float)lr, bc1, bc2);
+        LAUNCH_CHECK(c);
+        if ((rc = mlp_pack(c))) return rc;
+        if (loss) HIP_TRY(c, hipMemcpyAsync(&lsum[t], c->d_mlp_grad + kMlpParams, sizeof(float),
+                                            hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int t = 0; t < (loss ? steps : 0); ++t) loss[t] = (double)lsum[t] / (double)batch;
+    return ISING_OK;
+}
+
+int ising_mlp_forward(ising_ctx* c, const float* x, int n, float* f)
+{
+    if (!c || n < 0 || (n > 0 && (!x || !f))) return ISING_ERR_INVALID_ARG;
+    if (n == 0) return ISING_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    float *dx = nullptr, *df = nullptr;
+    int rc = dalloc(c, &dx, (size_t)n * kMlpIn);
+    if (rc) return rc;
+    if ((rc = dalloc(c, &df, (size_t)n))) { (void)hipFree(dx); return rc; }
+    if (hipMemcpyAsync(dx, x, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = set_err(c, ISING_ERR_HIP, "upload");
+    if (!rc) {
+        hipLaunchKernelGGL(k_mlp_forward_rows, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float*)dx, n,
+                           (const MlpDev*)c->d_mlp, df);
+        if (hipGetLastError() != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "launch");
+    }
+    if (!rc && hipMemcpyAsync(f, df, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = set_err(c, ISING_ERR_HIP, "download");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "sync");
+    (void)hipFree(dx);
+    (void)hipFree(df);
+    return rc;
+}
+
+int ising_mlp_grad(ising_ctx* c, const float* recs, int n, uint32_t step, double* loss, float* grad)
+{
+    if (!c || n < 1 || !recs || !grad) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    float* dr = nullptr;
+    int rc = dalloc(c, &dr, (size_t)n * kRecord);
+    if (rc) return rc;
+    if (hipMemcpyAsync(dr, recs, (size_t)n * kRecord * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = set_err(c, ISING_ERR_HIP, "upload");
+    if (!rc) rc = mlp_grad_launch(c, dr, (uint32_t)n, (uint32_t)n, step, 0);
+    std::vector<float> g(kMlpParams + 1);
+    if (!rc && hipMemcpyAsync(g.data(), c->d_mlp_grad, g.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = set_err(c, ISING_ERR_HIP, "download");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "sync");
+    (void)hipFree(dr);
+    if (rc) return rc;
+    for (int k = 0; k < kMlpParams; ++k) grad[k] = g[k];
+    if (loss) *loss = (double)g[kMlpParams] / (double)n;
+    return ISING_OK;
+}
+
+int ising_mfma_probe(ising_ctx* c, const float* A, const float* B, float* D)
+{
+    if (!c || !A || !B || !D) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    float* d = nullptr;
+    int rc = dalloc(c, &d, 32 * 16 + 16 * 32 + 32 * 32);
+    if (rc) return rc;
+    if (hipMemcpyAsync(d, A, 512 * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(d + 512, B, 512 * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = set_err(c, ISING_ERR_HIP, "upload");
+    if (!rc) {
+        hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, c->stream, (const float*)d, (const float*)(d + 512), d + 1024);
+        if (hipGetLastError() != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "launch");
+    }
+    if (!rc && hipMemcpyAsync(D, d + 1024, 1024 * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = set_err(c, ISING_ERR_HIP, "download");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "sync");
+    (void)hipFree(d);
+    return rc;
 }
 
 int ising_accept_counts(ising_ctx* c, int64_t* accepted)

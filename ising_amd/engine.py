@@ -303,6 +303,69 @@ class Engine:
                                              ms.ctypes.data_as(C.POINTER(C.c_double))))
         return best.value, ms
 
+    # -- learned proposal ---------------------------------------------------
+    @staticmethod
+    def _f32(a) -> np.ndarray:
+        return np.ascontiguousarray(a, dtype=np.float32)
+
+    def mlp_set_params(self, params) -> None:
+        p = self._f32(params)
+        self._check(self._lib.ising_mlp_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size))
+
+    def mlp_params(self) -> np.ndarray:
+        p = np.zeros(_capi.MLP_NPARAM, dtype=np.float32)
+        self._check(self._lib.ising_mlp_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size))
+        return p
+
+    def mlp_enable(self, on: bool = True, ladder: int = -1) -> None:
+        self._check(self._lib.ising_mlp_enable(self._h, int(ladder), 1 if on else 0))
+
+    def mlp_collect(self, per_replica: int) -> None:
+        self._check(self._lib.ising_mlp_collect(self._h, int(per_replica)))
+
+    def mlp_num_records(self) -> int:
+        n = C.c_int64()
+        self._check(self._lib.ising_mlp_num_records(self._h, C.byref(n)))
+        return n.value
+
+    def mlp_clear_records(self) -> None:
+        self._check(self._lib.ising_mlp_clear_records(self._h))
+
+    def mlp_records(self, n: int | None = None) -> np.ndarray:
+        n = self.mlp_num_records() if n is None else n
+        out = np.zeros((n, _capi.MLP_RECORD), dtype=np.float32)
+        self._check(self._lib.ising_mlp_get_records(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return out
+
+    def mlp_train(self, steps: int, batch: int = 16384, lr: float = 3e-3) -> np.ndarray:
+        loss = np.zeros(steps)
+        self._check(self._lib.ising_mlp_train(self._h, int(steps), int(batch), float(lr),
+                                              loss.ctypes.data_as(C.POINTER(C.c_double))))
+        return loss
+
+    def mlp_forward(self, x) -> np.ndarray:
+        xx = self._f32(x).reshape(-1, 21)
+        f = np.zeros(len(xx), dtype=np.float32)
+        self._check(self._lib.ising_mlp_forward(self._h, xx.ctypes.data_as(C.POINTER(C.c_float)), len(xx),
+                                                f.ctypes.data_as(C.POINTER(C.c_float))))
+        return f
+
+    def mlp_grad(self, records, step: int = 0) -> tuple[float, np.ndarray]:
+        r = self._f32(records).reshape(-1, _capi.MLP_RECORD)
+        g = np.zeros(_capi.MLP_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self._lib.ising_mlp_grad(self._h, r.ctypes.data_as(C.POINTER(C.c_float)), len(r), int(step),
+                                             C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss.value, g
+
+    def mfma_probe(self, A, B) -> np.ndarray:
+        a, b = self._f32(A).reshape(32, 16), self._f32(B).reshape(16, 32)
+        d = np.zeros((32, 32), dtype=np.float32)
+        self._check(self._lib.ising_mfma_probe(self._h, a.ctypes.data_as(C.POINTER(C.c_float)),
+                                               b.ctypes.data_as(C.POINTER(C.c_float)),
+                                               d.ctypes.data_as(C.POINTER(C.c_float))))
+        return d
+
     def synchronize(self) -> None:
         self._check(self._lib.ising_synchronize(self._h))
 

@@ -69,6 +69,10 @@ def lib() -> C.CDLL:
                                             C.c_uint32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
             "orc_num_threads": (C.c_int, []),
             "orc_set_threads": (None, [C.c_int]),
+            "orc_mlp_forward": (C.c_double, [dp, dp, C.c_int]),
+            "orc_mlp_loss_grad": (C.c_double, [dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
+            "orc_mlp_unit_ball": (None, [C.c_uint64, C.c_uint32, C.c_uint32, dp]),
+            "orc_mlp_nparam": (C.c_int, []),
         }
 
         class ref_rng(C.Structure):
@@ -249,3 +253,29 @@ def set_threads(n: int) -> None:
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+MLP_NPARAM = 1793
+MLP_LOG_RMIN, MLP_LOG_RMAX = float(np.log(0.01)), float(np.log(8.0))
+
+
+def mlp_forward(params, x, quantize=True) -> float:
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().orc_mlp_forward(_dp(p), _dp(xx), 1 if quantize else 0)
+
+
+def mlp_unit_ball(seed, rec, step) -> np.ndarray:
+    u = np.zeros(3)
+    lib().orc_mlp_unit_ball(seed, rec, step, _dp(u))
+    return u
+
+
+def mlp_loss_grad(params, recs, uvecs, quantize=True, lo=MLP_LOG_RMIN, hi=MLP_LOG_RMAX):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    r = np.ascontiguousarray(recs, dtype=np.float64)
+    u = np.ascontiguousarray(uvecs, dtype=np.float64)
+    assert r.shape[1] == 32 and u.shape == (r.shape[0], 3)
+    g = np.zeros(MLP_NPARAM)
+    loss = lib().orc_mlp_loss_grad(_dp(p), _dp(r), _dp(u), r.shape[0], 1 if quantize else 0, lo, hi, _dp(g))
+    return loss, g
