@@ -86,7 +86,7 @@ typedef struct {
     int32_t measure_every;   /* E and M recorded every k sweeps (0=off)        */
     int32_t winding_every;   /* per-layer Q recorded every k sweeps (0=off)    */
     int32_t use_graph;       /* 1: replay one captured hipGraph per period     */
-    int32_t reserved;
+    int32_t winding_layer;   /* 0: Q of every layer; z+1: Q of layer z only     */
 } ising_schedule;
 
 /* Measurement series of the last ising_pt_run, indexed by temperature
@@ -192,7 +192,8 @@ int  ising_autotune(ising_ctx* ctx, int n_sweeps, uint32_t* best, double* ms);
 /* ---- learned proposal (no reference counterpart) ---------------------- */
 /* A per-site network sets the proposal radius from the six neighbour spins
  * and beta*h (never the site's own spin, so the proposal stays symmetric):
- * x[21] -> 32 -> 32 -> 1 with SiLU, bfloat16 weights, FP32 accumulation,
+ * x[21] -> 32 -> 32 -> 1 with SiLU in base 2 (z / (1 + 2^-z)), bfloat16
+ * weights, FP32 accumulation,
  * output log r clamped to [ln 0.01, ln 8]. Parameters (FP32 master copy):
  * W1[32][21], b1[32], W2[32][32], b2[32], w3[32], b3. */
 #define ISING_MLP_NPARAM 1793

@@ -44,7 +44,7 @@ class ising_ladder_desc(C.Structure):
 
 class ising_schedule(C.Structure):
     _fields_ = [("n_sweeps", C.c_int32), ("swap_every", C.c_int32), ("measure_every", C.c_int32),
-                ("winding_every", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+                ("winding_every", C.c_int32), ("use_graph", C.c_int32), ("winding_layer", C.c_int32)]
 
 
 class ising_series(C.Structure):

@@ -78,6 +78,7 @@ struct Rep {
     int32_t nblocks;
     int32_t q_first_block;  // first workgroup in the winding grid
     int32_t q_blocks_layer; // winding workgroups per z layer
+    int32_t l_first_block;  // first workgroup in the one-layer winding grid
     int32_t zbase;          // first entry of this slot in [R][Zmax] arrays
     uint32_t half;          // N/2
     int64_t off;            // float offset of the x component block
@@ -559,19 +560,28 @@ __device__ __forceinline__ f3 spin_at(const float* __restrict__ spins, const Rep
     return ld3(planes(spins, R, (int)k), idx);
 }
 
+// layer < 0: every layer of every replica (grid = qblock table); layer >= 0:
+// that layer only (grid = one-layer table, q_blocks_layer blocks per replica).
 __global__ __launch_bounds__(kBlock) void k_winding(
-    const float* __restrict__ spins, const Rep* __restrict__ reps, const int32_t* __restrict__ qblock_rep,
+    const float* __restrict__ spins, const Rep* __restrict__ reps, const int32_t* __restrict__ table, int layer,
     double* __restrict__ qpartial)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
-    const int slot = qblock_rep[blockIdx.x];
+    const int slot = table[blockIdx.x];
     const Rep R = reps[slot];
-    const uint32_t bi = (uint32_t)(blockIdx.x - R.q_first_block);
-    const uint32_t z = bi / (uint32_t)R.q_blocks_layer;
-    const uint32_t pl = (bi - z * (uint32_t)R.q_blocks_layer) * kBlock + threadIdx.x;
+    uint32_t z, chunk;
+    if (layer < 0) {
+        const uint32_t bi = (uint32_t)(blockIdx.x - R.q_first_block);
+        z = bi / (uint32_t)R.q_blocks_layer;
+        chunk = bi - z * (uint32_t)R.q_blocks_layer;
+    } else {
+        z = (uint32_t)layer;
+        chunk = (uint32_t)(blockIdx.x - R.l_first_block);
+    }
+    const uint32_t pl = chunk * kBlock + threadIdx.x;
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
     const uint32_t lx = (uint32_t)R.lx, ly = (uint32_t)R.ly;
-    if (pl < lx * ly) {
+    if (pl < lx * ly && z < (uint32_t)R.lz) {
         const uint32_t j = fdiv(pl, R.div_lx);
         const uint32_t i = pl - j * lx;
         const bool px = R.per & 1, py = (R.per >> 1) & 1;
@@ -591,21 +601,22 @@ __global__ __launch_bounds__(kBlock) void k_winding(
     if (threadIdx.x == 0) qpartial[blockIdx.x] = v[0];
 }
 
-// One workgroup per (slot, layer): fixed-order sum, Q = sum / (4 pi).
+// One workgroup per (slot, layer): fixed-order sum, Q = sum / (4 pi). In
+// one-layer mode (layer >= 0) the other layers read 0.
 __global__ __launch_bounds__(kBlock) void k_winding_finalize(
-    const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax,
+    const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax, int layer,
     double* __restrict__ qraw, long long* __restrict__ qint)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
     const int slot = blockIdx.x / zmax;
     const int z = blockIdx.x - slot * zmax;
     const Rep R = reps[slot];
-    if (z >= R.lz) {
+    if (z >= R.lz || (layer >= 0 && z != layer)) {
         if (threadIdx.x == 0) { qraw[blockIdx.x] = 0.0; qint[blockIdx.x] = 0; }
         return;
     }
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
-    const int base = R.q_first_block + z * R.q_blocks_layer;
+    const int base = layer < 0 ? R.q_first_block + z * R.q_blocks_layer : R.l_first_block;
     for (int j = threadIdx.x; j < R.q_blocks_layer; j += kBlock) v[0] += qpartial[base + j];
     block_sum4(v, lds);
     if (threadIdx.x == 0) {
@@ -784,10 +795,10 @@ struct ising_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     uint64_t seed = 0;
-    int R = 0, zmax = 0, n_blocks = 0, n_qblocks = 0;
+    int R = 0, zmax = 0, n_blocks = 0, n_qblocks = 0, n_lblocks = 0;
     std::vector<LadHost> ladders;
     std::vector<Rep> reps;
-    std::vector<int32_t> block_rep, qblock_rep;
+    std::vector<int32_t> block_rep, qblock_rep, lblock_rep;
     int64_t total_floats = 0;
     uint32_t host_sweep = 0;
     uint32_t tune = 0;           // sweep tuning flags (kTune*)
@@ -813,6 +824,7 @@ struct ising_ctx {
     Lad* d_lad_override = nullptr;
     int32_t* d_block_rep = nullptr;
     int32_t* d_qblock_rep = nullptr;
+    int32_t* d_lblock_rep = nullptr;
     float* d_beta_slot = nullptr;
     double* d_beta_temp = nullptr;
     int32_t* d_slot_of_temp = nullptr;
@@ -976,13 +988,18 @@ int launch_measure_all(ising_ctx* c)
     return ISING_OK;
 }
 
-int launch_winding_all(ising_ctx* c)
+int launch_winding_all(ising_ctx* c, int layer = -1)
 {
-    hipLaunchKernelGGL(k_winding, dim3(c->n_qblocks), dim3(kBlock), 0, c->stream,
-                       (const float*)c->d_spins, c->d_reps, c->d_qblock_rep, c->d_qpartial);
+    if (layer < 0) {
+        hipLaunchKernelGGL(k_winding, dim3(c->n_qblocks), dim3(kBlock), 0, c->stream,
+                           (const float*)c->d_spins, c->d_reps, (const int32_t*)c->d_qblock_rep, -1, c->d_qpartial);
+    } else {
+        hipLaunchKernelGGL(k_winding, dim3(c->n_lblocks), dim3(kBlock), 0, c->stream,
+                           (const float*)c->d_spins, c->d_reps, (const int32_t*)c->d_lblock_rep, layer, c->d_qpartial);
+    }
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_winding_finalize, dim3(c->R * c->zmax), dim3(kBlock), 0, c->stream,
-                       c->d_reps, c->d_qpartial, c->zmax, c->d_qraw, c->d_qint);
+                       c->d_reps, c->d_qpartial, c->zmax, layer, c->d_qraw, c->d_qint);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
@@ -1037,7 +1054,7 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
             LAUNCH_CHECK(c);
         }
         if (do_q && sd.qr) {
-            if ((rc = launch_winding_all(c))) return rc;
+            if ((rc = launch_winding_all(c, s.winding_layer - 1))) return rc;
             hipLaunchKernelGGL(k_record_q, dim3(1), dim3(kBlock), 0, c->stream,
                                (const double*)c->d_qraw, (const long long*)c->d_qint, (const int32_t*)c->d_slot_of_temp,
                                c->R, c->zmax, c->d_ctr + 2, sd.n_q, sd.qr, sd.qi);
@@ -1086,7 +1103,7 @@ void ising_destroy(ising_ctx* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* ptrs[] = { c->d_spins, c->d_reps, c->d_lads, c->d_lad_override, c->d_block_rep, c->d_qblock_rep,
+    void* ptrs[] = { c->d_spins, c->d_reps, c->d_lads, c->d_lad_override, c->d_block_rep, c->d_qblock_rep, c->d_lblock_rep,
                      c->d_beta_slot, c->d_beta_temp, c->d_slot_of_temp, c->d_lad_of_pos, c->d_lad_start,
                      c->d_lad_n, c->d_partial, c->d_energy, c->d_mag, c->d_qpartial, c->d_qraw, c->d_qint,
                      c->d_ctr, c->d_acc, c->d_acc_wave, c->d_wave_part, c->d_mlp, c->d_mlp_prm, c->d_mlp_m1,
@@ -1154,6 +1171,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
             r.q_first_block = qblk;
             r.q_blocks_layer = (r.lx * r.ly + kBlock - 1) / kBlock;
             qblk += r.q_blocks_layer * r.lz;
+            r.l_first_block = (int32_t)c->lblock_rep.size();
             r.div_hx = make_fastdiv((uint32_t)r.hx);
             r.div_ly = make_fastdiv((uint32_t)r.ly);
             r.div_lx = make_fastdiv((uint32_t)r.lx);
@@ -1161,12 +1179,14 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
             c->reps.push_back(r);
             for (int b = 0; b < r.nblocks; ++b) c->block_rep.push_back((int32_t)c->reps.size() - 1);
             for (int b = 0; b < r.q_blocks_layer * r.lz; ++b) c->qblock_rep.push_back((int32_t)c->reps.size() - 1);
+            for (int b = 0; b < r.q_blocks_layer; ++b) c->lblock_rep.push_back((int32_t)c->reps.size() - 1);
         }
     }
     for (auto& r : c->reps) r.zbase = 0;
     c->zmax = zmax;
     c->n_blocks = blk;
     c->n_qblocks = qblk;
+    c->n_lblocks = (int)c->lblock_rep.size();
     c->total_floats = off;
 
     int ndev = 0;
@@ -1184,6 +1204,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if ((rc = dalloc(c, &c->d_lad_override, 1))) return fail(rc);
     if ((rc = dalloc(c, &c->d_block_rep, c->block_rep.size()))) return fail(rc);
     if ((rc = dalloc(c, &c->d_qblock_rep, c->qblock_rep.size()))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_lblock_rep, c->lblock_rep.size()))) return fail(rc);
     if ((rc = dalloc(c, &c->d_beta_slot, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_beta_temp, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_slot_of_temp, R))) return fail(rc);
@@ -1229,6 +1250,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     UP(c->d_reps, c->reps.data(), c->reps.size());
     UP(c->d_block_rep, c->block_rep.data(), c->block_rep.size());
     UP(c->d_qblock_rep, c->qblock_rep.data(), c->qblock_rep.size());
+    UP(c->d_lblock_rep, c->lblock_rep.data(), c->lblock_rep.size());
     UP(c->d_beta_slot, beta_slot.data(), R);
     UP(c->d_beta_temp, beta_temp.data(), R);
     UP(c->d_slot_of_temp, slot_of_temp.data(), R);
@@ -1503,6 +1525,8 @@ int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* 
     const ising_schedule s = *sched;
     if (s.n_sweeps < 0 || s.swap_every < 0 || s.measure_every < 0 || s.winding_every < 0)
         return set_err(c, ISING_ERR_INVALID_ARG, "negative schedule entry");
+    if (s.winding_layer < 0 || s.winding_layer > c->zmax)
+        return set_err(c, ISING_ERR_INVALID_ARG, "winding_layer %d outside 0..%d", s.winding_layer, c->zmax);
     HIP_TRY(c, hipSetDevice(c->device));
     int rc;
     SeriesDev sd;

@@ -1,8 +1,9 @@
 // mlp.hpp -- the learned cone-width network on CDNA4 matrix cores.
 //
 // Network (per site): x[21] = six neighbour spins (+x,-x,+y,-y,+z,-z) and the
-// reduced field beta*h; z1 = W1 x + b1, h1 = silu(z1); z2 = W2 h1 + b2,
-// h2 = silu(z2); f = w3.h2 + b3; cone radius r = exp(clamp(f)). The input
+// reduced field beta*h; z1 = W1 x + b1, h1 = g(z1); z2 = W2 h1 + b2,
+// h2 = g(z2); f = w3.h2 + b3; cone radius r = exp(clamp(f)); g is SiLU in
+// base 2 (see silu below). The input
 // never contains the site's own spin, so the proposal radius is the same for
 // S -> S' and S' -> S and the Metropolis step stays exact.
 //
@@ -81,14 +82,17 @@ __device__ __forceinline__ v4u xor32(const v4u& v)
                 (uint32_t)__shfl_xor((int)v.z, 32, 64), (uint32_t)__shfl_xor((int)v.w, 32, 64) };
 }
 
+// The activation is SiLU in base 2, g(z) = z / (1 + 2^-z) = log2(e) silu(z ln 2):
+// a SiLU network with its weights rescaled by ln 2, so it represents the same
+// functions, and each activation needs no extra multiply before the exp2.
 __device__ __forceinline__ float silu(float z)
 {
-    return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
+    return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-z));
 }
 __device__ __forceinline__ float dsilu(float z)
 {
-    const float s = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
-    return s * (1.f + z * (1.f - s));
+    const float s = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-z));
+    return s * (1.f + z * 0.6931471805599453f * (1.f - s));
 }
 
 // Accumulator registers 8kc..8kc+7 as a bfloat16 B fragment.
@@ -144,7 +148,7 @@ __device__ __forceinline__ float mlp_forward(const MlpDev& P, const float (&x)[k
     float f[2];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-        f32x16 acc = {};
+        f32x16 acc = b1;           // the bias enters as the accumulator's initial value
 #pragma unroll
         for (int kc = 0; kc < 2; ++kc) {
             const bf16x8 bx = as_bf(sel4(m == g, own[kc], recv[kc]));
@@ -153,9 +157,9 @@ __device__ __forceinline__ float mlp_forward(const MlpDev& P, const float (&x)[k
         }
         f32x16 h;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) { acc[r] += b1[r]; h[r] = silu(acc[r]); }
+        for (int r = 0; r < 16; ++r) h[r] = silu(acc[r]);
         if (kTape) tape->z1[m] = acc;
-        f32x16 acc2 = {};
+        f32x16 acc2 = b2;
 #pragma unroll
         for (int kc = 0; kc < 2; ++kc) {
             const bf16x8 hb = chunk_bf16(h, kc);
@@ -164,7 +168,7 @@ __device__ __forceinline__ float mlp_forward(const MlpDev& P, const float (&x)[k
         }
         float part = 0.f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) { acc2[r] += b2[r]; part += w3[r] * silu(acc2[r]); }
+        for (int r = 0; r < 16; ++r) part += w3[r] * silu(acc2[r]);
         if (kTape) tape->z2[m] = acc2;
         f[m] = part + __shfl_xor(part, 32, 64) + P.b3;
     }

@@ -42,7 +42,8 @@ module ising_gpu_c
         end type ising_ladder_desc
 
         type, bind(C) :: ising_schedule
-                integer(c_int32_t) :: n_sweeps, swap_every, measure_every, winding_every, use_graph, reserved
+                integer(c_int32_t) :: n_sweeps, swap_every, measure_every, winding_every, use_graph
+                integer(c_int32_t) :: winding_layer   ! 0: every layer; z+1: layer z only
         end type ising_schedule
 
         type, bind(C) :: ising_series

@@ -1,7 +1,8 @@
 """Learned proposal: data generation, training and evaluation protocol.
 
 The network (include/ising_gpu.h, ising_amd/csrc/mlp.hpp) maps a site's six
-neighbour spins and beta*h to the log of its proposal radius. Protocol:
+neighbour spins and beta*h to the log of its proposal radius through two
+hidden layers of 32 units with SiLU in base 2, z / (1 + 2^-z). Protocol:
 
 1. run the plain fixed-radius sampler and collect records (neighbours,
    beta*h, S, h, beta, K) of random sites every `collect_every` sweeps;

@@ -8,10 +8,12 @@
  *
  *   input x[21]: the six neighbour spins (+x, -x, +y, -y, +z, -z; 3 each)
  *                followed by beta*h, the reduced local field (3);
- *   z1 = W1 x + b1,  h1 = silu(z1)        W1: 32 x 21
- *   z2 = W2 h1 + b2, h2 = silu(z2)        W2: 32 x 32
+ *   z1 = W1 x + b1,  h1 = g(z1)           W1: 32 x 21
+ *   z2 = W2 h1 + b2, h2 = g(z2)           W2: 32 x 32
  *   f  = w3 . h2 + b3                     log of the cone radius
  *   r  = exp(clamp(f, log_rmin, log_rmax))
+ * with g(z) = z / (1 + 2^-z), SiLU in base 2 (= log2(e) silu(z ln 2), the
+ * same function family as SiLU with weights rescaled by ln 2).
  *
  * Parameter vector (1793): W1 row-major [32][21], b1[32], W2 row-major
  * [32][32], b2[32], w3[32], b3.
@@ -53,7 +55,7 @@ static double bf16q(double v)
 }
 
 static double q(double v, int quantize) { return quantize ? bf16q(v) : v; }
-static double sigm(double z) { return 1.0 / (1.0 + exp(-z)); }
+static double sigm(double z) { return 1.0 / (1.0 + exp2(-z)); }
 
 typedef struct {
     const double *W1, *b1, *W2, *b2, *w3;
@@ -106,7 +108,7 @@ double orc_mlp_forward(const double* params, const double* x, int quantize)
 static double dsilu(double z)
 {
     const double s = sigm(z);
-    return s * (1.0 + z * (1.0 - s));
+    return s * (1.0 + z * M_LN2 * (1.0 - s));
 }
 
 /* Loss of one record and d loss / d f. rec: x[0..20], S[21..23], h[24..26],

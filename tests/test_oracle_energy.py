@@ -123,3 +123,23 @@ def test_reference_total_energy_matches_per_axis_form(orc):
     a = orc.ref_total_energy(L, J_r, Dz, B, s)
     b = orc.total_energy(L, orc.ham(-J_r, Dz, "interfacial_ref", B, 0.0, 2.0), s)
     assert a == pytest.approx(b, abs=1e-11)
+
+
+def test_magnetisation_closed_forms(orc):
+    L = orc.lattice(8, 8, 8)
+    s = np.zeros((3, 512)); s[2] = 1.0
+    assert np.array_equal(orc.magnetisation(L, s), [0.0, 0.0, 1.0])
+    h, _ = helix((8, 8, 8), 0, 1, (1, 2))            # full periods: M = 0
+    assert np.allclose(orc.magnetisation(L, h), 0.0, atol=1e-15)
+
+
+def test_initial_states(orc):
+    L = orc.lattice(32, 32, 16)
+    s = orc.init_random(L, 3, 1)
+    assert np.allclose(np.linalg.norm(s, axis=0), 1.0, atol=1e-15)
+    # uniform on the sphere: <z> = 0, <z^2> = 1/3, azimuth uniform
+    assert abs(s[2].mean()) < 0.01 and abs((s[2] ** 2).mean() - 1 / 3) < 0.01
+    assert abs(np.mean(np.arctan2(s[1], s[0]) > 0) - 0.5) < 0.01
+    f = orc.init_ferro(L, 3, 1, 0.1)
+    assert set(np.unique(f[2])) <= {-1.0, 1.0} and np.all(f[:2] == 0)
+    assert abs(np.mean(f[2] < 0) - 0.1) < 0.005

@@ -10,7 +10,8 @@ SL = {"W1": slice(0, 672), "b1": slice(672, 704), "W2": slice(704, 1728), "b2": 
 
 
 def silu(z):
-    return z / (1 + np.exp(-z))
+    """SiLU in base 2, the network's activation: z / (1 + 2^-z)."""
+    return z / (1 + np.exp2(-z))
 
 
 def random_params(rng, scale=0.4):
@@ -38,6 +39,11 @@ def random_records(rng, n):
 def test_zero_weights_give_bias(orc):
     p = np.zeros(NP); p[SL["b3"]] = -0.7
     assert orc.mlp_forward(p, np.ones(21), quantize=False) == -0.7
+
+
+def test_base2_silu_is_rescaled_silu():
+    z = np.linspace(-6, 6, 101)
+    assert np.allclose(silu(z), np.log2(np.e) * (z * np.log(2)) / (1 + np.exp(-z * np.log(2))))
 
 
 def test_single_path_closed_form(orc):
