@@ -8,9 +8,11 @@ replica-exchange round; E and M are recorded every 2 sweeps and the
 topological charge of the middle layer every 2, all on device inside one
 replayed hipGraph.
 
-Before the timed runs (untimed): 2000 plain sweeps equilibrate and collect
-training records, and the cone-width network is trained on them. Then the
-same schedule is timed twice: fixed cone r = 0.5, and learned cone.
+Before the timed runs (untimed): 2000 plain sweeps collect training records,
+the cone-width network is trained on them, 4000 more sweeps equilibrate and
+the state is checkpointed on the device. Then the same schedule is timed
+twice from that checkpoint, with the same random-number counters: fixed cone
+r = 0.5, and learned cone.
 
 value  = site updates/s of the fixed-cone run (every replica's sites x sweeps
 / wall time); also reported per sampler: tau_int-corrected independent
@@ -43,8 +45,9 @@ WINDING_EVERY = 2            # Q of the middle layer only
 TRAIN_SWEEPS = 2000          # plain sweeps that generate the training records
 COLLECT_EVERY = 20
 RECORDS_PER_REPLICA = 2048
-TRAIN_STEPS = 1000
-TRAIN_BATCH = 16384
+TRAIN_STEPS = 2000
+TRAIN_BATCH = 65536
+EQUIL_SWEEPS = 4000          # after training, learned sampler, before the checkpoint
 TRAIN_LR = 3e-3
 MLP_FLOPS_PER_UPDATE = 2 * (21 * 32 + 32 * 32 + 32)
 CONE = 0.5
@@ -124,11 +127,18 @@ def main() -> int:
     n_records = generate_data(eng, sweeps=TRAIN_SWEEPS, collect_every=COLLECT_EVERY, per_replica=RECORDS_PER_REPLICA,
                               swap_every=SWAP_EVERY)
     loss = train(eng, steps=TRAIN_STEPS, batch=TRAIN_BATCH, lr=TRAIN_LR)
+    # Further equilibration (the trained sampler relaxes the cold replicas
+    # fastest), then one device checkpoint that both timed runs start from
+    # with the same random-number counters.
+    eng.mlp_enable(True)
+    eng.pt_run(EQUIL_SWEEPS, SWAP_EVERY, 0, 0, use_graph=True)
+    eng.save_state()
     prep_s = time.perf_counter() - t0
 
     n_sweeps = args.steps * SWAP_EVERY
 
     def timed_run(learned: bool) -> dict:
+        eng.restore_state()
         eng.mlp_enable(learned)
         if args.warmup > 0:
             eng.pt_run(args.warmup * SWAP_EVERY, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True,

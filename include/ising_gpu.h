@@ -152,6 +152,13 @@ int  ising_winding_all(ising_ctx* ctx, int64_t* q, double* q_raw);
 int  ising_delta_energy(ising_ctx* ctx, int replica, const int32_t* sites,
                         const double* props, int n, double* de);
 
+/* Device-resident checkpoint of the chain state: spins, which slot sits at
+ * which temperature, the betas and the sweep / record / swap counters (so a
+ * restored run repeats the same random numbers). One checkpoint per context;
+ * the first save allocates it. */
+int  ising_save_state(ising_ctx* ctx);
+int  ising_restore_state(ising_ctx* ctx);
+
 /* ---- parallel tempering ---------------------------------------------- */
 /* Replaces the PT iteration loop src/PT.f90:219-273 (sweeps + swaps) plus
  * the per-temperature statistics the reference takes at the end. */

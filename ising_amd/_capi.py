@@ -84,6 +84,8 @@ SIGNATURES = {
     "ising_delta_energy": (C.c_int, [_P, C.c_int, _i32p, _dp, C.c_int, _dp]),
     "ising_pt_run": (C.c_int, [_P, C.POINTER(ising_schedule), C.POINTER(ising_series)]),
     "ising_temperature_map": (C.c_int, [_P, _i32p]),
+    "ising_save_state": (C.c_int, [_P]),
+    "ising_restore_state": (C.c_int, [_P]),
     "ising_swap_stats": (C.c_int, [_P, _i64p, _i64p]),
     "ising_accept_counts": (C.c_int, [_P, _i64p]),
     "ising_time_sweeps": (C.c_int, [_P, C.c_int, _dp]),

@@ -844,6 +844,9 @@ struct ising_ctx {
     unsigned long long* d_swap_att = nullptr;
     unsigned long long* d_swap_acc = nullptr;
     int32_t* d_sites = nullptr; float* d_props = nullptr; double* d_de = nullptr; int de_cap = 0;
+    // checkpoint (ising_save_state)
+    float* ck_spins = nullptr; int32_t* ck_sot = nullptr; float* ck_bslot = nullptr; double* ck_btemp = nullptr;
+    uint32_t* ck_ctr = nullptr; uint32_t ck_host_sweep = 0; bool ck_valid = false;
 };
 
 namespace {
@@ -1107,7 +1110,8 @@ void ising_destroy(ising_ctx* c)
                      c->d_beta_slot, c->d_beta_temp, c->d_slot_of_temp, c->d_lad_of_pos, c->d_lad_start,
                      c->d_lad_n, c->d_partial, c->d_energy, c->d_mag, c->d_qpartial, c->d_qraw, c->d_qint,
                      c->d_ctr, c->d_acc, c->d_acc_wave, c->d_wave_part, c->d_mlp, c->d_mlp_prm, c->d_mlp_m1,
-                     c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de };
+                     c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de,
+                     c->ck_spins, c->ck_sot, c->ck_bslot, c->ck_btemp, c->ck_ctr };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1603,6 +1607,42 @@ int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* 
         if (e1 != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "series copy-out"); }
     }
     cleanup();
+    return ISING_OK;
+}
+
+int ising_save_state(ising_ctx* c)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc;
+    if (!c->ck_spins) {
+        if ((rc = dalloc(c, &c->ck_spins, (size_t)c->total_floats)) || (rc = dalloc(c, &c->ck_sot, c->R)) ||
+            (rc = dalloc(c, &c->ck_bslot, c->R)) || (rc = dalloc(c, &c->ck_btemp, c->R)) || (rc = dalloc(c, &c->ck_ctr, 4)))
+            return rc;
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->ck_spins, c->d_spins, (size_t)c->total_floats * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->ck_sot, c->d_slot_of_temp, c->R * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->ck_bslot, c->d_beta_slot, c->R * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->ck_btemp, c->d_beta_temp, c->R * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->ck_ctr, c->d_ctr, 4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->ck_host_sweep = c->host_sweep;
+    c->ck_valid = true;
+    return ISING_OK;
+}
+
+int ising_restore_state(ising_ctx* c)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (!c->ck_valid) return set_err(c, ISING_ERR_STATE, "no saved state: call ising_save_state first");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->d_spins, c->ck_spins, (size_t)c->total_floats * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_slot_of_temp, c->ck_sot, c->R * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_beta_slot, c->ck_bslot, c->R * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_beta_temp, c->ck_btemp, c->R * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_ctr, c->ck_ctr, 4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->host_sweep = c->ck_host_sweep;
     return ISING_OK;
 }
 

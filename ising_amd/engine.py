@@ -265,6 +265,13 @@ class Engine:
         self._check(self._lib.ising_pt_run(self._h, C.byref(sch), C.byref(ser)))
         return out
 
+    def save_state(self) -> None:
+        """Device-resident checkpoint: spins, temperature map, betas, counters."""
+        self._check(self._lib.ising_save_state(self._h))
+
+    def restore_state(self) -> None:
+        self._check(self._lib.ising_restore_state(self._h))
+
     def temperature_map(self) -> np.ndarray:
         m = np.empty(self.n_replicas, dtype=np.int32)
         self._check(self._lib.ising_temperature_map(self._h, m.ctypes.data_as(C.POINTER(C.c_int32))))

@@ -208,6 +208,16 @@ module ising_gpu_c
                         type(ising_series), intent(in) :: series
                         integer(c_int) :: rc
                 end function
+                function ising_save_state(ctx) bind(C, name="ising_save_state") result(rc)
+                        import :: c_ptr, c_int
+                        type(c_ptr), value :: ctx
+                        integer(c_int) :: rc
+                end function
+                function ising_restore_state(ctx) bind(C, name="ising_restore_state") result(rc)
+                        import :: c_ptr, c_int
+                        type(c_ptr), value :: ctx
+                        integer(c_int) :: rc
+                end function
                 function ising_temperature_map(ctx, slot_of_temp) bind(C, name="ising_temperature_map") result(rc)
                         import :: c_ptr, c_int, c_int32_t
                         type(c_ptr), value :: ctx

@@ -319,3 +319,22 @@ def test_fused_measurement_matches_exact(lname, hname):
             scale = np.maximum(np.abs(exact), lat.sites)
             assert np.all(np.abs(o["energy"][-1] - exact) <= 1e-6 * scale)
             assert np.allclose(o["magnetisation"][-1], mag, atol=1e-6)
+
+
+def test_checkpoint_restore_replays_exactly():
+    """ising_save_state / ising_restore_state: a restored run repeats the
+    same random numbers, swaps and records bit for bit."""
+    lat = Lattice(12, 12, 8)
+    betas = 1.0 / geometric_temperatures(0.4, 2.0, 4)
+    with engine(lat, HAMS["bulk_K"], betas) as e:
+        e.init_random()
+        e.pt_run(n_sweeps=30, swap_every=5, measure_every=0)
+        e.save_state()
+        a = e.pt_run(n_sweeps=40, swap_every=5, measure_every=2, winding_every=10)
+        sa, ma = e.get_spins(2), e.temperature_map()
+        e.restore_state()
+        b = e.pt_run(n_sweeps=40, swap_every=5, measure_every=2, winding_every=10)
+        sb, mb = e.get_spins(2), e.temperature_map()
+    assert np.array_equal(sa, sb) and np.array_equal(ma, mb)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
