@@ -213,7 +213,8 @@ def main() -> int:
                      "kernel": "k_half_sweep", "ms_per_launch": fixed["ms_per_launch"],
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "copy_probe_GBs": 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9,
-                     "tuning_flags": tune_flags, "tuning_ms_per_launch": [float(x) for x in tune_ms],
+                     "tuning_flags": tune_flags,
+                     "tuning_ms_per_half_sweep": {str(f): float(t) for f, t in tune_ms.items()},
                      "learned_kernel_mfma_TFLOPs": mlp_tflops, "mfma_bf16_peak_TFLOPs": 2500.0},
         "cpu_baseline": cpu,
     }

@@ -174,9 +174,10 @@ int  ising_swap_stats(ising_ctx* ctx, int64_t* attempts, int64_t* accepts);
 int  ising_accept_counts(ising_ctx* ctx, int64_t* accepted);
 
 /* ---- measurement (no reference counterpart) --------------------------- */
-/* Runs n_sweeps sweeps like ising_sweep with one hipEvent pair around the
- * 2*n_sweeps back-to-back half-sweep launches on the context's stream;
- * *ms_per_launch = elapsed / (2*n_sweeps). */
+/* Runs n_sweeps sweeps like ising_sweep with one hipEvent pair around their
+ * back-to-back launches on the context's stream (two half-sweep launches per
+ * sweep, or one fused launch); *ms_per_launch = elapsed / (2*n_sweeps), the
+ * time per half sweep (one update of every site of one colour). */
 int  ising_time_sweeps(ising_ctx* ctx, int n_sweeps, double* ms_per_launch);
 /* hipDeviceSynchronize on the context's device. */
 int  ising_synchronize(ising_ctx* ctx);
@@ -187,13 +188,25 @@ int  ising_synchronize(ising_ctx* ctx);
 int  ising_bandwidth_probe(ising_ctx* ctx, int64_t bytes, double* ms);
 
 /* ---- launch tuning (no reference counterpart) ------------------------- */
+/* Every flag set gives bit-identical chains; only the launch shape changes. */
 #define ISING_TUNE_XCD_REMAP 1u  /* contiguous block ranges per accelerator die */
-#define ISING_TUNE_NT_OWN    2u  /* non-temporal access to the updated colour    */
+#define ISING_TUNE_NT_OWN    2u  /* non-temporal access to the updated colour (half sweeps) */
+/* One fused launch per sweep instead of two half-sweep launches: each
+ * workgroup stages whole z planes in LDS and updates both colours of lc
+ * consecutive planes (1..255); ISING_ERR_UNSUPPORTED when a plane of the
+ * largest replica does not fit in LDS. */
+#define ISING_TUNE_FUSED_SHIFT 8
+#define ISING_TUNE_FUSED(lc) ((uint32_t)(lc) << ISING_TUNE_FUSED_SHIFT)
+#define ISING_TUNE_MAX_CANDIDATES 16
 int  ising_set_tuning(ising_ctx* ctx, uint32_t flags);
 int  ising_get_tuning(const ising_ctx* ctx, uint32_t* flags);
-/* Times every flag combination over n_sweeps real sweeps each (they advance
- * the chains like ising_sweep) and keeps the fastest; ms[4] (may be NULL)
- * receives the time per half-sweep launch of flags 0..3. */
+/* The flag sets ising_autotune times, in order (at most
+ * ISING_TUNE_MAX_CANDIDATES); flags may be NULL to query the count. */
+int  ising_tuning_candidates(const ising_ctx* ctx, uint32_t* flags, int* count);
+/* Times every candidate over n_sweeps real sweeps each (they advance the
+ * chains like ising_sweep) and keeps the fastest; ms (may be NULL, room for
+ * ISING_TUNE_MAX_CANDIDATES) receives the time per half sweep of each
+ * candidate in ising_tuning_candidates order. */
 int  ising_autotune(ising_ctx* ctx, int n_sweeps, uint32_t* best, double* ms);
 
 /* ---- learned proposal (no reference counterpart) ---------------------- */

@@ -26,6 +26,16 @@ ERRORS = {
 DMI_NONE, DMI_BULK, DMI_INTERFACIAL_REF = 0, 1, 2
 MLP_NPARAM = 1793
 MLP_RECORD = 32
+TUNE_XCD_REMAP = 1
+TUNE_NT_OWN = 2
+TUNE_FUSED_SHIFT = 8
+TUNE_MAX_CANDIDATES = 16
+
+
+def tune_fused(planes: int) -> int:
+    """Tuning flag: one fused launch per sweep, `planes` z planes per workgroup."""
+    return int(planes) << TUNE_FUSED_SHIFT
+
 
 
 class ising_lattice(C.Structure):
@@ -94,6 +104,7 @@ SIGNATURES = {
     "ising_set_tuning": (C.c_int, [_P, C.c_uint32]),
     "ising_get_tuning": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "ising_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint32), _dp]),
+    "ising_tuning_candidates": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
     "ising_mlp_set_params": (C.c_int, [_P, _fp, C.c_int]),
     "ising_mlp_get_params": (C.c_int, [_P, _fp, C.c_int]),
     "ising_mlp_enable": (C.c_int, [_P, C.c_int, C.c_int]),

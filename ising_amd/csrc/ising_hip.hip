@@ -39,6 +39,8 @@ constexpr int kXcds = 8;          // MI355X: 8 accelerator dies, workgroups deal
 // Sweep tuning flags (bit mask, chosen by ising_autotune).
 constexpr uint32_t kTuneXcdRemap = 1u;     // give each die a contiguous range of blocks
 constexpr uint32_t kTuneNtOwn = 2u;        // non-temporal loads/stores of the updated colour
+constexpr uint32_t kTuneFusedShift = 8;    // bits 8..15: z planes per fused-sweep workgroup (0: half sweeps)
+constexpr uint32_t kTuneFusedMask = 0xFFu << kTuneFusedShift;
 
 // Logical workgroup index: with kTuneXcdRemap the blocks a die receives
 // (b mod 8 == die) map to one contiguous range, so neighbouring rows of a
@@ -53,20 +55,30 @@ __device__ __forceinline__ uint32_t logical_block(uint32_t b, uint32_t n, uint32
 // ---------------------------------------------------------------------------
 // Division by an invariant 32-bit divisor (Granlund & Montgomery, PLDI'94,
 // fig. 4.1): n / d = (t + ((n - t) >> s1)) >> s2 with t = mulhi(n, m).
+// A power-of-two divisor is stored with m = 0 and s2 = log2(d): the quotient
+// is a shift and the remainder a mask (a wavefront-uniform branch).
 struct FastDiv { uint32_t m, s1, s2, d; };
 
 static FastDiv make_fastdiv(uint32_t d)
 {
     uint32_t l = 0;
     while ((1ull << l) < d) ++l;
+    if ((1ull << l) == d) return FastDiv{ 0u, 0u, l, d };
     uint64_t m = (((1ull << 32) * ((1ull << l) - d)) / d) + 1;
     return FastDiv{ (uint32_t)m, l ? 1u : 0u, l ? l - 1u : 0u, d };
 }
 
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f)
 {
+    if (f.m == 0u) return n >> f.s2;
     uint32_t t = __umulhi(n, f.m);
     return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+// n - quot * d for quot = fdiv(n, f).
+__device__ __forceinline__ uint32_t fmodq(uint32_t n, uint32_t quot, const FastDiv& f)
+{
+    return f.m == 0u ? (n & (f.d - 1u)) : n - quot * f.d;
 }
 
 // Per replica storage slot.
@@ -80,6 +92,9 @@ struct Rep {
     int32_t q_blocks_layer; // winding workgroups per z layer
     int32_t l_first_block;  // first workgroup in the one-layer winding grid
     int32_t zbase;          // first entry of this slot in [R][Zmax] arrays
+    int32_t f_first_block;  // first workgroup in the fused-sweep grid
+    int32_t f_nblocks;      // fused-sweep workgroups (z chunks)
+    int32_t plane;          // ly*hx: sites of one colour on one z plane
     uint32_t half;          // N/2
     int64_t off;            // float offset of the x component block
     int64_t n;              // N
@@ -89,6 +104,7 @@ struct Rep {
 // Per ladder Hamiltonian, FP32 copy for sweeps and FP64 copy for energies.
 struct Lad {
     float J, Bz, Kz, r;
+    float lr8;              // log2(r) - 8 (-inf for r = 0)
     float zsf;              // Zeeman coefficient of total energies
     float d[9];
     int32_t dmi;
@@ -128,12 +144,6 @@ __device__ __forceinline__ f3 ld3(const Planes& P, uint32_t i)
 {
     return f3{ ldf(P.x, i), ldf(P.y, i), ldf(P.z, i) };
 }
-__device__ __forceinline__ f3 ld3c(const Planes& P, uint32_t i, bool ok)
-{
-    f3 v{ 0.f, 0.f, 0.f };
-    if (ok) v = ld3(P, i);
-    return v;
-}
 
 __device__ __forceinline__ Planes planes(const float* spins, const Rep& R, int colour)
 {
@@ -141,47 +151,54 @@ __device__ __forceinline__ Planes planes(const float* spins, const Rep& R, int c
     return Planes{ b, b + R.n, b + 2 * R.n };
 }
 
-// Neighbour indices (inside the other colour's arrays) of the site at half
-// index h of row (iy, iz), parity p = ix & 1. ok[] false across open faces.
-// kPer: every axis periodic (wraps become selects, every neighbour exists).
+// Neighbour indices (inside the other colour's arrays, which share the
+// (row, half index) layout) of the site at q = row*hx + h on row (iy, iz),
+// parity p = ix & 1: q plus a fixed step per direction, corrected by one
+// period at a periodic face. ok[] false across open faces.
+// kPer: every axis periodic (every neighbour exists).
 struct Nbr { uint32_t i[6]; bool ok[6]; };
 
 template <bool kPer>
-__device__ __forceinline__ Nbr neighbours(const Rep& R, uint32_t row, uint32_t h, uint32_t iy, uint32_t iz, uint32_t p)
+__device__ __forceinline__ Nbr neighbours(const Rep& R, uint32_t q, uint32_t h, uint32_t iy, uint32_t iz, uint32_t p)
 {
     Nbr nb;
     const uint32_t hx = (uint32_t)R.hx, ly = (uint32_t)R.ly, lz = (uint32_t)R.lz;
+    const uint32_t plane = ly * hx, vol = R.half;
     const bool px = kPer || (R.per & 1), py = kPer || ((R.per >> 1) & 1), pz = kPer || ((R.per >> 2) & 1);
-    const uint32_t rb = row * hx;
-    // +x: ix+1 has half index h+p unless it wraps (p=1, h=hx-1).
-    const bool xwrap_p = (p == 1u) && (h == hx - 1u);
-    nb.i[0] = xwrap_p ? rb : rb + h + p;
-    nb.ok[0] = !xwrap_p || px;
-    // -x: ix-1 has half index h+p-1 unless it wraps (p=0, h=0).
-    const bool xwrap_m = (p == 0u) && (h == 0u);
-    nb.i[1] = xwrap_m ? rb + hx - 1u : rb + h + p - 1u;
-    nb.ok[1] = !xwrap_m || px;
-    // +-y: same half index on row iy+-1.
+    // +x: half index h+p; ix+1 wraps to 0 when p = 1 and h = hx-1.
+    const bool xp_w = (p != 0u) && (h == hx - 1u);
+    nb.i[0] = q + p - (xp_w ? hx : 0u);
+    nb.ok[0] = !xp_w || px;
+    // -x: half index h+p-1; ix-1 wraps to lx-1 when p = 0 and h = 0.
+    const bool xm_w = (p == 0u) && (h == 0u);
+    nb.i[1] = q + p - 1u + (xm_w ? hx : 0u);
+    nb.ok[1] = !xm_w || px;
     const bool yp_w = (iy + 1u == ly), ym_w = (iy == 0u);
-    nb.i[2] = (yp_w ? row - (ly - 1u) : row + 1u) * hx + h;
+    nb.i[2] = q + hx - (yp_w ? plane : 0u);
     nb.ok[2] = !yp_w || py;
-    nb.i[3] = (ym_w ? row + (ly - 1u) : row - 1u) * hx + h;
+    nb.i[3] = q - hx + (ym_w ? plane : 0u);
     nb.ok[3] = !ym_w || py;
-    // +-z: same half index on row (iz+-1, iy).
     const bool zp_w = (iz + 1u == lz), zm_w = (iz == 0u);
-    nb.i[4] = (zp_w ? row - (lz - 1u) * ly : row + ly) * hx + h;
+    nb.i[4] = q + plane - (zp_w ? vol : 0u);
     nb.ok[4] = !zp_w || pz;
-    nb.i[5] = (zm_w ? row + (lz - 1u) * ly : row - ly) * hx + h;
+    nb.i[5] = q - plane + (zm_w ? vol : 0u);
     nb.ok[5] = !zm_w || pz;
     return nb;
 }
 
 // The six neighbour spins (+x, -x, +y, -y, +z, -z); zero across open faces.
-template <bool kPer>
-__device__ __forceinline__ void load_nbrs(const Planes& oth, const Nbr& nb, f3 (&v)[6])
+// Every index of neighbours() is in range (an open face yields the periodic
+// image), so all lanes load unconditionally in one straight-line block and
+// open lattices clear the absent neighbours afterwards (a uniform branch).
+__device__ __forceinline__ void load_nbrs(const Rep& R, const Planes& oth, const Nbr& nb, f3 (&v)[6])
 {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = kPer ? ld3(oth, nb.i[k]) : ld3c(oth, nb.i[k], nb.ok[k]);
+    for (int k = 0; k < 6; ++k) v[k] = ld3(oth, nb.i[k]);
+    if (R.per != 7) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (!nb.ok[k]) v[k] = f3{ 0.f, 0.f, 0.f };
+    }
 }
 
 // Local field h = J sum_j S_j - sum_a (S_{+a} - S_{-a}) x d_a + Bz z.
@@ -207,11 +224,10 @@ __device__ __forceinline__ f3 field_from(const Lad& L, const f3 (&v)[6])
     return f3{ L.J * sum.x - dm.x, L.J * sum.y - dm.y, L.J * sum.z - dm.z + L.Bz };
 }
 
-template <bool kPer>
-__device__ __forceinline__ f3 local_field(const Lad& L, const Planes& oth, const Nbr& nb)
+__device__ __forceinline__ f3 local_field(const Rep& R, const Lad& L, const Planes& oth, const Nbr& nb)
 {
     f3 v[6];
-    load_nbrs<kPer>(oth, nb, v);
+    load_nbrs(R, oth, nb, v);
     return field_from(L, v);
 }
 
@@ -232,26 +248,106 @@ __device__ __forceinline__ double wave_sum(double v)
 
 
 // ---------------------------------------------------------------------------
-// Half-sweep: one Metropolis proposal for every site of one colour of every
-// replica. Proposal: u uniform in a ball of radius r (cos(theta) = 2U-1,
-// phi = 2 pi U', radius r U''^(1/3)), S' = (S+u)/|S+u|; accept iff
-// U''' < exp(-beta dE). One Philox block supplies the four uniforms.
+// Two spin buffers of equal layout. Half-sweep launches update the current
+// buffer in place; a fused sweep reads the current buffer and writes every
+// site to the other one, which then becomes current. The current buffer is
+// the parity of a device counter of fused sweeps, so launches captured in a
+// graph stay valid on every replay.
+struct SpinBuf {
+    float* base;
+    int64_t total;          // floats per buffer
+    const uint32_t* ctr;    // device counter of fused sweeps
+};
+
+__device__ __forceinline__ float* spin_buffer(const SpinBuf& b, uint32_t fused_done)
+{
+    return b.base + (int64_t)(fused_done & 1u) * b.total;
+}
+
+// Site number (ix*ly + iy)*lz + iz of the Philox counter, in 24-bit
+// multiplies when the replica has fewer than 2^24 sites.
+__device__ __forceinline__ uint32_t site_number(const Rep& R, uint32_t ix, uint32_t iy, uint32_t iz)
+{
+    return R.n < (1 << 24) ? __umul24(__umul24(ix, (uint32_t)R.ly) + iy, (uint32_t)R.lz) + iz
+                           : (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
+}
+
+// One Metropolis proposal for a site with spin s, neighbour spins nbr and
+// local field hf. Proposal: u uniform in a ball of radius r (cos(theta) =
+// 2U1-1, phi = 2 pi U2, radius r U3^(1/3)), S' = (S+u)/|S+u|; accept iff
+// U4 < exp(-beta dE). One Philox block keyed by (seed, slot) with counter
+// (site, sweep) supplies the four uniforms. Returns the spin after the
+// decision.
 //
 // kLearned: ladders with the learned flag take r from the network of
 // mlp.hpp evaluated on the six neighbour spins and beta*h (never the site's
 // own spin), so r is the same for S -> S' and S' -> S. Every lane of the
-// wavefront computes (lanes past the end of a replica on a clamped valid
-// site) because the network exchanges data across lanes; only in-range
-// lanes store or count.
+// wavefront must call this (lanes past the end of a replica on a clamped
+// valid site, active = false) because the network exchanges data across
+// lanes.
+template <bool kLearned>
+__device__ __forceinline__ f3 metropolis_site(const Lad& L, float beta, const f3& s, const f3 (&nbr)[6], const f3& hf,
+                                             uint32_t site, uint32_t sweep, uint32_t seed_lo, uint32_t seed_hi,
+                                             uint32_t slot, const MlpDev* __restrict__ mlp, bool active, bool& accepted)
+{
+    // log2(radius) - 8: the radius enters the proposal as
+    // r U3^(1/3) = 2^(log2(x)/3 + log2(r) - 8) for U3 = x 2^-24.
+    float lr8 = L.lr8;
+    if constexpr (kLearned) {
+        if (L.learned) {      // uniform per workgroup (one replica)
+            const float x[kMlpIn] = { nbr[0].x, nbr[0].y, nbr[0].z, nbr[1].x, nbr[1].y, nbr[1].z,
+                                      nbr[2].x, nbr[2].y, nbr[2].z, nbr[3].x, nbr[3].y, nbr[3].z,
+                                      nbr[4].x, nbr[4].y, nbr[4].z, nbr[5].x, nbr[5].y, nbr[5].z,
+                                      beta * hf.x, beta * hf.y, beta * hf.z };
+            const float f = mlp_forward<false>(*mlp, x, nullptr);
+            lr8 = fmaf(clamp_log_radius(f), 1.4426950408889634f, -8.f);
+        }
+    }
+    const u32x4 rnd = philox4x32_10(u32x4{ site, sweep, kStreamMetropolis, seed_hi }, seed_lo, slot);
+    const float ct = s24f(rnd.x);
+    const float uphi = u24f(rnd.y);
+    const float uacc = u24f(rnd.w);
+    const float st = __builtin_amdgcn_sqrtf(fmaxf(0.f, (1.f - ct) * (1.f + ct)));
+    const float sp = __builtin_amdgcn_sinf(uphi);   // sin(2 pi uphi)
+    const float cp = __builtin_amdgcn_cosf(uphi);   // cos(2 pi uphi)
+    const float rr = __builtin_amdgcn_exp2f(fmaf(__builtin_amdgcn_logf((float)(rnd.z >> 8)), 1.f / 3.f, lr8));
+    const f3 t{ s.x + rr * st * cp, s.y + rr * st * sp, s.z + rr * ct };
+    const float n2 = t.x * t.x + t.y * t.y + t.z * t.z;
+    accepted = false;
+    f3 fs = s;
+    if (active && n2 > 1e-30f) {
+        const float inv = __builtin_amdgcn_rsqf(n2);
+        const f3 sn{ t.x * inv, t.y * inv, t.z * inv };
+        const float de = delta_e(s, sn, hf, L.Kz);
+        const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
+        if (uacc < w) {
+            accepted = true;
+            fs = sn;
+        }
+    }
+    return fs;
+}
+
+// Energy of a colour-1 site after the sweep and of its on-site partner: its
+// six bonds, -S.(h - Bz z) with h formed from the updated colour-0
+// neighbours (each bond has exactly one colour-1 end), plus the on-site
+// terms of itself and of the colour-0 site c0 at the same half index.
+__device__ __forceinline__ float pair_energy(const Lad& L, const f3& fs, const f3& hf, const f3& c0)
+{
+    return -(fs.x * hf.x + fs.y * hf.y + fs.z * (hf.z - L.Bz))
+           - L.zsf * L.Bz * (fs.z + c0.z) - L.Kz * (fs.z * fs.z + c0.z * c0.z);
+}
+
+// ---------------------------------------------------------------------------
+// Half-sweep: one Metropolis proposal for every site of one colour of every
+// replica, one thread per site, in place in the current buffer (accepted
+// moves are stored).
 //
-// kMeasure (colour-1 launches only): after its update every colour-1 site adds
-// its six bonds, -S.(h - Bz z) with the field h already formed (each bond has
-// exactly one colour-1 end), plus the on-site terms and spins of itself and of
-// the colour-0 site at the same half index; per-site FP32, summed in FP64 per
-// wavefront into wave_part[wave][4] = (E, Mx, My, Mz).
+// kMeasure (colour-1 launches only): per-site pair_energy and spins in FP32,
+// summed in FP64 per wavefront into wave_part[wave][4] = (E, Mx, My, Mz).
 template <bool kMeasure, bool kLearned>
 __global__ __launch_bounds__(kBlock) void k_half_sweep(
-    float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    SpinBuf sb, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ block_rep,
     const uint32_t* __restrict__ sweep_ctr, uint32_t sweep_off, int colour,
     uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count,
@@ -263,6 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
     const Lad L = lads[R.ladder];
     const float beta = beta_slot[slot];
     const uint32_t sweep = *sweep_ctr + sweep_off;
+    float* spins = spin_buffer(sb, *sb.ctr);
     const uint32_t q0 = (blk - (uint32_t)R.first_block) * kBlock + threadIdx.x;
     const bool active = q0 < R.half;
     const uint32_t q = active ? q0 : R.half - 1u;
@@ -271,68 +368,30 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
     const Planes own = planes(spins, R, colour);
     const Planes oth = planes(spins, R, colour ^ 1);
     const uint32_t row = fdiv(q, R.div_hx);
-    const uint32_t h = q - row * (uint32_t)R.hx;
+    const uint32_t h = fmodq(q, row, R.div_hx);
     const uint32_t iz = fdiv(row, R.div_ly);
-    const uint32_t iy = row - iz * (uint32_t)R.ly;
+    const uint32_t iy = fmodq(row, iz, R.div_ly);
     const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
     const uint32_t ix = 2u * h + p;
     const f3 s = nt ? f3{ ldf_nt(own.x, q), ldf_nt(own.y, q), ldf_nt(own.z, q) } : ld3(own, q);
     f3 nbr[6];
-    if (R.per == 7) load_nbrs<true>(oth, neighbours<true>(R, row, h, iy, iz, p), nbr);
-    else load_nbrs<false>(oth, neighbours<false>(R, row, h, iy, iz, p), nbr);
+    load_nbrs(R, oth, neighbours<false>(R, q, h, iy, iz, p), nbr);
     const f3 hf = field_from(L, nbr);
-
-    float radius = L.r;
-    if constexpr (kLearned) {
-        if (L.learned) {      // uniform per workgroup (one replica)
-            const float x[kMlpIn] = { nbr[0].x, nbr[0].y, nbr[0].z, nbr[1].x, nbr[1].y, nbr[1].z,
-                                      nbr[2].x, nbr[2].y, nbr[2].z, nbr[3].x, nbr[3].y, nbr[3].z,
-                                      nbr[4].x, nbr[4].y, nbr[4].z, nbr[5].x, nbr[5].y, nbr[5].z,
-                                      beta * hf.x, beta * hf.y, beta * hf.z };
-            const float f = mlp_forward<false>(*mlp, x, nullptr);
-            radius = __builtin_amdgcn_exp2f(clamp_log_radius(f) * 1.4426950408889634f);
-        }
-    }
-
-    const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
-    const u32x4 rnd = philox4x32_10(u32x4{ site, sweep, kStreamMetropolis, seed_hi }, seed_lo, (uint32_t)slot);
-    const float ct = s24f(rnd.x);
-    const float uphi = u24f(rnd.y);
-    const float urad = u24f(rnd.z);
-    const float uacc = u24f(rnd.w);
-    const float st = sqrtf(fmaxf(0.f, (1.f - ct) * (1.f + ct)));
-    const float sp = __builtin_amdgcn_sinf(uphi);   // sin(2 pi uphi)
-    const float cp = __builtin_amdgcn_cosf(uphi);   // cos(2 pi uphi)
-    const float rr = radius * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(urad) * (1.f / 3.f));
-    const f3 t{ s.x + rr * st * cp, s.y + rr * st * sp, s.z + rr * ct };
-    const float n2 = t.x * t.x + t.y * t.y + t.z * t.z;
-    bool accepted = false;
-    f3 fs = s;                 // spin after the decision
-    if (active && n2 > 1e-30f) {
-        const float inv = __builtin_amdgcn_rsqf(n2);
-        const f3 sn{ t.x * inv, t.y * inv, t.z * inv };
-        const float de = delta_e(s, sn, hf, L.Kz);
-        const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
-        if (uacc < w) {
-            if (nt) {
-                stf_nt((float*)own.x, q, sn.x);
-                stf_nt((float*)own.y, q, sn.y);
-                stf_nt((float*)own.z, q, sn.z);
-            } else {
-                stf((float*)own.x, q, sn.x);
-                stf((float*)own.y, q, sn.y);
-                stf((float*)own.z, q, sn.z);
-            }
-            accepted = true;
-            fs = sn;
+    bool accepted;
+    const f3 fs = metropolis_site<kLearned>(L, beta, s, nbr, hf, site_number(R, ix, iy, iz), sweep, seed_lo, seed_hi,
+                                            (uint32_t)slot, mlp, active, accepted);
+    if (accepted) {
+        if (nt) {
+            stf_nt((float*)own.x, q, fs.x); stf_nt((float*)own.y, q, fs.y); stf_nt((float*)own.z, q, fs.z);
+        } else {
+            stf((float*)own.x, q, fs.x); stf((float*)own.y, q, fs.y); stf((float*)own.z, q, fs.z);
         }
     }
     if constexpr (kMeasure) {
         float me = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
         if (active) {
             const f3 c0 = ld3(planes(spins, R, 0), q);
-            me = -(fs.x * hf.x + fs.y * hf.y + fs.z * (hf.z - L.Bz))
-                 - L.zsf * L.Bz * (fs.z + c0.z) - L.Kz * (fs.z * fs.z + c0.z * c0.z);
+            me = pair_energy(L, fs, hf, c0);
             mx = fs.x + c0.x; my = fs.y + c0.y; mz = fs.z + c0.z;
         }
         double v[4] = { me, mx, my, mz };
@@ -350,6 +409,206 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
         unsigned long long* c = acc_count + (size_t)blk * (kBlock / kWave) + threadIdx.x / kWave;
         *c += (unsigned long long)__popcll(ball);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fused sweep over LDS-staged planes: one workgroup per (replica, chunk of
+// lc z planes [z0, z1)) updates colour 0 and then colour 1 of its planes in
+// one pass. A z plane of one colour (P = ly*hx sites) is staged whole in LDS,
+// so x and y neighbours never leave it; the workgroup marches in z with two
+// rings of three planes:
+//   ring 0: colour-1 spins as they were before the sweep (planes s-1, s, s+1),
+//   ring 1: colour-0 spins after their update (planes s-2, s-1, s).
+// Step s updates colour 0 on plane s from ring 0, then (after a barrier)
+// colour 1 on plane s-1 from ring 1, and moves the colour-1 plane s+2, loaded
+// from HBM at the start of the step, into the ring-0 slot of plane s-1. The
+// halo planes z0-1 and z1 of colour 0 are recomputed from the same Philox
+// counters as their owners', so results equal the two half-sweep launches bit
+// for bit; every site of [z0, z1) is written to the destination buffer, and
+// halos of neighbouring chunks read only the source buffer.
+// Each thread keeps the same sites (k = j*blockDim + tid) on every plane, so
+// the ring-0 slot it reads its own colour-1 spin from in step s is the slot
+// it refills, at the same positions, in the same step.
+// kMeasure: per-thread FP64 sums of pair_energy and spins over the owned
+// colour-1 sites, then per wavefront into wave_part.
+constexpr int kFusedMaxThreads = 1024;
+constexpr int kFusedPasses = 2;          // sites per colour and thread on a plane
+
+template <bool kMeasure, bool kLearned>
+__global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
+    SpinBuf sb, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const float* __restrict__ beta_slot, const int32_t* __restrict__ fblock_rep,
+    const uint32_t* __restrict__ sweep_ctr, uint32_t sweep_off, int lc, uint32_t pstride,
+    uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count,
+    double* __restrict__ wave_part, uint32_t tune, const MlpDev* __restrict__ mlp)
+{
+    extern __shared__ float lds[];    // [ring 2][slot 3][component 3][pstride]
+    const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
+    const int slot = fblock_rep[blk];
+    const Rep R = reps[slot];
+    const Lad L = lads[R.ladder];
+    const float beta = beta_slot[slot];
+    const uint32_t sweep = *sweep_ctr + sweep_off;
+    const uint32_t buf = *sb.ctr + sweep_off;       // every sweep of the period is fused
+    const float* src = spin_buffer(sb, buf);
+    float* dst = spin_buffer(sb, buf + 1u);
+    const uint32_t P = (uint32_t)R.plane;
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    const uint32_t hx = (uint32_t)R.hx, ly = (uint32_t)R.ly;
+    const int lz = R.lz;
+    const bool pz = (R.per >> 2) & 1;
+    const bool px = R.per & 1, py = (R.per >> 1) & 1;
+    const int z0 = ((int)blk - R.f_first_block) * lc;
+    const int z1 = min(z0 + lc, lz);
+    const Planes src0 = planes(src, R, 0), src1 = planes(src, R, 1);
+    const Planes dst0 = planes(dst, R, 0), dst1 = planes(dst, R, 1);
+
+    // Plane z of the lattice (periodic image), or -1 beyond an open face.
+    auto wrapz = [&](int z) -> int {
+        if (z < 0) return pz ? z + lz : -1;
+        if (z >= lz) return pz ? z - lz : -1;
+        return z;
+    };
+    auto ring = [&](int r, int z) -> float* { return lds + (size_t)(r * 3 + (z + 3) % 3) * 3 * pstride; };
+
+    uint32_t kk[kFusedPasses], iyv[kFusedPasses], hv[kFusedPasses];
+    bool act[kFusedPasses];
+#pragma unroll
+    for (int j = 0; j < kFusedPasses; ++j) {
+        const uint32_t k = (uint32_t)j * T + tid;
+        act[j] = k < P;
+        kk[j] = act[j] ? k : P - 1u;
+        iyv[j] = fdiv(kk[j], R.div_hx);
+        hv[j] = fmodq(kk[j], iyv[j], R.div_hx);
+    }
+    auto load_plane = [&](const Planes& S, int z, f3 (&v)[kFusedPasses]) {
+        const int zz = wrapz(z);
+#pragma unroll
+        for (int j = 0; j < kFusedPasses; ++j)
+            v[j] = (zz >= 0 && act[j]) ? ld3(S, (uint32_t)zz * P + kk[j]) : f3{ 0.f, 0.f, 0.f };
+    };
+    auto put = [&](float* b, int j, const f3& v) {
+        b[kk[j]] = v.x; b[pstride + kk[j]] = v.y; b[2 * pstride + kk[j]] = v.z;
+    };
+    auto get = [&](const float* b, uint32_t i) -> f3 { return f3{ b[i], b[pstride + i], b[2 * pstride + i] }; };
+    // Six neighbours of site j of a plane with parity p (ix & 1), from the
+    // other colour's ring slots of the planes below, at and above it (zero
+    // across open faces).
+    auto gather = [&](const float* bm, const float* b0, const float* bp, int j, uint32_t p, f3 (&v)[6]) {
+        const uint32_t k = kk[j], h = hv[j], iy = iyv[j];
+        const bool xp_w = (p != 0u) && (h == hx - 1u), xm_w = (p == 0u) && (h == 0u);
+        const bool yp_w = iy + 1u == ly, ym_w = iy == 0u;
+        v[0] = get(b0, k + p - (xp_w ? hx : 0u));
+        v[1] = get(b0, k + p - 1u + (xm_w ? hx : 0u));
+        v[2] = get(b0, k + hx - (yp_w ? P : 0u));
+        v[3] = get(b0, k - hx + (ym_w ? P : 0u));
+        v[4] = get(bp, k);
+        v[5] = get(bm, k);
+        if (!px || !py) {
+            if (!px && xp_w) v[0] = f3{ 0.f, 0.f, 0.f };
+            if (!px && xm_w) v[1] = f3{ 0.f, 0.f, 0.f };
+            if (!py && yp_w) v[2] = f3{ 0.f, 0.f, 0.f };
+            if (!py && ym_w) v[3] = f3{ 0.f, 0.f, 0.f };
+        }
+    };
+
+    f3 own0[kFusedPasses], pre0[kFusedPasses], pre1[kFusedPasses];
+    for (int z = z0 - 2; z <= z0; ++z) {
+        load_plane(src1, z, pre1);
+#pragma unroll
+        for (int j = 0; j < kFusedPasses; ++j)
+            if (act[j]) put(ring(0, z), j, pre1[j]);
+    }
+    load_plane(src0, z0 - 1, own0);
+    __syncthreads();
+
+    double me = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    uint32_t n_acc = 0;        // wavefront-uniform
+    const bool two = (uint32_t)T < P;    // second pass has sites (uniform)
+    for (int s = z0 - 1; s <= z1; ++s) {
+        if (s <= z1 - 1) load_plane(src1, s + 2, pre1);
+        if (s + 1 <= z1) load_plane(src0, s + 1, pre0);
+        // colour 0 on plane s
+        const int zs = wrapz(s);
+        float* b1s = ring(1, s);
+        if (zs >= 0) {
+            const bool owned = s >= z0 && s < z1;
+            const float *am = ring(0, s - 1), *a0 = ring(0, s), *ap = ring(0, s + 1);
+#pragma unroll
+            for (int j = 0; j < kFusedPasses; ++j) {
+                if (j > 0 && !two) break;
+                const uint32_t p = (iyv[j] + (uint32_t)zs) & 1u;
+                f3 nbr[6];
+                gather(am, a0, ap, j, p, nbr);
+                const f3 hf = field_from(L, nbr);
+                bool accepted;
+                const f3 fs = metropolis_site<kLearned>(L, beta, own0[j], nbr, hf,
+                                                        site_number(R, 2u * hv[j] + p, iyv[j], (uint32_t)zs), sweep,
+                                                        seed_lo, seed_hi, (uint32_t)slot, mlp, act[j], accepted);
+                if (act[j]) {
+                    put(b1s, j, fs);
+                    if (owned) {
+                        const uint32_t q = (uint32_t)zs * P + kk[j];
+                        stf((float*)dst0.x, q, fs.x); stf((float*)dst0.y, q, fs.y); stf((float*)dst0.z, q, fs.z);
+                    }
+                }
+                if (owned) n_acc += (uint32_t)__popcll(__ballot(accepted));
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kFusedPasses; ++j)
+                if (act[j]) put(b1s, j, f3{ 0.f, 0.f, 0.f });
+        }
+        __syncthreads();
+        // colour 1 on plane s-1
+        if (s - 1 >= z0) {
+            const uint32_t zb = (uint32_t)(s - 1);
+            const float* a0 = ring(0, s - 1);
+            const float *bm = ring(1, s - 2), *b0 = ring(1, s - 1);
+#pragma unroll
+            for (int j = 0; j < kFusedPasses; ++j) {
+                if (j > 0 && !two) break;
+                const uint32_t p = (iyv[j] + zb + 1u) & 1u;
+                const f3 s1 = get(a0, kk[j]);
+                f3 nbr[6];
+                gather(bm, b0, b1s, j, p, nbr);
+                const f3 hf = field_from(L, nbr);
+                bool accepted;
+                const f3 fs = metropolis_site<kLearned>(L, beta, s1, nbr, hf,
+                                                        site_number(R, 2u * hv[j] + p, iyv[j], zb), sweep,
+                                                        seed_lo, seed_hi, (uint32_t)slot, mlp, act[j], accepted);
+                if (act[j]) {
+                    const uint32_t q = zb * P + kk[j];
+                    stf((float*)dst1.x, q, fs.x); stf((float*)dst1.y, q, fs.y); stf((float*)dst1.z, q, fs.z);
+                    if constexpr (kMeasure) {
+                        const f3 c0 = get(b0, kk[j]);
+                        me += (double)pair_energy(L, fs, hf, c0);
+                        mx += (double)(fs.x + c0.x); my += (double)(fs.y + c0.y); mz += (double)(fs.z + c0.z);
+                    }
+                }
+                n_acc += (uint32_t)__popcll(__ballot(accepted));
+            }
+        }
+        if (s <= z1 - 1) {
+            float* a = ring(0, s + 2);     // the slot of plane s-1
+#pragma unroll
+            for (int j = 0; j < kFusedPasses; ++j)
+                if (act[j]) put(a, j, pre1[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kFusedPasses; ++j) own0[j] = pre0[j];
+        __syncthreads();
+    }
+
+    const uint32_t wpb = T / kWave, wave = tid / kWave;
+    if constexpr (kMeasure) {
+        me = wave_sum(me); mx = wave_sum(mx); my = wave_sum(my); mz = wave_sum(mz);
+        if ((tid & (kWave - 1)) == 0) {
+            double* o = wave_part + 4 * ((size_t)blk * wpb + wave);
+            o[0] = me; o[1] = mx; o[2] = my; o[3] = mz;
+        }
+    }
+    if ((tid & (kWave - 1)) == 0 && n_acc) acc_count[(size_t)blk * wpb + wave] += n_acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -378,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_collect(
     const Planes own = planes(spins, Rr, colour);
     const Planes oth = planes(spins, Rr, colour ^ 1);
     f3 nbr[6];
-    load_nbrs<false>(oth, neighbours<false>(Rr, row, h, iy, iz, p), nbr);
+    load_nbrs(Rr, oth, neighbours<false>(Rr, q, h, iy, iz, p), nbr);
     const f3 hf = field_from(L, nbr);
     const f3 s = ld3(own, q);
     float* o = out + (size_t)e * kRecord;
@@ -390,23 +649,48 @@ __global__ __launch_bounds__(kBlock) void k_mlp_collect(
     o[27] = beta; o[28] = L.Kz; o[29] = 0.f; o[30] = 0.f; o[31] = 0.f;
 }
 
-// Per-slot sum of the per-wavefront acceptance counters.
-__global__ __launch_bounds__(kBlock) void k_acc_finalize(
-    const Rep* __restrict__ reps, const unsigned long long* __restrict__ acc_wave, unsigned long long* __restrict__ acc)
+// Sum of n counters over a workgroup of kBlock threads; valid in thread 0.
+__device__ __forceinline__ unsigned long long block_sum_u64(const unsigned long long* __restrict__ a, int n,
+                                                            unsigned long long* lds /* [kBlock] */)
 {
-    __shared__ unsigned long long lds[kBlock];
-    const Rep R = reps[blockIdx.x];
     unsigned long long v = 0;
-    const int n = R.nblocks * (kBlock / kWave);
-    const unsigned long long* base = acc_wave + (size_t)R.first_block * (kBlock / kWave);
-    for (int j = threadIdx.x; j < n; j += kBlock) v += base[j];
+    for (int j = threadIdx.x; j < n; j += kBlock) v += a[j];
     lds[threadIdx.x] = v;
     __syncthreads();
     for (int o = kBlock / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) acc[blockIdx.x] = lds[0];
+    return lds[0];
+}
+
+// Per-slot acceptance total: counters retired from earlier fused layouts,
+// plus the per-wavefront counters of the half-sweep grid and of the fused
+// grid (wpb_f wavefronts per fused workgroup).
+__global__ __launch_bounds__(kBlock) void k_acc_finalize(
+    const Rep* __restrict__ reps, const unsigned long long* __restrict__ acc_wave,
+    const unsigned long long* __restrict__ acc_wave_f, int wpb_f,
+    const unsigned long long* __restrict__ acc_base, unsigned long long* __restrict__ acc)
+{
+    __shared__ unsigned long long lds[kBlock];
+    const Rep R = reps[blockIdx.x];
+    const unsigned long long a = block_sum_u64(acc_wave + (size_t)R.first_block * (kBlock / kWave),
+                                               R.nblocks * (kBlock / kWave), lds);
+    __syncthreads();
+    const unsigned long long b = block_sum_u64(acc_wave_f + (size_t)R.f_first_block * wpb_f, R.f_nblocks * wpb_f, lds);
+    if (threadIdx.x == 0) acc[blockIdx.x] = acc_base[blockIdx.x] + a + b;
+}
+
+// Moves the fused-grid counters of each slot into acc_base (before the fused
+// layout changes); the caller then clears acc_wave_f.
+__global__ __launch_bounds__(kBlock) void k_acc_fold(
+    const Rep* __restrict__ reps, const unsigned long long* __restrict__ acc_wave_f, int wpb_f,
+    unsigned long long* __restrict__ acc_base)
+{
+    __shared__ unsigned long long lds[kBlock];
+    const Rep R = reps[blockIdx.x];
+    const unsigned long long b = block_sum_u64(acc_wave_f + (size_t)R.f_first_block * wpb_f, R.f_nblocks * wpb_f, lds);
+    if (threadIdx.x == 0) acc_base[blockIdx.x] += b;
 }
 
 // ---------------------------------------------------------------------------
@@ -437,11 +721,12 @@ __device__ __forceinline__ void block_sum4(double v[4], double* lds /* [4][kBloc
 // Per-site energy of the +x, +y, +z bonds plus on-site terms, and the spin,
 // for both colours at half index q; per-workgroup partial sums.
 __global__ __launch_bounds__(kBlock) void k_measure(
-    const float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    SpinBuf sb, uint32_t sweep_off, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const Lad* __restrict__ lad_override, const int32_t* __restrict__ block_rep, int block_base,
     double* __restrict__ partial)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
+    const float* spins = spin_buffer(sb, *sb.ctr + sweep_off);
     const int b = blockIdx.x + block_base;
     const int slot = block_rep[b];
     const Rep R = reps[slot];
@@ -457,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(
             const Planes own = planes(spins, R, colour);
             const Planes oth = planes(spins, R, colour ^ 1);
             const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
-            const Nbr nb = neighbours<false>(R, row, h, iy, iz, p);
+            const Nbr nb = neighbours<false>(R, q, h, iy, iz, p);
             const f3 sf = ld3(own, q);
             const double sx = sf.x, sy = sf.y, sz = sf.z;
             double e = 0.0;
@@ -485,16 +770,19 @@ __global__ __launch_bounds__(kBlock) void k_measure(
 
 // One workgroup per slot: fixed-order FP64 sum of the per-wavefront
 // measurement partials of the fused colour-1 half-sweep (deterministic).
+// wpb_f = 0: partials of the half-sweep grid; else of the fused grid with
+// wpb_f wavefronts per workgroup.
 __global__ __launch_bounds__(kBlock) void k_wave_finalize(
-    const Rep* __restrict__ reps, const double* __restrict__ wave_part,
+    const Rep* __restrict__ reps, const double* __restrict__ wave_part, int wpb_f,
     double* __restrict__ energy, double* __restrict__ mag)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
     const int slot = blockIdx.x;
     const Rep R = reps[slot];
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
-    const int nw = R.nblocks * (kBlock / kWave);
-    const double* base = wave_part + 4 * (size_t)R.first_block * (kBlock / kWave);
+    const int nw = wpb_f ? R.f_nblocks * wpb_f : R.nblocks * (kBlock / kWave);
+    const double* base = wave_part + 4 * (wpb_f ? (size_t)R.f_first_block * wpb_f
+                                               : (size_t)R.first_block * (kBlock / kWave));
     for (int j = threadIdx.x; j < nw; j += kBlock) {
         v[0] += base[4 * j]; v[1] += base[4 * j + 1]; v[2] += base[4 * j + 2]; v[3] += base[4 * j + 3];
     }
@@ -563,10 +851,11 @@ __device__ __forceinline__ f3 spin_at(const float* __restrict__ spins, const Rep
 // layer < 0: every layer of every replica (grid = qblock table); layer >= 0:
 // that layer only (grid = one-layer table, q_blocks_layer blocks per replica).
 __global__ __launch_bounds__(kBlock) void k_winding(
-    const float* __restrict__ spins, const Rep* __restrict__ reps, const int32_t* __restrict__ table, int layer,
+    SpinBuf sb, uint32_t sweep_off, const Rep* __restrict__ reps, const int32_t* __restrict__ table, int layer,
     double* __restrict__ qpartial)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
+    const float* spins = spin_buffer(sb, *sb.ctr + sweep_off);
     const int slot = table[blockIdx.x];
     const Rep R = reps[slot];
     uint32_t z, chunk;
@@ -705,7 +994,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(const float4* __restrict_
         dst[i] = src[i];
 }
 
-__global__ void k_add_u32(uint32_t* __restrict__ p, uint32_t v) { if (threadIdx.x == 0) *p += v; }
+// Advances the sweep counter by `sweeps` and the fused-sweep counter by `fused`.
+__global__ void k_advance(uint32_t* __restrict__ ctr, uint32_t sweeps, uint32_t fused)
+{
+    if (threadIdx.x == 0) { ctr[0] += sweeps; ctr[4] += fused; }
+}
 __global__ void k_set_u32(uint32_t* __restrict__ p, int n, uint32_t v)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
@@ -768,9 +1061,9 @@ __global__ __launch_bounds__(kBlock) void k_delta_energy(
     const uint32_t q = row * (uint32_t)R.hx + h;
     const Planes own = planes(spins, R, (int)colour);
     const Planes oth = planes(spins, R, (int)(colour ^ 1u));
-    const Nbr nb = neighbours<false>(R, row, h, iy, iz, ix & 1u);
+    const Nbr nb = neighbours<false>(R, q, h, iy, iz, ix & 1u);
     const f3 s = ld3(own, q);
-    const f3 hf = local_field<false>(L, oth, nb);
+    const f3 hf = local_field(R, L, oth, nb);
     const f3 sn{ props[3 * e], props[3 * e + 1], props[3 * e + 2] };
     out[e] = (double)delta_e(s, sn, hf, L.Kz);
 }
@@ -799,9 +1092,16 @@ struct ising_ctx {
     std::vector<LadHost> ladders;
     std::vector<Rep> reps;
     std::vector<int32_t> block_rep, qblock_rep, lblock_rep;
-    int64_t total_floats = 0;
+    int64_t total_floats = 0;    // floats per spin buffer (two buffers, sweep parity)
     uint32_t host_sweep = 0;
-    uint32_t tune = 0;           // sweep tuning flags (kTune*)
+    uint32_t host_fused = 0;     // fused sweeps enqueued: parity selects the current spin buffer
+    uint32_t tune = 0;           // sweep tuning flags (kTune*, fused chunk in bits 8..15)
+    // fused sweep (k_sweep_fused): available when every plane fits in LDS
+    bool fused_ok = false;
+    int fused_threads = 0, fused_wpb = 0, n_fblocks = 0, max_fblocks = 0;
+    uint32_t fused_pstride = 0;
+    size_t fused_lds = 0;
+    std::vector<int32_t> fblock_rep;
     bool any_learned = false;    // some ladder takes its cone radius from the network
     std::vector<int> learned;    // per ladder
 
@@ -837,16 +1137,20 @@ struct ising_ctx {
     double* d_qpartial = nullptr;
     double* d_qraw = nullptr;
     long long* d_qint = nullptr;
-    uint32_t* d_ctr = nullptr;  // [0] sweep, [1] meas, [2] winding meas, [3] swap round
+    uint32_t* d_ctr = nullptr;  // [0] sweep, [1] meas, [2] winding meas, [3] swap round, [4] fused sweeps
     unsigned long long* d_acc = nullptr;       // [R] per-slot totals (filled on request)
     unsigned long long* d_acc_wave = nullptr;  // [n_blocks * waves per block]
     double* d_wave_part = nullptr;             // [n_blocks * waves per block][4]
+    int32_t* d_fblock_rep = nullptr;           // [max_fblocks]
+    unsigned long long* d_acc_wave_f = nullptr;  // [max_fblocks * fused_wpb]
+    unsigned long long* d_acc_base = nullptr;    // [R] counters of earlier fused layouts
+    double* d_wave_part_f = nullptr;             // [max_fblocks * fused_wpb][4]
     unsigned long long* d_swap_att = nullptr;
     unsigned long long* d_swap_acc = nullptr;
     int32_t* d_sites = nullptr; float* d_props = nullptr; double* d_de = nullptr; int de_cap = 0;
     // checkpoint (ising_save_state)
     float* ck_spins = nullptr; int32_t* ck_sot = nullptr; float* ck_bslot = nullptr; double* ck_btemp = nullptr;
-    uint32_t* ck_ctr = nullptr; uint32_t ck_host_sweep = 0; bool ck_valid = false;
+    uint32_t* ck_ctr = nullptr; uint32_t ck_host_sweep = 0, ck_host_fused = 0; bool ck_valid = false;
 };
 
 namespace {
@@ -901,6 +1205,7 @@ Lad make_lad(const ising_hamiltonian& h, double cone)
     if (h.dmi_type == ISING_DMI_BULK) { d[0] = h.D; d[4] = h.D; d[8] = h.D; }
     else if (h.dmi_type == ISING_DMI_INTERFACIAL_REF) { d[1] = -h.D; d[3] = h.D; }
     L.J = (float)h.J; L.Bz = (float)h.Bz; L.Kz = (float)h.Kz; L.r = (float)cone;
+    L.lr8 = (float)(std::log2(cone) - 8.0);
     L.zsf = (float)h.zeeman_total_scale;
     for (int k = 0; k < 9; ++k) { L.d[k] = (float)d[k]; L.dd[k] = d[k]; }
     L.dmi = h.dmi_type;
@@ -954,36 +1259,84 @@ int check_replica(ising_ctx* c, int r)
 
 // ---- launch helpers --------------------------------------------------------
 
+constexpr int kCounters = 5;
+
+SpinBuf spin_buf(const ising_ctx* c) { return SpinBuf{ c->d_spins, c->total_floats, c->d_ctr + 4 }; }
+
+// The buffer that holds the state after the sweeps the host has enqueued.
+float* cur_spins(const ising_ctx* c) { return c->d_spins + (int64_t)(c->host_fused & 1u) * c->total_floats; }
+
+// z planes per fused-sweep workgroup (0: two half-sweep launches per sweep).
+int fused_lc(const ising_ctx* c) { return (int)((c->tune >> 8) & 0xFFu); }
+
 template <bool kMeasure, bool kLearned>
-void launch_sweep_kernel(ising_ctx* c, uint32_t off, int colour)
+void launch_half_kernel(ising_ctx* c, uint32_t off, int colour)
 {
     hipLaunchKernelGGL((k_half_sweep<kMeasure, kLearned>), dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
-                       c->d_spins, c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
-                       c->d_ctr + 0, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
-                       kMeasure ? c->d_wave_part : (double*)nullptr, c->tune, (const MlpDev*)c->d_mlp);
+                       spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
+                       (const uint32_t*)c->d_ctr, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
+                       kMeasure ? c->d_wave_part : (double*)nullptr, c->tune & 0xFFu, (const MlpDev*)c->d_mlp);
 }
 
-int launch_half_sweeps(ising_ctx* c, uint32_t off, int colour, bool measure = false)
+template <bool kMeasure, bool kLearned>
+void launch_fused_kernel(ising_ctx* c, uint32_t off)
+{
+    hipLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned>), dim3(c->n_fblocks), dim3(c->fused_threads), c->fused_lds,
+                       c->stream, spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_fblock_rep,
+                       (const uint32_t*)c->d_ctr, off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
+                       c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu,
+                       (const MlpDev*)c->d_mlp);
+}
+
+// Sweep number `off` of a period (relative to the device sweep counter):
+// colour 0 then colour 1 of every replica, from the buffer of the current
+// parity into the other one; with `measure`, E and M of the new state are
+// recorded by the same launches.
+int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false)
 {
     const bool learned = c->any_learned;
+    if (fused_lc(c) > 0) {
+        if (measure) {
+            if (learned) launch_fused_kernel<true, true>(c, off);
+            else launch_fused_kernel<true, false>(c, off);
+        } else {
+            if (learned) launch_fused_kernel<false, true>(c, off);
+            else launch_fused_kernel<false, false>(c, off);
+        }
+        LAUNCH_CHECK(c);
+        if (measure) {
+            hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
+                               (const double*)c->d_wave_part_f, c->fused_wpb, c->d_energy, c->d_mag);
+            LAUNCH_CHECK(c);
+        }
+        return ISING_OK;
+    }
+    if (learned) launch_half_kernel<false, true>(c, off, 0);
+    else launch_half_kernel<false, false>(c, off, 0);
+    LAUNCH_CHECK(c);
     if (measure) {
-        if (learned) launch_sweep_kernel<true, true>(c, off, colour);
-        else launch_sweep_kernel<true, false>(c, off, colour);
+        if (learned) launch_half_kernel<true, true>(c, off, 1);
+        else launch_half_kernel<true, false>(c, off, 1);
         LAUNCH_CHECK(c);
         hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
-                           (const double*)c->d_wave_part, c->d_energy, c->d_mag);
+                           (const double*)c->d_wave_part, 0, c->d_energy, c->d_mag);
     } else {
-        if (learned) launch_sweep_kernel<false, true>(c, off, colour);
-        else launch_sweep_kernel<false, false>(c, off, colour);
+        if (learned) launch_half_kernel<false, true>(c, off, 1);
+        else launch_half_kernel<false, false>(c, off, 1);
     }
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
 
-int launch_measure_all(ising_ctx* c)
+// Buffer offset of the state after `done` sweeps of a period.
+uint32_t buf_off(const ising_ctx* c, int done) { return fused_lc(c) > 0 ? (uint32_t)done : 0u; }
+
+// Exact FP64 E and M of every slot, for the state after `done` sweeps of the
+// period.
+int launch_measure_all(ising_ctx* c, int done = 0)
 {
     hipLaunchKernelGGL(k_measure, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
-                       (const float*)c->d_spins, c->d_reps, c->d_lads, (const Lad*)nullptr, c->d_block_rep, 0, c->d_partial);
+                       spin_buf(c), buf_off(c, done), c->d_reps, c->d_lads, (const Lad*)nullptr, c->d_block_rep, 0, c->d_partial);
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_measure_finalize, dim3(c->R), dim3(kBlock), 0, c->stream,
                        c->d_reps, c->d_partial, 0, c->d_energy, c->d_mag);
@@ -991,14 +1344,15 @@ int launch_measure_all(ising_ctx* c)
     return ISING_OK;
 }
 
-int launch_winding_all(ising_ctx* c, int layer = -1)
+int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0)
 {
+    const uint32_t off = buf_off(c, done);
     if (layer < 0) {
         hipLaunchKernelGGL(k_winding, dim3(c->n_qblocks), dim3(kBlock), 0, c->stream,
-                           (const float*)c->d_spins, c->d_reps, (const int32_t*)c->d_qblock_rep, -1, c->d_qpartial);
+                           spin_buf(c), off, c->d_reps, (const int32_t*)c->d_qblock_rep, -1, c->d_qpartial);
     } else {
         hipLaunchKernelGGL(k_winding, dim3(c->n_lblocks), dim3(kBlock), 0, c->stream,
-                           (const float*)c->d_spins, c->d_reps, (const int32_t*)c->d_lblock_rep, layer, c->d_qpartial);
+                           spin_buf(c), off, c->d_reps, (const int32_t*)c->d_lblock_rep, layer, c->d_qpartial);
     }
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_winding_finalize, dim3(c->R * c->zmax), dim3(kBlock), 0, c->stream,
@@ -1017,11 +1371,19 @@ int launch_swap(ising_ctx* c)
     return ISING_OK;
 }
 
-int launch_add(ising_ctx* c, int which, uint32_t v)
+// Device counters after n sweeps (graph-capturable); advance_host follows
+// once they are enqueued.
+int launch_advance(ising_ctx* c, uint32_t n)
 {
-    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, c->stream, c->d_ctr + which, v);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_ctr, n, fused_lc(c) > 0 ? n : 0u);
     LAUNCH_CHECK(c);
     return ISING_OK;
+}
+
+void advance_host(ising_ctx* c, uint32_t n)
+{
+    c->host_sweep += n;
+    if (fused_lc(c) > 0) c->host_fused += n;
 }
 
 struct SeriesDev {
@@ -1041,13 +1403,11 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
         const bool do_swap = s.swap_every > 0 && done % s.swap_every == 0;
         const bool do_meas = s.measure_every > 0 && done % s.measure_every == 0;
         const bool do_q = s.winding_every > 0 && done % s.winding_every == 0;
-        // Measurements ride on the colour-1 launch, except before a swap, whose
+        // Measurements ride on the sweep launches, except before a swap, whose
         // exponent takes the per-site FP64 energies of the measure kernel.
-        const bool fused = do_meas && !do_swap;
-        if ((rc = launch_half_sweeps(c, (uint32_t)k, 0))) return rc;
-        if ((rc = launch_half_sweeps(c, (uint32_t)k, 1, fused))) return rc;
+        if ((rc = launch_sweep(c, (uint32_t)k, do_meas && !do_swap))) return rc;
         if (do_swap)
-            if ((rc = launch_measure_all(c))) return rc;
+            if ((rc = launch_measure_all(c, done))) return rc;
         if (do_swap)
             if ((rc = launch_swap(c))) return rc;
         if (do_meas && sd.e) {
@@ -1057,14 +1417,79 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
             LAUNCH_CHECK(c);
         }
         if (do_q && sd.qr) {
-            if ((rc = launch_winding_all(c, s.winding_layer - 1))) return rc;
+            if ((rc = launch_winding_all(c, s.winding_layer - 1, done))) return rc;
             hipLaunchKernelGGL(k_record_q, dim3(1), dim3(kBlock), 0, c->stream,
                                (const double*)c->d_qraw, (const long long*)c->d_qint, (const int32_t*)c->d_slot_of_temp,
                                c->R, c->zmax, c->d_ctr + 2, sd.n_q, sd.qr, sd.qi);
             LAUNCH_CHECK(c);
         }
     }
-    return launch_add(c, 0, (uint32_t)count);
+    return launch_advance(c, (uint32_t)count);
+}
+
+// Fused-sweep resources: available when every replica's z plane of one
+// colour fits the two LDS rings (72 bytes per site) and two sites per thread
+// in one workgroup. Counter and partial arrays are sized for one plane per
+// workgroup, the finest chunking.
+int fused_setup(ising_ctx* c)
+{
+    int pmax = 0, zsum = 0;
+    for (const Rep& r : c->reps) { pmax = std::max(pmax, r.plane); zsum += r.lz; }
+    int lds_max = 0, optin = 0;
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, c->device);
+    lds_max = std::max(lds_max, optin);
+    const size_t lds = (size_t)18 * pmax * sizeof(float);
+    int threads = (int)((((pmax + kFusedPasses - 1) / kFusedPasses) + kWave - 1) / kWave * kWave);
+    c->fused_ok = pmax > 0 && threads <= kFusedMaxThreads && lds <= (size_t)lds_max;
+    c->fused_threads = std::max(threads, kWave);
+    c->fused_wpb = c->fused_threads / kWave;
+    c->fused_pstride = (uint32_t)pmax;
+    c->fused_lds = lds;
+    c->max_fblocks = zsum;
+    if (c->fused_ok) {
+        const void* fns[] = { (const void*)k_sweep_fused<false, false>, (const void*)k_sweep_fused<true, false>,
+                              (const void*)k_sweep_fused<false, true>, (const void*)k_sweep_fused<true, true> };
+        for (const void* f : fns)
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+                c->fused_ok = false;
+        (void)hipGetLastError();
+    }
+    int rc;
+    if ((rc = dalloc(c, &c->d_fblock_rep, (size_t)zsum))) return rc;
+    if ((rc = dalloc(c, &c->d_acc_wave_f, (size_t)zsum * c->fused_wpb))) return rc;
+    if ((rc = dalloc(c, &c->d_wave_part_f, 4 * (size_t)zsum * c->fused_wpb))) return rc;
+    if ((rc = dalloc(c, &c->d_acc_base, (size_t)c->R))) return rc;
+    return ISING_OK;
+}
+
+// Chunks of lc z planes per fused workgroup (lc = 0: none). Acceptance
+// counters of the previous layout are folded into the per-slot base first.
+int fused_layout(ising_ctx* c, int lc)
+{
+    if (c->n_fblocks > 0) {
+        hipLaunchKernelGGL(k_acc_fold, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
+                           (const unsigned long long*)c->d_acc_wave_f, c->fused_wpb, c->d_acc_base);
+        LAUNCH_CHECK(c);
+        HIP_TRY(c, hipMemsetAsync(c->d_acc_wave_f, 0, (size_t)c->max_fblocks * c->fused_wpb * sizeof(unsigned long long),
+                                  c->stream));
+    }
+    c->fblock_rep.clear();
+    int fb = 0;
+    for (size_t k = 0; k < c->reps.size(); ++k) {
+        Rep& r = c->reps[k];
+        r.f_first_block = fb;
+        r.f_nblocks = lc > 0 ? (r.lz + lc - 1) / lc : 0;
+        fb += r.f_nblocks;
+        for (int b = 0; b < r.f_nblocks; ++b) c->fblock_rep.push_back((int32_t)k);
+    }
+    c->n_fblocks = fb;
+    HIP_TRY(c, hipMemcpyAsync(c->d_reps, c->reps.data(), c->reps.size() * sizeof(Rep), hipMemcpyHostToDevice, c->stream));
+    if (fb > 0)
+        HIP_TRY(c, hipMemcpyAsync(c->d_fblock_rep, c->fblock_rep.data(), (size_t)fb * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
 }
 
 int64_t gcd64(int64_t a, int64_t b) { while (b) { int64_t t = a % b; a = b; b = t; } return a; }
@@ -1111,7 +1536,8 @@ void ising_destroy(ising_ctx* c)
                      c->d_lad_n, c->d_partial, c->d_energy, c->d_mag, c->d_qpartial, c->d_qraw, c->d_qint,
                      c->d_ctr, c->d_acc, c->d_acc_wave, c->d_wave_part, c->d_mlp, c->d_mlp_prm, c->d_mlp_m1,
                      c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de,
-                     c->ck_spins, c->ck_sot, c->ck_bslot, c->ck_btemp, c->ck_ctr };
+                     c->ck_spins, c->ck_sot, c->ck_bslot, c->ck_btemp, c->ck_ctr,
+                     c->d_fblock_rep, c->d_acc_wave_f, c->d_acc_base, c->d_wave_part_f };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1176,6 +1602,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
             r.q_blocks_layer = (r.lx * r.ly + kBlock - 1) / kBlock;
             qblk += r.q_blocks_layer * r.lz;
             r.l_first_block = (int32_t)c->lblock_rep.size();
+            r.plane = r.ly * r.hx;
             r.div_hx = make_fastdiv((uint32_t)r.hx);
             r.div_ly = make_fastdiv((uint32_t)r.ly);
             r.div_lx = make_fastdiv((uint32_t)r.lx);
@@ -1202,7 +1629,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(c, ISING_ERR_HIP, "hipStreamCreate"));
 
-    if ((rc = dalloc(c, &c->d_spins, (size_t)c->total_floats))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_spins, 2 * (size_t)c->total_floats))) return fail(rc);
     if ((rc = dalloc(c, &c->d_reps, c->reps.size()))) return fail(rc);
     if ((rc = dalloc(c, &c->d_lads, c->ladders.size()))) return fail(rc);
     if ((rc = dalloc(c, &c->d_lad_override, 1))) return fail(rc);
@@ -1221,7 +1648,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if ((rc = dalloc(c, &c->d_qpartial, (size_t)qblk))) return fail(rc);
     if ((rc = dalloc(c, &c->d_qraw, (size_t)R * zmax))) return fail(rc);
     if ((rc = dalloc(c, &c->d_qint, (size_t)R * zmax))) return fail(rc);
-    if ((rc = dalloc(c, &c->d_ctr, 4))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_ctr, kCounters))) return fail(rc);
     if ((rc = dalloc(c, &c->d_acc, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_acc_wave, (size_t)blk * (kBlock / kWave)))) return fail(rc);
     if ((rc = dalloc(c, &c->d_wave_part, 4 * (size_t)blk * (kBlock / kWave)))) return fail(rc);
@@ -1233,6 +1660,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     c->learned.assign(n_ladders, 0);
     if ((rc = dalloc(c, &c->d_swap_att, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_swap_acc, R))) return fail(rc);
+    if ((rc = fused_setup(c))) return fail(rc);
 
     std::vector<float> beta_slot(R);
     std::vector<double> beta_temp(R);
@@ -1262,8 +1690,10 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     UP(c->d_lad_start, lad_start.data(), n_ladders);
     UP(c->d_lad_n, lad_n.data(), n_ladders);
 #undef UP
-    if (hipMemsetAsync(c->d_spins, 0, (size_t)c->total_floats * sizeof(float), s) != hipSuccess ||
-        hipMemsetAsync(c->d_ctr, 0, 4 * sizeof(uint32_t), s) != hipSuccess ||
+    if (hipMemsetAsync(c->d_spins, 0, 2 * (size_t)c->total_floats * sizeof(float), s) != hipSuccess ||
+        hipMemsetAsync(c->d_acc_base, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
+        hipMemsetAsync(c->d_acc_wave_f, 0, (size_t)c->max_fblocks * c->fused_wpb * sizeof(unsigned long long), s) != hipSuccess ||
+        hipMemsetAsync(c->d_ctr, 0, kCounters * sizeof(uint32_t), s) != hipSuccess ||
         hipMemsetAsync(c->d_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
         hipMemsetAsync(c->d_acc_wave, 0, (size_t)blk * (kBlock / kWave) * sizeof(unsigned long long), s) != hipSuccess ||
         hipMemsetAsync(c->d_mlp_prm, 0, kMlpParams * sizeof(float), s) != hipSuccess ||
@@ -1355,7 +1785,7 @@ int ising_set_spins(ising_ctx* c, int replica, const double* soa, int64_t n)
                 h[idx] = (float)x; h[r.n + idx] = (float)y; h[2 * r.n + idx] = (float)z;
             }
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipMemcpyAsync(c->d_spins + r.off, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(cur_spins(c) + r.off, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return ISING_OK;
 }
@@ -1368,7 +1798,7 @@ int ising_get_spins(ising_ctx* c, int replica, double* soa, int64_t n)
     if (!soa || n != r.n) return set_err(c, ISING_ERR_INVALID_ARG, "spin buffer size");
     std::vector<float> h((size_t)3 * r.n);
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_spins + r.off, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(h.data(), cur_spins(c) + r.off, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int ix = 0; ix < r.lx; ++ix)
         for (int iy = 0; iy < r.ly; ++iy)
@@ -1388,7 +1818,7 @@ static int init_common(ising_ctx* c, int replica, int mode, double frac)
     HIP_TRY(c, hipSetDevice(c->device));
     int base = 0, nb = c->n_blocks;
     if (replica >= 0) { base = c->reps[replica].first_block; nb = c->reps[replica].nblocks; }
-    hipLaunchKernelGGL(k_init, dim3(nb), dim3(kBlock), 0, c->stream, c->d_spins, c->d_reps, c->d_block_rep, base,
+    hipLaunchKernelGGL(k_init, dim3(nb), dim3(kBlock), 0, c->stream, cur_spins(c), c->d_reps, c->d_block_rep, base,
                        mode, frac, (uint32_t)c->seed, (uint32_t)(c->seed >> 32));
     LAUNCH_CHECK(c);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1411,13 +1841,11 @@ int ising_sweep(ising_ctx* c, int n_sweeps)
     if (n_sweeps == 0) return ISING_OK;
     HIP_TRY(c, hipSetDevice(c->device));
     int rc;
-    for (int k = 0; k < n_sweeps; ++k) {
-        if ((rc = launch_half_sweeps(c, (uint32_t)k, 0))) return rc;
-        if ((rc = launch_half_sweeps(c, (uint32_t)k, 1))) return rc;
-    }
-    if ((rc = launch_add(c, 0, (uint32_t)n_sweeps))) return rc;
+    for (int k = 0; k < n_sweeps; ++k)
+        if ((rc = launch_sweep(c, (uint32_t)k))) return rc;
+    if ((rc = launch_advance(c, (uint32_t)n_sweeps))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    c->host_sweep += (uint32_t)n_sweeps;
+    advance_host(c, (uint32_t)n_sweeps);
     return ISING_OK;
 }
 
@@ -1453,7 +1881,7 @@ int ising_energy_with(ising_ctx* c, int replica, const ising_hamiltonian* h, dou
     const Rep& r = c->reps[replica];
     Lad L = make_lad(*h, c->ladders[r.ladder].cone);
     HIP_TRY(c, hipMemcpyAsync(c->d_lad_override, &L, sizeof L, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_measure, dim3(r.nblocks), dim3(kBlock), 0, c->stream, (const float*)c->d_spins, c->d_reps,
+    hipLaunchKernelGGL(k_measure, dim3(r.nblocks), dim3(kBlock), 0, c->stream, spin_buf(c), 0u, c->d_reps,
                        c->d_lads, (const Lad*)c->d_lad_override, c->d_block_rep, r.first_block, c->d_partial);
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_measure_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->d_reps, c->d_partial, replica,
@@ -1515,7 +1943,7 @@ int ising_delta_energy(ising_ctx* c, int replica, const int32_t* sites, const do
     HIP_TRY(c, hipMemcpyAsync(c->d_sites, sites, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_props, pf.data(), pf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_delta_energy, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                       (const float*)c->d_spins, c->d_reps, c->d_lads, replica, (const int32_t*)c->d_sites,
+                       (const float*)cur_spins(c), c->d_reps, c->d_lads, replica, (const int32_t*)c->d_sites,
                        (const float*)c->d_props, n, c->d_de);
     LAUNCH_CHECK(c);
     HIP_TRY(c, hipMemcpyAsync(de, c->d_de, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1593,7 +2021,7 @@ int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* 
     }
     hipError_t es = hipStreamSynchronize(c->stream);
     if (es != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "pt_run: %s", hipGetErrorString(es)); }
-    c->host_sweep += (uint32_t)s.n_sweeps;
+    advance_host(c, (uint32_t)s.n_sweeps);
     if (out) {
         hipError_t e1 = hipSuccess;
         if (out->energy && sd.n_meas)
@@ -1617,16 +2045,17 @@ int ising_save_state(ising_ctx* c)
     int rc;
     if (!c->ck_spins) {
         if ((rc = dalloc(c, &c->ck_spins, (size_t)c->total_floats)) || (rc = dalloc(c, &c->ck_sot, c->R)) ||
-            (rc = dalloc(c, &c->ck_bslot, c->R)) || (rc = dalloc(c, &c->ck_btemp, c->R)) || (rc = dalloc(c, &c->ck_ctr, 4)))
+            (rc = dalloc(c, &c->ck_bslot, c->R)) || (rc = dalloc(c, &c->ck_btemp, c->R)) || (rc = dalloc(c, &c->ck_ctr, kCounters)))
             return rc;
     }
-    HIP_TRY(c, hipMemcpyAsync(c->ck_spins, c->d_spins, (size_t)c->total_floats * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->ck_spins, cur_spins(c), (size_t)c->total_floats * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->ck_sot, c->d_slot_of_temp, c->R * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->ck_bslot, c->d_beta_slot, c->R * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->ck_btemp, c->d_beta_temp, c->R * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->ck_ctr, c->d_ctr, 4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->ck_ctr, c->d_ctr, kCounters * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->ck_host_sweep = c->host_sweep;
+    c->ck_host_fused = c->host_fused;
     c->ck_valid = true;
     return ISING_OK;
 }
@@ -1636,13 +2065,15 @@ int ising_restore_state(ising_ctx* c)
     if (!c) return ISING_ERR_INVALID_ARG;
     if (!c->ck_valid) return set_err(c, ISING_ERR_STATE, "no saved state: call ising_save_state first");
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipMemcpyAsync(c->d_spins, c->ck_spins, (size_t)c->total_floats * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    float* dst = c->d_spins + (int64_t)(c->ck_host_fused & 1u) * c->total_floats;
+    HIP_TRY(c, hipMemcpyAsync(dst, c->ck_spins, (size_t)c->total_floats * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_slot_of_temp, c->ck_sot, c->R * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_beta_slot, c->ck_bslot, c->R * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_beta_temp, c->ck_btemp, c->R * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->d_ctr, c->ck_ctr, 4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_ctr, c->ck_ctr, kCounters * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->host_sweep = c->ck_host_sweep;
+    c->host_fused = c->ck_host_fused;
     return ISING_OK;
 }
 
@@ -1675,18 +2106,17 @@ int ising_time_sweeps(ising_ctx* c, int n_sweeps, double* ms_per_launch)
     int rc = ISING_OK;
     if (hipEventRecord(e0, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
     for (int k = 0; k < n_sweeps && !rc; ++k) {
-        rc = launch_half_sweeps(c, (uint32_t)k, 0);
-        if (!rc) rc = launch_half_sweeps(c, (uint32_t)k, 1);
+        rc = launch_sweep(c, (uint32_t)k);
     }
     if (!rc && hipEventRecord(e1, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
-    if (!rc) rc = launch_add(c, 0, (uint32_t)n_sweeps);
+    if (!rc) rc = launch_advance(c, (uint32_t)n_sweeps);
     float ms = 0.f;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "sync");
     if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventElapsedTime");
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    c->host_sweep += (uint32_t)n_sweeps;
+    advance_host(c, (uint32_t)n_sweeps);
     *ms_per_launch = (double)ms / (2.0 * n_sweeps);
     return ISING_OK;
 }
@@ -1732,7 +2162,17 @@ int ising_bandwidth_probe(ising_ctx* c, int64_t bytes, double* ms)
 int ising_set_tuning(ising_ctx* c, uint32_t flags)
 {
     if (!c) return ISING_ERR_INVALID_ARG;
-    if (flags & ~(kTuneXcdRemap | kTuneNtOwn)) return set_err(c, ISING_ERR_INVALID_ARG, "unknown tuning flags %u", flags);
+    if (flags & ~(kTuneXcdRemap | kTuneNtOwn | kTuneFusedMask))
+        return set_err(c, ISING_ERR_INVALID_ARG, "unknown tuning flags %u", flags);
+    const int lc = (int)((flags & kTuneFusedMask) >> kTuneFusedShift);
+    if (lc > 0 && !c->fused_ok)
+        return set_err(c, ISING_ERR_UNSUPPORTED,
+                       "fused sweep unavailable: a z plane of %u sites per colour does not fit in LDS", c->fused_pstride);
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (lc != fused_lc(c)) {
+        int rc = fused_layout(c, lc);
+        if (rc) return rc;
+    }
     c->tune = flags;
     return ISING_OK;
 }
@@ -1744,20 +2184,53 @@ int ising_get_tuning(const ising_ctx* c, uint32_t* flags)
     return ISING_OK;
 }
 
+// Flag sets the autotuner times: the half-sweep pair with and without
+// die-contiguous blocks and non-temporal own-colour access, then (when the
+// planes fit in LDS) the fused sweep with 8, 16, 32 or all z planes per
+// workgroup, with and without die-contiguous blocks.
+static std::vector<uint32_t> tuning_candidates(const ising_ctx* c)
+{
+    std::vector<uint32_t> v = { 0u, kTuneXcdRemap, kTuneNtOwn, kTuneXcdRemap | kTuneNtOwn };
+    if (c->fused_ok) {
+        std::vector<int> lcs;
+        for (int lc : { 8, 16, 32, c->zmax }) {
+            lc = std::min(std::min(lc, c->zmax), 255);
+            if (lc > 0 && std::find(lcs.begin(), lcs.end(), lc) == lcs.end()) lcs.push_back(lc);
+        }
+        for (int lc : lcs) {
+            v.push_back((uint32_t)lc << kTuneFusedShift);
+            v.push_back(((uint32_t)lc << kTuneFusedShift) | kTuneXcdRemap);
+        }
+    }
+    if (v.size() > ISING_TUNE_MAX_CANDIDATES) v.resize(ISING_TUNE_MAX_CANDIDATES);
+    return v;
+}
+
+int ising_tuning_candidates(const ising_ctx* c, uint32_t* flags, int* count)
+{
+    if (!c || !count) return ISING_ERR_INVALID_ARG;
+    const std::vector<uint32_t> v = tuning_candidates(c);
+    *count = (int)v.size();
+    if (flags) std::copy(v.begin(), v.end(), flags);
+    return ISING_OK;
+}
+
 int ising_autotune(ising_ctx* c, int n_sweeps, uint32_t* best, double* ms)
 {
     if (!c || n_sweeps < 1) return ISING_ERR_INVALID_ARG;
-    double t[4];
-    uint32_t arg = 0;
-    for (uint32_t f = 0; f < 4; ++f) {
-        c->tune = f;
-        int rc = ising_time_sweeps(c, n_sweeps, &t[f]);
+    const std::vector<uint32_t> cand = tuning_candidates(c);
+    std::vector<double> t(cand.size());
+    size_t arg = 0;
+    for (size_t k = 0; k < cand.size(); ++k) {
+        int rc = ising_set_tuning(c, cand[k]);
+        if (!rc) rc = ising_time_sweeps(c, n_sweeps, &t[k]);
         if (rc) return rc;
-        if (t[f] < t[arg]) arg = f;
+        if (t[k] < t[arg]) arg = k;
     }
-    c->tune = arg;
-    if (best) *best = arg;
-    if (ms) for (int f = 0; f < 4; ++f) ms[f] = t[f];
+    int rc = ising_set_tuning(c, cand[arg]);
+    if (rc) return rc;
+    if (best) *best = cand[arg];
+    if (ms) for (size_t k = 0; k < cand.size(); ++k) ms[k] = t[k];
     return ISING_OK;
 }
 
@@ -1829,7 +2302,7 @@ int ising_mlp_collect(ising_ctx* c, int per_replica)
         c->cap_recs = cap;
     }
     hipLaunchKernelGGL(k_mlp_collect, dim3((unsigned)((add + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
-                       (const float*)c->d_spins, c->d_reps, c->d_lads, (const float*)c->d_beta_slot, c->R, per_replica,
+                       (const float*)cur_spins(c), c->d_reps, c->d_lads, (const float*)c->d_beta_slot, c->R, per_replica,
                        c->collect_calls, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
                        c->d_recs + (size_t)c->n_recs * kRecord);
     LAUNCH_CHECK(c);
@@ -1979,7 +2452,8 @@ int ising_accept_counts(ising_ctx* c, int64_t* accepted)
     if (!c || !accepted) return ISING_ERR_INVALID_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
     hipLaunchKernelGGL(k_acc_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
-                       (const unsigned long long*)c->d_acc_wave, c->d_acc);
+                       (const unsigned long long*)c->d_acc_wave, (const unsigned long long*)c->d_acc_wave_f,
+                       c->fused_wpb, (const unsigned long long*)c->d_acc_base, c->d_acc);
     LAUNCH_CHECK(c);
     HIP_TRY(c, hipMemcpyAsync(accepted, c->d_acc, c->R * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

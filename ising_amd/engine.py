@@ -23,7 +23,7 @@ from typing import Sequence
 import numpy as np
 
 from . import _capi
-from ._capi import DMI_BULK, DMI_INTERFACIAL_REF, DMI_NONE
+from ._capi import DMI_BULK, DMI_INTERFACIAL_REF, DMI_NONE, TUNE_MAX_CANDIDATES
 
 _DMI = {"none": DMI_NONE, "bulk": DMI_BULK, "interfacial_ref": DMI_INTERFACIAL_REF}
 
@@ -304,14 +304,22 @@ class Engine:
         self._check(self._lib.ising_get_tuning(self._h, C.byref(f)))
         return f.value
 
-    def autotune(self, n_sweeps: int = 20) -> tuple[int, np.ndarray]:
+    def tuning_candidates(self) -> list[int]:
+        """Flag sets that autotune() times, in order."""
+        buf = (C.c_uint32 * TUNE_MAX_CANDIDATES)()
+        n = C.c_int()
+        self._check(self._lib.ising_tuning_candidates(self._h, buf, C.byref(n)))
+        return [int(buf[k]) for k in range(n.value)]
+
+    def autotune(self, n_sweeps: int = 20) -> tuple[int, dict[int, float]]:
         """Times each sweep-tuning flag set over n_sweeps real sweeps and keeps
-        the fastest; returns (flags, ms per half-sweep launch of flags 0..3)."""
+        the fastest; returns (flags, {flags: ms per half sweep})."""
+        cand = self.tuning_candidates()
         best = C.c_uint32()
-        ms = np.zeros(4)
+        ms = np.zeros(TUNE_MAX_CANDIDATES)
         self._check(self._lib.ising_autotune(self._h, int(n_sweeps), C.byref(best),
                                              ms.ctypes.data_as(C.POINTER(C.c_double))))
-        return best.value, ms
+        return best.value, {f: float(t) for f, t in zip(cand, ms)}
 
     # -- learned proposal ---------------------------------------------------
     @staticmethod

@@ -86,11 +86,14 @@ def test_batched_uneven_256_replicas():
         with Engine(ladders, seed=303) as e:
             assert e.n_replicas == 256
             e.init_random()
+            n_cand = len(e.tuning_candidates())
+            assert any(f >> 8 for f in e.tuning_candidates())      # 48^3 planes fit: fused candidates
             if tuned:
-                flags, ms = e.autotune(2)     # 2 x 4 timing sweeps, then 2 more to match
-                e.sweep(2 * 4 - 8 + 2)
+                flags, ms = e.autotune(2)     # 2 timing sweeps per candidate, then 2 more
+                assert len(ms) == n_cand
+                e.sweep(2)
             else:
-                e.sweep(10)
+                e.sweep(2 * n_cand + 2)
             e.pt_run(n_sweeps=40, swap_every=10, measure_every=10)
             en = e.energy()
             picks = {0: e.get_spins(0), 32 + 5: e.get_spins(37), 64 + 31: e.get_spins(95)}
