@@ -76,14 +76,14 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
             "sample": f"FP64 C/OpenMP oracle, 1 replica 64^3, T=0.5, r=0.5, {n} sweeps in {dt:.1f} s"}
 
 
-def traffic_from_profiles() -> float | None:
+def counters_from_profiles() -> dict:
+    """DRAM bytes per half-sweep launch and VALU busy of the sweep kernels,
+    from the rocprofv3 counter runs summarised in profiles/."""
     p = ROOT / "profiles" / "sweep_traffic.json"
-    if p.exists():
-        try:
-            return float(json.loads(p.read_text())["hbm_bytes_per_launch"])
-        except Exception:
-            return None
-    return None
+    try:
+        return json.loads(p.read_text())
+    except (OSError, ValueError):
+        return {}
 
 
 def main() -> int:
@@ -92,8 +92,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tuning", type=int, default=None,
+                    help="with --profile-sweeps: use these launch-tuning flags instead of autotuning")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every period eagerly instead of replaying a captured hipGraph (same kernels)")
     ap.add_argument("--profile-sweeps", type=int, default=0,
-                    help="only run N sweeps plus one 256 MiB copy probe (for rocprofv3 counter runs)")
+                    help="only run N plain sweeps, N learned sweeps (untrained network) and one 256 MiB "
+                         "copy probe (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the path shards by PT ladder with no data exchange, "
@@ -111,11 +116,19 @@ def main() -> int:
     n_sites = L ** 3 * N_REP
 
     if args.profile_sweeps:
-        flags, _ = eng.autotune(5)
+        from ising_amd.learned import initial_params
+        if args.tuning is None:
+            flags, _ = eng.autotune(5)
+        else:
+            flags = args.tuning
+            eng.set_tuning(flags)
         ms = eng.time_sweeps(args.profile_sweeps)
+        eng.mlp_set_params(initial_params(scale=0.3))
+        eng.mlp_enable(True)
+        ms_learned = eng.time_sweeps(args.profile_sweeps)
         copy_ms = eng.bandwidth_probe(256 << 20)
-        print(json.dumps({"tuning": flags, "ms_per_launch": ms, "copy_probe_bytes": 2 * (256 << 20),
-                          "copy_probe_ms": copy_ms}), flush=True)
+        print(json.dumps({"tuning": flags, "ms_per_half_sweep": ms, "ms_per_half_sweep_learned": ms_learned,
+                          "copy_probe_bytes": 2 * (256 << 20), "copy_probe_ms": copy_ms}), flush=True)
         eng.close()
         return 0
 
@@ -125,13 +138,13 @@ def main() -> int:
     tune_flags, tune_ms = eng.autotune(10)
     t0 = time.perf_counter()
     n_records = generate_data(eng, sweeps=TRAIN_SWEEPS, collect_every=COLLECT_EVERY, per_replica=RECORDS_PER_REPLICA,
-                              swap_every=SWAP_EVERY)
+                              swap_every=SWAP_EVERY, use_graph=not args.eager)
     loss = train(eng, steps=TRAIN_STEPS, batch=TRAIN_BATCH, lr=TRAIN_LR)
     # Further equilibration (the trained sampler relaxes the cold replicas
     # fastest), then one device checkpoint that both timed runs start from
     # with the same random-number counters.
     eng.mlp_enable(True)
-    eng.pt_run(EQUIL_SWEEPS, SWAP_EVERY, 0, 0, use_graph=True)
+    eng.pt_run(EQUIL_SWEEPS, SWAP_EVERY, 0, 0, use_graph=not args.eager)
     eng.save_state()
     prep_s = time.perf_counter() - t0
 
@@ -141,13 +154,13 @@ def main() -> int:
         eng.restore_state()
         eng.mlp_enable(learned)
         if args.warmup > 0:
-            eng.pt_run(args.warmup * SWAP_EVERY, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True,
+            eng.pt_run(args.warmup * SWAP_EVERY, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=not args.eager,
                        winding_layer=L // 2)
         # Device-wide synchronisation around the timed region goes through the
         # library (torch is not imported: it bundles its own HIP runtime).
         eng.synchronize()
         t0 = time.perf_counter()
-        out = eng.pt_run(n_sweeps, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=True, winding_layer=L // 2)
+        out = eng.pt_run(n_sweeps, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=not args.eager, winding_layer=L // 2)
         eng.synchronize()
         wall = time.perf_counter() - t0
         e_series = out["energy"]                                  # [n_meas, R]
@@ -167,7 +180,8 @@ def main() -> int:
 
     bytes_launch = BYTES_PER_UPDATE * n_sites / 2
     achieved = bytes_launch / (fixed["ms_per_launch"] * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    counters = counters_from_profiles()
+    traffic = counters.get("hbm_bytes_per_launch")
     copy_ms = eng.bandwidth_probe(256 << 20)
     mlp_tflops = MLP_FLOPS_PER_UPDATE * n_sites / 2 / (learned["ms_per_launch"] * 1e-3) / 1e12
 
@@ -210,6 +224,7 @@ def main() -> int:
                             "mean": float((acc[:-1] / np.maximum(att[:-1], 1)).mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "valu_busy": counters.get("valu_busy"), "learned_valu_busy": counters.get("learned_valu_busy"),
                      "kernel": "k_half_sweep", "ms_per_launch": fixed["ms_per_launch"],
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "copy_probe_GBs": 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9,

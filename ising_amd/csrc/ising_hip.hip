@@ -1113,6 +1113,7 @@ struct ising_ctx {
     float* d_mlp_grad = nullptr; // [kMlpParams + 1], last = loss sum
     float* d_wave_grad = nullptr;
     int wave_grad_waves = 0;
+    float* d_grad_part = nullptr;   // [kMlpReduceGroups][kMlpParams + 1]
     float* d_recs = nullptr;     // training records [cap][kRecord]
     int64_t n_recs = 0, cap_recs = 0;
     uint32_t collect_calls = 0;
@@ -1535,7 +1536,7 @@ void ising_destroy(ising_ctx* c)
                      c->d_beta_slot, c->d_beta_temp, c->d_slot_of_temp, c->d_lad_of_pos, c->d_lad_start,
                      c->d_lad_n, c->d_partial, c->d_energy, c->d_mag, c->d_qpartial, c->d_qraw, c->d_qint,
                      c->d_ctr, c->d_acc, c->d_acc_wave, c->d_wave_part, c->d_mlp, c->d_mlp_prm, c->d_mlp_m1,
-                     c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de,
+                     c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_grad_part, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de,
                      c->ck_spins, c->ck_sot, c->ck_bslot, c->ck_btemp, c->ck_ctr,
                      c->d_fblock_rep, c->d_acc_wave_f, c->d_acc_base, c->d_wave_part_f };
     for (void* p : ptrs)
@@ -1657,6 +1658,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if ((rc = dalloc(c, &c->d_mlp_m1, kMlpParams))) return fail(rc);
     if ((rc = dalloc(c, &c->d_mlp_m2, kMlpParams))) return fail(rc);
     if ((rc = dalloc(c, &c->d_mlp_grad, kMlpParams + 1))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_grad_part, (size_t)kMlpReduceGroups * (kMlpParams + 1)))) return fail(rc);
     c->learned.assign(n_ladders, 0);
     if ((rc = dalloc(c, &c->d_swap_att, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_swap_acc, R))) return fail(rc);
@@ -2352,8 +2354,12 @@ static int mlp_grad_launch(ising_ctx* c, const float* recs, uint32_t n_recs, uin
     hipLaunchKernelGGL(k_mlp_grad, dim3(blocks), dim3(256), 0, c->stream, recs, n_recs, batch, step, hashed,
                        (const MlpDev*)c->d_mlp, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_wave_grad);
     LAUNCH_CHECK(c);
+    const int groups = std::min(kMlpReduceGroups, waves), per_group = (waves + groups - 1) / groups;
+    hipLaunchKernelGGL(k_mlp_reduce_groups, dim3((kMlpParams + 1 + 255) / 256, groups), dim3(256), 0, c->stream,
+                       (const float*)c->d_wave_grad, waves, per_group, c->d_grad_part);
+    LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, c->stream,
-                       (const float*)c->d_wave_grad, waves, c->d_mlp_grad);
+                       (const float*)c->d_grad_part, groups, c->d_mlp_grad);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }

@@ -370,13 +370,30 @@ __global__ __launch_bounds__(256) void k_mlp_grad(
     if (l == 0) { out[kOffB3] = t3; out[kMlpParams] = tl; }
 }
 
-// grad[p] = sum over wavefronts, fixed order (deterministic).
-__global__ void k_mlp_reduce(const float* __restrict__ wave_grad, int n_waves, float* __restrict__ grad)
+// Deterministic sum of the per-wavefront gradients (and loss, entry
+// kMlpParams) in two fixed-order stages: group g = blockIdx.y of
+// `per_group` consecutive wavefronts into part[g][p], then the groups in
+// order into grad[p].
+constexpr int kMlpReduceGroups = 32;
+
+__global__ void k_mlp_reduce_groups(const float* __restrict__ wave_grad, int n_waves, int per_group,
+                                    float* __restrict__ part)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kMlpParams) return;
+    const int w0 = blockIdx.y * per_group, w1 = min(w0 + per_group, n_waves);
+    float s = 0.f;
+#pragma unroll 8
+    for (int w = w0; w < w1; ++w) s += wave_grad[(size_t)w * (kMlpParams + 1) + p];
+    part[(size_t)blockIdx.y * (kMlpParams + 1) + p] = s;
+}
+
+__global__ void k_mlp_reduce(const float* __restrict__ part, int n_groups, float* __restrict__ grad)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > kMlpParams) return;
     float s = 0.f;
-    for (int w = 0; w < n_waves; ++w) s += wave_grad[(size_t)w * (kMlpParams + 1) + p];
+    for (int g = 0; g < n_groups; ++g) s += part[(size_t)g * (kMlpParams + 1) + p];
     grad[p] = s;
 }
 

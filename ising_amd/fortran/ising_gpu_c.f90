@@ -22,6 +22,14 @@ module ising_gpu_c
         integer(c_int32_t), parameter :: ISING_DMI_BULK = 1
         integer(c_int32_t), parameter :: ISING_DMI_INTERFACIAL_REF = 2
 
+        ! launch tuning flags (ISING_TUNE_FUSED(lc) = lc * 2**ISING_TUNE_FUSED_SHIFT)
+        integer(c_int32_t), parameter :: ISING_TUNE_XCD_REMAP = 1
+        integer(c_int32_t), parameter :: ISING_TUNE_NT_OWN = 2
+        integer(c_int32_t), parameter :: ISING_TUNE_FUSED_SHIFT = 8
+        integer(c_int), parameter :: ISING_TUNE_MAX_CANDIDATES = 16
+        integer(c_int), parameter :: ISING_MLP_PARAMS = 1793
+        integer(c_int), parameter :: ISING_MLP_RECORD = 32
+
         type, bind(C) :: ising_lattice
                 integer(c_int32_t) :: lx, ly, lz
                 integer(c_int32_t) :: periodic(3)
@@ -239,6 +247,124 @@ module ising_gpu_c
                 function ising_synchronize(ctx) bind(C, name="ising_synchronize") result(rc)
                         import :: c_ptr, c_int
                         type(c_ptr), value :: ctx
+                        integer(c_int) :: rc
+                end function
+                function ising_time_sweeps(ctx, n_sweeps, ms_per_launch) bind(C, name="ising_time_sweeps") result(rc)
+                        import :: c_ptr, c_int, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: n_sweeps
+                        real(c_double), intent(out) :: ms_per_launch
+                        integer(c_int) :: rc
+                end function
+                function ising_bandwidth_probe(ctx, bytes, ms) bind(C, name="ising_bandwidth_probe") result(rc)
+                        import :: c_ptr, c_int, c_int64_t, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int64_t), value :: bytes
+                        real(c_double), intent(out) :: ms
+                        integer(c_int) :: rc
+                end function
+                function ising_set_tuning(ctx, flags) bind(C, name="ising_set_tuning") result(rc)
+                        import :: c_ptr, c_int, c_int32_t
+                        type(c_ptr), value :: ctx
+                        integer(c_int32_t), value :: flags
+                        integer(c_int) :: rc
+                end function
+                function ising_get_tuning(ctx, flags) bind(C, name="ising_get_tuning") result(rc)
+                        import :: c_ptr, c_int, c_int32_t
+                        type(c_ptr), value :: ctx
+                        integer(c_int32_t), intent(out) :: flags
+                        integer(c_int) :: rc
+                end function
+                function ising_tuning_candidates(ctx, flags, count) bind(C, name="ising_tuning_candidates") result(rc)
+                        import :: c_ptr, c_int, c_int32_t
+                        type(c_ptr), value :: ctx
+                        integer(c_int32_t), intent(out) :: flags(*)
+                        integer(c_int), intent(out) :: count
+                        integer(c_int) :: rc
+                end function
+                function ising_autotune(ctx, n_sweeps, best, ms) bind(C, name="ising_autotune") result(rc)
+                        import :: c_ptr, c_int, c_int32_t, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: n_sweeps
+                        integer(c_int32_t), intent(out) :: best
+                        real(c_double), intent(out) :: ms(*)
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_set_params(ctx, params, n) bind(C, name="ising_mlp_set_params") result(rc)
+                        import :: c_ptr, c_int, c_float
+                        type(c_ptr), value :: ctx
+                        real(c_float), intent(in) :: params(*)
+                        integer(c_int), value :: n
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_get_params(ctx, params, n) bind(C, name="ising_mlp_get_params") result(rc)
+                        import :: c_ptr, c_int, c_float
+                        type(c_ptr), value :: ctx
+                        real(c_float), intent(out) :: params(*)
+                        integer(c_int), value :: n
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_enable(ctx, ladder, on) bind(C, name="ising_mlp_enable") result(rc)
+                        import :: c_ptr, c_int
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: ladder, on
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_collect(ctx, per_replica) bind(C, name="ising_mlp_collect") result(rc)
+                        import :: c_ptr, c_int
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: per_replica
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_num_records(ctx, n) bind(C, name="ising_mlp_num_records") result(rc)
+                        import :: c_ptr, c_int, c_int64_t
+                        type(c_ptr), value :: ctx
+                        integer(c_int64_t), intent(out) :: n
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_clear_records(ctx) bind(C, name="ising_mlp_clear_records") result(rc)
+                        import :: c_ptr, c_int
+                        type(c_ptr), value :: ctx
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_get_records(ctx, out, n) bind(C, name="ising_mlp_get_records") result(rc)
+                        import :: c_ptr, c_int, c_int64_t, c_float
+                        type(c_ptr), value :: ctx
+                        real(c_float), intent(out) :: out(*)
+                        integer(c_int64_t), value :: n
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_train(ctx, steps, batch, lr, loss) bind(C, name="ising_mlp_train") result(rc)
+                        import :: c_ptr, c_int, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: steps, batch
+                        real(c_double), value :: lr
+                        real(c_double), intent(out) :: loss(*)
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_forward(ctx, x, n, f) bind(C, name="ising_mlp_forward") result(rc)
+                        import :: c_ptr, c_int, c_float
+                        type(c_ptr), value :: ctx
+                        real(c_float), intent(in) :: x(*)
+                        integer(c_int), value :: n
+                        real(c_float), intent(out) :: f(*)
+                        integer(c_int) :: rc
+                end function
+                function ising_mlp_grad(ctx, records, n, step, loss, grad) bind(C, name="ising_mlp_grad") result(rc)
+                        import :: c_ptr, c_int, c_int32_t, c_float, c_double
+                        type(c_ptr), value :: ctx
+                        real(c_float), intent(in) :: records(*)
+                        integer(c_int), value :: n
+                        integer(c_int32_t), value :: step
+                        real(c_double), intent(out) :: loss
+                        real(c_float), intent(out) :: grad(*)
+                        integer(c_int) :: rc
+                end function
+                function ising_mfma_probe(ctx, a, b, d) bind(C, name="ising_mfma_probe") result(rc)
+                        import :: c_ptr, c_int, c_float
+                        type(c_ptr), value :: ctx
+                        real(c_float), intent(in) :: a(*), b(*)
+                        real(c_float), intent(out) :: d(*)
                         integer(c_int) :: rc
                 end function
                 function c_strlen(s) bind(C, name="strlen") result(n)

@@ -35,14 +35,15 @@ def initial_params(log_r0: float = float(np.log(0.5)), seed: int = 1, scale: flo
 
 
 def generate_data(engine: Engine, sweeps: int = 2000, collect_every: int = 20, per_replica: int = 4096,
-                  swap_every: int = 10) -> int:
+                  swap_every: int = 10, use_graph: bool = True) -> int:
     """Plain PT sweeps with record collection; returns the number of records."""
     engine.mlp_enable(False)
     engine.mlp_clear_records()
     done = 0
     while done < sweeps:
         n = min(collect_every, sweeps - done)
-        engine.pt_run(n, swap_every=swap_every if n >= swap_every else 0, measure_every=0, winding_every=0)
+        engine.pt_run(n, swap_every=swap_every if n >= swap_every else 0, measure_every=0, winding_every=0,
+                      use_graph=use_graph)
         done += n
         engine.mlp_collect(per_replica)
     return engine.mlp_num_records()

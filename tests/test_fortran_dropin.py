@@ -24,6 +24,17 @@ def run(cmd, env=None, timeout=120):
     return subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=e)
 
 
+def test_binding_declares_every_header_function():
+    """ising_gpu_c.f90 gives Fortran callers every entry point of the header."""
+    import re
+    root = Path(__file__).resolve().parents[1]
+    header = (root / "include" / "ising_gpu.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|void|const char\*)\s+(ising_\w+)\s*\(", header, re.M))
+    binding = (root / "ising_amd" / "fortran" / "ising_gpu_c.f90").read_text()
+    bound = set(re.findall(r'bind\(C, name="(ising_\w+)"\)', binding))
+    assert declared and declared <= bound, sorted(declared - bound)
+
+
 @needs_fortran
 def test_driver_rejects_bad_cli():
     p = run([str(LIB / "pt_gpu"), "1", "2"])
