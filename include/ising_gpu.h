@@ -146,6 +146,12 @@ int  ising_magnetisation(ising_ctx* ctx, double* m);
 int  ising_winding(ising_ctx* ctx, int replica, int z, int64_t* q, double* q_raw);
 /* All layers of all replicas: q[R][Zmax], q_raw[R][Zmax]. */
 int  ising_winding_all(ising_ctx* ctx, int64_t* q, double* q_raw);
+/* Replaces calculate_winding_number_density(chainMesh, Z_index, density_matrix)
+ * (src/reciprocalSpaceProcesses.f90:438-500): per-plaquette charge density of
+ * layer z (0-based), density[j*lx + i] = (sigma1 + sigma2) / (4 pi) for the
+ * plaquette with corner (i, j) -- the column-major order of
+ * density_matrix(numCellsX, numCellsY); it sums to the layer's q_raw. */
+int  ising_winding_density(ising_ctx* ctx, int replica, int z, double* density);
 /* Test hook standing in for calculateHeisenbergEnergy
  * (src/energyMin.f90:86-171): FP32 device dE = E(S') - E(S) of single-site
  * proposals props[n][3] at sites[n] (reference site order) of one replica. */

@@ -91,6 +91,7 @@ SIGNATURES = {
     "ising_magnetisation": (C.c_int, [_P, _dp]),
     "ising_winding": (C.c_int, [_P, C.c_int, C.c_int, _i64p, _dp]),
     "ising_winding_all": (C.c_int, [_P, _i64p, _dp]),
+    "ising_winding_density": (C.c_int, [_P, C.c_int, C.c_int, _dp]),
     "ising_delta_energy": (C.c_int, [_P, C.c_int, _i32p, _dp, C.c_int, _dp]),
     "ising_pt_run": (C.c_int, [_P, C.POINTER(ising_schedule), C.POINTER(ising_series)]),
     "ising_temperature_map": (C.c_int, [_P, _i32p]),

@@ -890,6 +890,33 @@ __global__ __launch_bounds__(kBlock) void k_winding(
     if (threadIdx.x == 0) qpartial[blockIdx.x] = v[0];
 }
 
+// Charge density of layer z of one slot: out[j*lx + i] = (sigma(s1,s2,s4) +
+// sigma(s1,s4,s3)) / (4 pi) per plaquette, the terms k_winding sums (0 where
+// a corner lies across an open face).
+__global__ __launch_bounds__(kBlock) void k_winding_density(
+    const float* __restrict__ spins, const Rep* __restrict__ reps, int slot, int z, double* __restrict__ out)
+{
+    const Rep R = reps[slot];
+    const uint32_t lx = (uint32_t)R.lx, ly = (uint32_t)R.ly;
+    const uint32_t pl = blockIdx.x * kBlock + threadIdx.x;
+    if (pl >= lx * ly) return;
+    const uint32_t j = fdiv(pl, R.div_lx);
+    const uint32_t i = pl - j * lx;
+    const bool px = R.per & 1, py = (R.per >> 1) & 1;
+    double d = 0.0;
+    if (((i + 1u < lx) || px) && ((j + 1u < ly) || py)) {
+        const uint32_t i2 = (i + 1u == lx) ? 0u : i + 1u;
+        const uint32_t j2 = (j + 1u == ly) ? 0u : j + 1u;
+        double s1[3], s2[3], s3[3], s4[3];
+        renorm3(spin_at(spins, R, i, j, (uint32_t)z), s1);
+        renorm3(spin_at(spins, R, i2, j, (uint32_t)z), s2);
+        renorm3(spin_at(spins, R, i, j2, (uint32_t)z), s3);
+        renorm3(spin_at(spins, R, i2, j2, (uint32_t)z), s4);
+        d = (triangle_sigma(s1, s2, s4) + triangle_sigma(s1, s4, s3)) / (4.0 * M_PI);
+    }
+    out[pl] = d;
+}
+
 // One workgroup per (slot, layer): fixed-order sum, Q = sum / (4 pi). In
 // one-layer mode (layer >= 0) the other layers read 0.
 __global__ __launch_bounds__(kBlock) void k_winding_finalize(
@@ -1904,6 +1931,27 @@ int ising_winding_all(ising_ctx* c, int64_t* q, double* q_raw)
     if (q) HIP_TRY(c, hipMemcpyAsync(q, c->d_qint, cnt * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     if (q_raw) HIP_TRY(c, hipMemcpyAsync(q_raw, c->d_qraw, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_winding_density(ising_ctx* c, int replica, int z, double* density)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    if (!density) return set_err(c, ISING_ERR_INVALID_ARG, "null density");
+    const Rep& r = c->reps[replica];
+    if (z < 0 || z >= r.lz) return set_err(c, ISING_ERR_INVALID_ARG, "layer %d out of range", z);
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t n = (size_t)r.lx * r.ly;
+    double* d = nullptr;
+    if ((rc = dalloc(c, &d, n))) return rc;
+    hipLaunchKernelGGL(k_winding_density, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                       (const float*)cur_spins(c), c->d_reps, replica, z, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(density, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(c, ISING_ERR_HIP, "winding density: %s", hipGetErrorString(e));
     return ISING_OK;
 }
 

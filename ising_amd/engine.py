@@ -225,6 +225,13 @@ class Engine:
                                                 qr.ctypes.data_as(C.POINTER(C.c_double))))
         return q, qr
 
+    def winding_density(self, replica: int, z: int) -> np.ndarray:
+        """Per-plaquette charge density of layer z, [ly, lx] (x fastest)."""
+        lat = self.replica_lattice[replica]
+        d = np.empty(lat.lx * lat.ly)
+        self._check(self._lib.ising_winding_density(self._h, replica, z, d.ctypes.data_as(C.POINTER(C.c_double))))
+        return d.reshape(lat.ly, lat.lx)
+
     def delta_energy(self, replica: int, sites: np.ndarray, props: np.ndarray) -> np.ndarray:
         s = np.ascontiguousarray(sites, dtype=np.int32)
         p = np.ascontiguousarray(props, dtype=np.float64).reshape(-1, 3)

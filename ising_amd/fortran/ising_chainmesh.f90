@@ -16,6 +16,7 @@ module ChainMesh
                 integer :: numCellsX = 0, numCellsY = 0, numCellsZ = 0
                 integer :: numAtoms = 0
                 logical :: periodic(3) = [.true., .true., .true.]
+                real(kind=8) :: a = 1.0_8     ! lattice constant (positions in output files)
                 real(kind=8), allocatable :: atomSpins(:,:)
                 type(c_ptr) :: gpu = c_null_ptr
                 integer(c_int64_t) :: seed = 0

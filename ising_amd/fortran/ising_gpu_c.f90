@@ -192,6 +192,13 @@ module ising_gpu_c
                         real(c_double), intent(out) :: q_raw
                         integer(c_int) :: rc
                 end function
+                function ising_winding_density(ctx, replica, z, density) bind(C, name="ising_winding_density") result(rc)
+                        import :: c_ptr, c_int, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: replica, z
+                        real(c_double), intent(out) :: density(*)
+                        integer(c_int) :: rc
+                end function
                 function ising_winding_all(ctx, q, q_raw) bind(C, name="ising_winding_all") result(rc)
                         import :: c_ptr, c_int, c_double, c_int64_t
                         type(c_ptr), value :: ctx

@@ -23,6 +23,7 @@ program pt_gpu
         use iso_fortran_env, only: error_unit, dp => real64
         use ising_gpu_c
         use PT_Utils, only: indicesFromSlot
+        use reciprocal_space_processes, only: write_density_line
         implicit none
 
         real(dp), parameter :: Kb = 8.6173303e-5_dp
@@ -35,7 +36,7 @@ program pt_gpu
         integer :: numIterations, numMCSSweepsPerSwap, numRelax
         real(dp) :: JMin, JMax, DMin, DMax, BMin, BMax
         integer :: NJ, ND, NB, NumParams, i, j, k, JIndex, DIndex, BIndex, slot, mid
-        character(len=400) :: outputPath, outputPathSpins
+        character(len=400) :: outputPath, outputPathSpins, outputPathDensity
         character(len=500) :: string_buff
         character(len=:), allocatable :: output_string, filename_string
         real(dp), allocatable :: ParamArray(:,:)
@@ -46,7 +47,7 @@ program pt_gpu
         integer(c_int) :: rc
         integer(c_int32_t), allocatable :: slot_of_temp(:)
         integer(c_int64_t), allocatable :: q(:)
-        real(c_double), allocatable :: qr(:), mag(:), spins(:)
+        real(c_double), allocatable :: qr(:), mag(:), spins(:), density(:)
         type(ising_schedule) :: sched
         type(ising_series) :: series
         integer :: fileHandle, nsites, ix, iy, iz, site
@@ -122,8 +123,11 @@ program pt_gpu
         rc = ising_magnetisation(ctx, mag)
         call ising_check(rc, ctx, "pt_gpu: magnetisation")
         outputPathSpins = trim(adjustl(outputPath)) // "_spins"
+        outputPathDensity = trim(adjustl(outputPath)) // "_density"
         call execute_command_line("mkdir -p " // trim(adjustl(outputPath)))
         call execute_command_line("mkdir -p " // trim(adjustl(outputPathSpins)))
+        call execute_command_line("mkdir -p " // trim(adjustl(outputPathDensity)))
+        allocate(density(numCellsX * numCellsY))
         mid = max(1, numCellsZ / 2)
         do i = 1, NumParams
                 do j = 1, numTemps
@@ -162,6 +166,23 @@ program pt_gpu
                                                 write(fileHandle, '(ES22.14, 5(",", ES22.14))') a_bravais * ix, &
                                                         a_bravais * iy, a_bravais * iz, spins(site), &
                                                         spins(nsites + site), spins(2 * nsites + site)
+                                        end do
+                                end do
+                        end do
+                        close(fileHandle)
+                        ! Charge density per plaquette as x,y,z,Winding_Density
+                        ! (write_winding_number_density, src/reciprocalSpaceProcesses.f90:544-577).
+                        filename_string = trim(adjustl(outputPathDensity)) // "/density_" // &
+                                trim(adjustl(string_buff)) // ".csv"
+                        open(newunit=fileHandle, status="replace", action="write", file=filename_string)
+                        write(fileHandle, '(A)') "x,y,z,Winding_Density"
+                        do iz = 1, numCellsZ
+                                rc = ising_winding_density(ctx, int(slot, c_int), int(iz - 1, c_int), density)
+                                call ising_check(rc, ctx, "pt_gpu: density")
+                                do ix = 1, numCellsX
+                                        do iy = 1, numCellsY
+                                                call write_density_line(fileHandle, a_bravais * ix, a_bravais * iy, &
+                                                        a_bravais * iz, density(ix + numCellsX * (iy - 1)))
                                         end do
                                 end do
                         end do

@@ -22,6 +22,53 @@ contains
                 call ising_check(rc, chainMesh%gpu, "calculate_winding_number2")
                 winding_number = qr
         end function calculate_winding_number2
+
+        ! Replaces calculate_winding_number_density (src/reciprocalSpaceProcesses.f90:438-500):
+        ! density_matrix(i, j) = (sigma1 + sigma2) / (4 pi) of the plaquette with
+        ! corner cell (i, j) of layer Z_index (1-based), from the device.
+        subroutine calculate_winding_number_density(chainMesh, Z_index, density_matrix)
+                type(ChainMesh_t), intent(inout) :: chainMesh
+                integer, intent(in) :: Z_index
+                real(kind=8), dimension(:,:), allocatable, intent(out) :: density_matrix
+                integer(c_int) :: rc
+                if (Z_index < 1 .or. Z_index > chainMesh%numCellsZ) error stop "Z_index out of bounds"
+                allocate(density_matrix(chainMesh%numCellsX, chainMesh%numCellsY))
+                call chainmesh_push_spins(chainMesh)
+                rc = ising_winding_density(chainMesh%gpu, 0_c_int, int(Z_index - 1, c_int), density_matrix)
+                call ising_check(rc, chainMesh%gpu, "calculate_winding_number_density")
+        end subroutine calculate_winding_number_density
+
+        ! Replaces write_winding_number_density (src/reciprocalSpaceProcesses.f90:544-577):
+        ! header "x,y,z,Winding_Density", then for every layer i, cell j along x and
+        ! k along y the position (j, k, i) * a (1-based cell indices, lattice
+        ! constant a of the mesh) and the density, each F0.8.
+        subroutine write_winding_number_density(chainMesh, filepath)
+                type(ChainMesh_t), intent(inout) :: chainMesh
+                character(len=*), intent(in) :: filepath
+                real(kind=8), dimension(:,:), allocatable :: density_matrix
+                integer :: i, j, k, unit, stat
+                open(newunit=unit, file=filepath, status="replace", action="write", iostat=stat)
+                if (stat /= 0) error stop "Error opening file to write density information"
+                write(unit, '(A)') "x,y,z,Winding_Density"
+                do i = 1, chainMesh%numCellsZ
+                        call calculate_winding_number_density(chainMesh, i, density_matrix)
+                        do j = 1, chainMesh%numCellsX
+                                do k = 1, chainMesh%numCellsY
+                                        call write_density_line(unit, chainMesh%a * dble(j), chainMesh%a * dble(k), &
+                                                                chainMesh%a * dble(i), density_matrix(j, k))
+                                end do
+                        end do
+                end do
+                close(unit)
+        end subroutine write_winding_number_density
+
+        subroutine write_density_line(unit, x, y, z, d)
+                integer, intent(in) :: unit
+                real(kind=8), intent(in) :: x, y, z, d
+                character(len=160) :: buf
+                write(buf, '(F0.8, A, F0.8, A, F0.8, A, F0.8)') x, ",", y, ",", z, ",", d
+                write(unit, '(A)') trim(adjustl(buf))
+        end subroutine write_density_line
 end module reciprocal_space_processes
 
 module PT_Utils

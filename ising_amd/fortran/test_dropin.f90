@@ -16,6 +16,8 @@ program test_dropin
         real(kind=8) :: J, Dz, B, beta, pi
         integer :: skyr, i, ix, iy, iz, k
         real(kind=8) :: q, th
+        real(kind=8), allocatable :: density(:,:)
+        integer :: peak(2)
 
         pi = acos(-1.0_8)
         J = -1.0_8; Dz = 0.6_8; B = 0.3_8; beta = 2.0_8
@@ -75,6 +77,11 @@ program test_dropin
         print '(A,ES24.16)', "sk_winding_middle ", winding_middle
         print '(A,I0)', "sk_number ", skyr
         print '(A,ES24.16)', "sk_layer1 ", calculate_winding_number2(fm, 1)
+        call calculate_winding_number_density(fm, 2, density)
+        print '(A,ES24.16)', "sk_density_sum ", sum(density)
+        print '(A,ES24.16)', "sk_density_max ", maxval(abs(density))
+        peak = maxloc(abs(density))
+        print '(A,I0,A,I0)', "sk_density_peak ", peak(1), " ", peak(2)
         call deallocateChainMesh(mesh)
         call deallocateChainMesh(fm)
         print '(A)', "done"

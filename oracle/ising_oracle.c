@@ -17,6 +17,9 @@
  *   orc_ref_total_energy    Neel-spiral closed form; equals orc_total_energy in reference mode
  *   orc_ref_winding_number  ferromagnet -> 0, skyrmion -> +-1, Q(-S) = -Q(S), mirror -> -Q
  *   orc_winding_layer       same invariants; equals orc_ref_winding_number to 1e-6
+ *   orc_ref_winding_density sums to orc_ref_winding_number per layer; FM -> 0
+ *   orc_winding_density     sums to orc_winding_layer; odd under S -> -S; equals
+ *                           orc_ref_winding_density to 1e-6 (tests/test_oracle_winding.py)
  *   orc_propose             analytic distribution of S.S' (KS test), symmetry
  *   orc_sweeps              single-spin Langevin <Sz>, two-spin dimer quadrature,
  *                           high-temperature series (tests/test_oracle_metropolis.py)
@@ -368,6 +371,38 @@ double orc_ref_winding_number(const orc_lattice* L, const double* s, int zi)
     return wn / (4.0 * pi_ref);
 }
 
+/* calculate_winding_number_density (src/reciprocalSpaceProcesses.f90:438-500):
+ * per plaquette (i, j) of layer zi (1-based), the same two triangles as
+ * calculate_winding_number2, density(i, j) = (sigma1 + sigma2) / (4 pi) with
+ * the single-precision pi; out[j*lx + i] (x fastest, the column-major order of
+ * density_matrix(numCellsX, numCellsY)). */
+void orc_ref_winding_density(const orc_lattice* L, const double* s, int zi, double* out)
+{
+    const int64_t N = nsites(L);
+    const double pi_ref = (double)3.14159265358979323846f;
+    for (int i = 0; i < L->lx; ++i) {
+        for (int j = 0; j < L->ly; ++j) {
+            int64_t a1 = orc_site(L, i, j, zi - 1);
+            int64_t a2 = orc_neighbour(L, a1, 0, +1);
+            int64_t a4 = orc_neighbour(L, a2, 1, +1);
+            int64_t a3 = orc_neighbour(L, a1, 1, +1);
+            double v[4][3];
+            int64_t ids[4] = { a1, a2, a3, a4 };
+            for (int k = 0; k < 4; ++k) {
+                load(s, N, ids[k], v[k]);
+                double n = 0.0;
+                n = n + v[k][0] * v[k][0];
+                n = n + v[k][1] * v[k][1];
+                n = n + v[k][2] * v[k][2];
+                n = sqrt(n);
+                v[k][0] /= n; v[k][1] /= n; v[k][2] /= n;
+            }
+            out[(int64_t)j * L->lx + i] = (orc_ref_arc_winding(v[0], v[1], v[3]) + orc_ref_arc_winding(v[0], v[3], v[2]))
+                                          / (4.0 * pi_ref);
+        }
+    }
+}
+
 /* ---- topological charge, build form --------------------------------- */
 
 /* Solid angle of one triangle from FP32 spins, every operation in FP64 in a
@@ -420,6 +455,31 @@ double orc_winding_layer(const orc_lattice* L, const double* s, int z, int64_t* 
     double q = acc / (4.0 * M_PI);
     if (q_int) *q_int = (int64_t)llround(q);
     return q;
+}
+
+/* Per-plaquette topological charge density of layer z (0-based), build form:
+ * out[j*lx + i] = (sigma(s1,s2,s4) + sigma(s1,s4,s3)) / (4 pi), the terms of
+ * orc_winding_layer; 0 for plaquettes skipped at open boundaries. */
+void orc_winding_density(const orc_lattice* L, const double* s, int z, double* out)
+{
+    const int64_t N = nsites(L);
+    for (int i = 0; i < L->lx; ++i) {
+        for (int j = 0; j < L->ly; ++j) {
+            double d = 0.0;
+            int64_t a1 = orc_site(L, i, j, z);
+            int64_t a2 = orc_neighbour(L, a1, 0, +1);
+            int64_t a3 = orc_neighbour(L, a1, 1, +1);
+            if (a2 >= 0 && a3 >= 0) {
+                int64_t a4 = orc_neighbour(L, a2, 1, +1);
+                float v[4][3];
+                int64_t ids[4] = { a1, a2, a3, a4 };
+                for (int k = 0; k < 4; ++k)
+                    for (int m = 0; m < 3; ++m) v[k][m] = (float)s[m * N + ids[k]];
+                d = (orc_triangle_sigma(v[0], v[1], v[3]) + orc_triangle_sigma(v[0], v[3], v[2])) / (4.0 * M_PI);
+            }
+            out[(int64_t)j * L->lx + i] = d;
+        }
+    }
 }
 
 /* ---- Metropolis ------------------------------------------------------ */

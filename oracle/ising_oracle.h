@@ -65,6 +65,8 @@ double  orc_ref_winding_number(const orc_lattice* L, const double* s, int z_inde
 /* ---- topological charge (build form, same op order as the GPU) ---- */
 double  orc_triangle_sigma(const float a[3], const float b[3], const float c[3]);
 double  orc_winding_layer(const orc_lattice* L, const double* s, int z, int64_t* q_int);
+void    orc_winding_density(const orc_lattice* L, const double* s, int z, double* out);
+void    orc_ref_winding_density(const orc_lattice* L, const double* s, int z_index_1based, double* out);
 
 /* ---- Metropolis ---- */
 int     orc_propose(const double sold[3], double radius, const uint32_t rnd[4], double snew[3], double* u_acc);

@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
             "orc_ref_arc_winding": (C.c_double, [dp, dp, dp]),
             "orc_ref_winding_number": (C.c_double, [lp, dp, C.c_int]),
             "orc_winding_layer": (C.c_double, [lp, dp, C.c_int, C.POINTER(C.c_int64)]),
+            "orc_winding_density": (None, [lp, dp, C.c_int, dp]),
+            "orc_ref_winding_density": (None, [lp, dp, C.c_int, dp]),
             "orc_propose": (C.c_int, [dp, C.c_double, u32p, dp, dp]),
             "orc_half_sweep": (C.c_int64, [lp, hp, dp, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
                                            C.c_uint32, C.c_int]),
@@ -184,6 +186,22 @@ def ref_arc_winding(a, b, c) -> float:
 def ref_winding_number(L, s, z1) -> float:
     s = _spins(s)
     return lib().orc_ref_winding_number(C.byref(L), _dp(s), z1)
+
+
+def winding_density(L, s, z) -> np.ndarray:
+    """Per-plaquette charge density of layer z (0-based), [ly, lx] (x fastest)."""
+    s = _spins(s)
+    out = np.zeros(L.lx * L.ly)
+    lib().orc_winding_density(C.byref(L), _dp(s), z, _dp(out))
+    return out.reshape(L.ly, L.lx)
+
+
+def ref_winding_density(L, s, z1) -> np.ndarray:
+    """calculate_winding_number_density restated (z1 1-based), [ly, lx]."""
+    s = _spins(s)
+    out = np.zeros(L.lx * L.ly)
+    lib().orc_ref_winding_density(C.byref(L), _dp(s), z1, _dp(out))
+    return out.reshape(L.ly, L.lx)
 
 
 def winding_layer(L, s, z) -> tuple[int, float]:

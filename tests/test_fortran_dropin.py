@@ -82,6 +82,12 @@ def test_dropin_entry_points_on_gpu():
     assert np.allclose(v["fm_mag"], [0, 0, 1]) and v["fm_winding"][0] == 0.0
     assert abs(abs(v["sk_winding_middle"][0]) - 1) < 1e-9 and v["sk_number"][0] == 1
     assert abs(v["sk_layer1"][0] - v["sk_winding_middle"][0]) < 1e-12
+    # calculate_winding_number_density of the same skyrmion: sums to the layer's
+    # charge, peaks at the plaquette whose corner cell is next to the centre
+    # (7.7, 7.8) -> 1-based (8, 8) or its neighbours
+    assert abs(v["sk_density_sum"][0] - v["sk_winding_middle"][0]) < 1e-12
+    assert 0 < v["sk_density_max"][0] < 1
+    assert all(abs(c - 7.5) <= 1.5 for c in v["sk_density_peak"])
 
 
 @pytest.mark.gpu
@@ -104,3 +110,16 @@ def test_pt_driver_reference_cli_and_csv(tmp_path):
     data = np.loadtxt(spins[0], delimiter=",", skiprows=1)
     assert data.shape == (30 * 30 * 6, 6)
     assert np.allclose(np.linalg.norm(data[:, 3:], axis=1), 1.0, atol=1e-6)
+    # write_winding_number_density: x,y,z,Winding_Density per plaquette, layer
+    # sums equal to the layer charge (middle layer in the data file, F0.4)
+    dens = sorted((tmp_path / "run_density").glob("density_*.csv"))
+    assert len(dens) == 20 and [d.name[8:] for d in dens] == [f.name[5:] for f in files]
+    assert dens[0].read_text().splitlines()[0] == "x,y,z,Winding_Density"
+    for df, f in zip(dens, files):
+        d = np.loadtxt(df, delimiter=",", skiprows=1)
+        assert d.shape == (30 * 30 * 6, 4)
+        assert np.allclose(d[:, :3] / 2.8, np.rint(d[:, :3] / 2.8), atol=1e-6)
+        assert d[:, 2].min() == pytest.approx(2.8) and d[:, 2].max() == pytest.approx(6 * 2.8)
+        mid = d[np.isclose(d[:, 2], 3 * 2.8), 3].sum()
+        row = [float(x) for x in list(csv.reader(f.open()))[1]]
+        assert abs(mid - row[4]) < 1e-4 + 1e-6 * 900

@@ -13,6 +13,7 @@ import pytest
 
 from gpu_util import de_tolerance, engine, olat, random_unit
 from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
+from ising_amd._capi import tune_fused
 from oracle import pyoracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -179,6 +180,31 @@ def test_winding_exact(kind):
     assert (q1, qr1) == (q[0, 2], qr[0, 2])
     if kind.startswith("skyrmion"):
         assert abs(q[0, 0]) == 1
+
+
+@pytest.mark.parametrize("lname", ["periodic_8x6x4", "open_z_8x8x5", "open_all_6x5x3"])
+def test_winding_density_exact(lname):
+    """ising_winding_density against the oracle's per-plaquette density (same
+    FP64 operation order: equal to 1e-15 absolute), its layer sums against
+    the layer charge (1e-12), and the reference restatement (1e-6)."""
+    lat = LATS[lname]
+    rng = np.random.default_rng(21)
+    st = random_unit(rng, lat.sites)
+    with engine(lat, HAMS["bulk_K"], [1.0, 0.5]) as e:
+        e.set_spins(1, st)
+        e.set_tuning(tune_fused(2))
+        e.sweep(1)                      # a fused sweep: the state moves to the other spin buffer
+        s = e.get_spins(1)
+        dens = [e.winding_density(1, z) for z in range(lat.lz)]
+        _, qr = e.winding_all()
+    lo = olat(lat)
+    for z in range(lat.lz):
+        d = dens[z]
+        assert d.shape == (lat.ly, lat.lx)
+        assert np.max(np.abs(d - orc.winding_density(lo, s, z))) <= 1e-15
+        assert abs(d.sum() - qr[1, z]) < 1e-12
+        if all(lat.periodic):
+            assert np.max(np.abs(d - orc.ref_winding_density(lo, s, z + 1))) < 1e-6
 
 
 def test_sweep_decision_replay():

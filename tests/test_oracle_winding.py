@@ -93,3 +93,42 @@ def test_mirror_flips_sign(orc):
     q, _ = orc.winding_layer(L, v, 0)
     qm, _ = orc.winding_layer(L, m, 0)
     assert abs(q) == 1 and qm == -q
+
+
+def test_density_ferromagnet_and_sums(orc):
+    """Charge density per plaquette (calculate_winding_number_density): zero
+    for a ferromagnet; per layer it sums to the layer's charge, for both the
+    reference restatement (single-precision pi) and the build form."""
+    L = orc.lattice(16, 16, 2)
+    s = np.zeros((3, 512)); s[2] = 1
+    assert not np.any(orc.winding_density(L, s, 0))
+    assert not np.any(orc.ref_winding_density(L, s, 1))
+    rng = np.random.default_rng(12)
+    L = orc.lattice(12, 10, 3)
+    v = rng.normal(size=(3, 360)); v /= np.linalg.norm(v, axis=0)
+    v = v.astype(np.float32).astype(np.float64)
+    for z in range(3):
+        d = orc.winding_density(L, v, z)
+        q, raw = orc.winding_layer(L, v, z)
+        assert d.shape == (10, 12) and abs(d.sum() - raw) < 1e-12
+        dr = orc.ref_winding_density(L, v, z + 1)
+        assert abs(dr.sum() - orc.ref_winding_number(L, v, z + 1)) < 1e-12
+        assert np.max(np.abs(dr - d)) < 1e-6                     # float pi, float Im in the reference
+        assert np.array_equal(orc.winding_density(L, -v, z), -d)  # odd under S -> -S, bit for bit
+
+
+def test_density_locates_skyrmion(orc):
+    """A skyrmion centred at (cx, cy): the density integrates to -1/+1, is
+    concentrated within its radius, and its centroid sits at the centre."""
+    L = orc.lattice(24, 24, 1)
+    cx, cy = 10.7, 13.4
+    s = skyrmion(24, 24, 1, radius=3.0, cx=cx, cy=cy)
+    d = orc.winding_density(L, s, 0)
+    q, _ = orc.winding_layer(L, s, 0)
+    assert abs(q) == 1 and abs(d.sum() - q) < 1e-9
+    iy, ix = np.mgrid[0:24, 0:24]
+    w = d / d.sum()
+    # plaquette (i, j) spans corners i..i+1, j..j+1: its centre is (i + 1/2, j + 1/2)
+    assert abs((w * (ix + 0.5)).sum() - cx) < 0.3 and abs((w * (iy + 0.5)).sum() - cy) < 0.3
+    near = np.hypot(ix + 0.5 - cx, iy + 0.5 - cy) < 6
+    assert abs(w[near].sum() - 1) < 0.02
