@@ -51,9 +51,18 @@ extern "C" {
 
 typedef struct ising_ctx ising_ctx;
 
+/* ---- unit cells ------------------------------------------------------- */
+#define ISING_CELL_SC  0  /* one atom per cell, 6 nearest neighbours          */
+#define ISING_CELL_BCC 1  /* atoms at (0,0,0) and (1/2,1/2,1/2), 8 nearest
+                             neighbours (src/unit_cells.f90:7-30)             */
+
 typedef struct {
-    int32_t lx, ly, lz;      /* cells per axis, one atom per cell (SC)      */
+    int32_t lx, ly, lz;      /* cells per axis                              */
     int32_t periodic[3];     /* 1 = periodic, 0 = open, per axis x,y,z      */
+    int32_t cell;            /* ISING_CELL_*; sites = lx*ly*lz*(atoms per
+                                cell), in the reference order
+                                ((ix*ly + iy)*lz + iz)*atoms + atom
+                                (src/chainMesh.f90:684,878)                 */
 } ising_lattice;
 
 typedef struct {

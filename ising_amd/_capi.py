@@ -24,6 +24,7 @@ ERRORS = {
     -8: "ISING_ERR_STATE",
 }
 DMI_NONE, DMI_BULK, DMI_INTERFACIAL_REF = 0, 1, 2
+CELL_SC, CELL_BCC = 0, 1
 MLP_NPARAM = 1793
 MLP_RECORD = 32
 TUNE_XCD_REMAP = 1
@@ -39,7 +40,8 @@ def tune_fused(planes: int) -> int:
 
 
 class ising_lattice(C.Structure):
-    _fields_ = [("lx", C.c_int32), ("ly", C.c_int32), ("lz", C.c_int32), ("periodic", C.c_int32 * 3)]
+    _fields_ = [("lx", C.c_int32), ("ly", C.c_int32), ("lz", C.c_int32), ("periodic", C.c_int32 * 3),
+                ("cell", C.c_int32)]
 
 
 class ising_hamiltonian(C.Structure):

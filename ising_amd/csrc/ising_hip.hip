@@ -83,7 +83,9 @@ __device__ __forceinline__ uint32_t fmodq(uint32_t n, uint32_t quot, const FastD
 
 // Per replica storage slot.
 struct Rep {
-    int32_t lx, ly, lz, hx;
+    int32_t lx, ly, lz;
+    int32_t hx;             // row length in one colour's arrays: lx/2 (SC), lx (bcc)
+    int32_t cell;           // ISING_CELL_SC / ISING_CELL_BCC
     int32_t per;            // bit 0/1/2: axis x/y/z periodic
     int32_t ladder;
     int32_t first_block;    // first workgroup in the sweep/measure grid
@@ -201,16 +203,11 @@ __device__ __forceinline__ void load_nbrs(const Rep& R, const Planes& oth, const
     }
 }
 
-// Local field h = J sum_j S_j - sum_a (S_{+a} - S_{-a}) x d_a + Bz z.
-__device__ __forceinline__ f3 field_from(const Lad& L, const f3 (&v)[6])
+// Local field h = J sum - sum_a A_a x d_a + Bz z from the sum of the
+// neighbour spins and the signed sums A_a = sum_b sgn_a(b) S_b over the bonds
+// (SC: A_a = S_{+a} - S_{-a}; bcc: eight bonds, d_a pre-scaled by 1/sqrt 3).
+__device__ __forceinline__ f3 field_from_sums(const Lad& L, const f3& sum, const f3& dx, const f3& dy, const f3& dz)
 {
-    const f3 &xp = v[0], &xm = v[1], &yp = v[2], &ym = v[3], &zp = v[4], &zm = v[5];
-    const f3 sum{ (xp.x + xm.x) + (yp.x + ym.x) + (zp.x + zm.x),
-                  (xp.y + xm.y) + (yp.y + ym.y) + (zp.y + zm.y),
-                  (xp.z + xm.z) + (yp.z + ym.z) + (zp.z + zm.z) };
-    const f3 dx{ xp.x - xm.x, xp.y - xm.y, xp.z - xm.z };
-    const f3 dy{ yp.x - ym.x, yp.y - ym.y, yp.z - ym.z };
-    const f3 dz{ zp.x - zm.x, zp.y - zm.y, zp.z - zm.z };
     f3 dm{ 0.f, 0.f, 0.f };
     if (L.dmi == ISING_DMI_BULK) {
         const float D = L.d[0];
@@ -222,6 +219,97 @@ __device__ __forceinline__ f3 field_from(const Lad& L, const f3 (&v)[6])
         dm.z = (dx.x * d[1] - dx.y * d[0]) + (dy.x * d[4] - dy.y * d[3]) + (dz.x * d[7] - dz.y * d[6]);
     }
     return f3{ L.J * sum.x - dm.x, L.J * sum.y - dm.y, L.J * sum.z - dm.z + L.Bz };
+}
+
+// SC: h = J sum_j S_j - sum_a (S_{+a} - S_{-a}) x d_a + Bz z.
+__device__ __forceinline__ f3 field_from(const Lad& L, const f3 (&v)[6])
+{
+    const f3 &xp = v[0], &xm = v[1], &yp = v[2], &ym = v[3], &zp = v[4], &zm = v[5];
+    const f3 sum{ (xp.x + xm.x) + (yp.x + ym.x) + (zp.x + zm.x),
+                  (xp.y + xm.y) + (yp.y + ym.y) + (zp.y + zm.y),
+                  (xp.z + xm.z) + (yp.z + ym.z) + (zp.z + zm.z) };
+    const f3 dx{ xp.x - xm.x, xp.y - xm.y, xp.z - xm.z };
+    const f3 dy{ yp.x - ym.x, yp.y - ym.y, yp.z - ym.z };
+    const f3 dz{ zp.x - zm.x, zp.y - zm.y, zp.z - zm.z };
+    return field_from_sums(L, sum, dx, dy, dz);
+}
+
+// bcc: the eight neighbours (the other atom) of the site at q = row*lx + ix
+// of cell (ix, iy, iz) with colour (= atom) k are the cells (ix - dx, iy - dy,
+// iz - dz) for k = 0 and (ix + dx, iy + dy, iz + dz) for k = 1, dx, dy, dz in
+// {0, 1}; the bond vector's sign on an axis is +1 (k = 0) or -1 (k = 1) for
+// d = 0 and the opposite for d = 1. Neighbours across open faces count as
+// zero (the index is then the periodic image, always in range). Returns the
+// sum of the spins and the signed sums A_a.
+struct BccSums { f3 sum, ax, ay, az; };
+
+// Steps (modular uint32) from q to the shifted cell along x, y, z and whether
+// that cell exists; bond b = (dx, dy, dz) reaches q + dx*ox + dy*oy + dz*oz.
+struct BccSteps { uint32_t ox, oy, oz; bool okx, oky, okz; };
+
+__device__ __forceinline__ BccSteps bcc_steps(const Rep& R, uint32_t ix, uint32_t iy, uint32_t iz, int colour)
+{
+    const uint32_t lx = (uint32_t)R.lx, ly = (uint32_t)R.ly, lz = (uint32_t)R.lz;
+    const uint32_t plane = ly * lx;
+    BccSteps t;
+    if (colour == 0) {
+        t.ox = ix == 0u ? lx - 1u : 0u - 1u;                     t.okx = ix > 0u || (R.per & 1);
+        t.oy = iy == 0u ? (ly - 1u) * lx : 0u - lx;              t.oky = iy > 0u || ((R.per >> 1) & 1);
+        t.oz = iz == 0u ? (lz - 1u) * plane : 0u - plane;        t.okz = iz > 0u || ((R.per >> 2) & 1);
+    } else {
+        t.ox = ix + 1u == lx ? 0u - (lx - 1u) : 1u;              t.okx = ix + 1u < lx || (R.per & 1);
+        t.oy = iy + 1u == ly ? 0u - (ly - 1u) * lx : lx;         t.oky = iy + 1u < ly || ((R.per >> 1) & 1);
+        t.oz = iz + 1u == lz ? 0u - (lz - 1u) * plane : plane;   t.okz = iz + 1u < lz || ((R.per >> 2) & 1);
+    }
+    return t;
+}
+
+__device__ __forceinline__ uint32_t bcc_index(const BccSteps& t, uint32_t q, int b)
+{
+    return q + (((b >> 2) & 1) ? t.ox : 0u) + (((b >> 1) & 1) ? t.oy : 0u) + ((b & 1) ? t.oz : 0u);
+}
+
+__device__ __forceinline__ bool bcc_ok(const BccSteps& t, int b)
+{
+    return (!((b >> 2) & 1) || t.okx) && (!((b >> 1) & 1) || t.oky) && (!(b & 1) || t.okz);
+}
+
+__device__ __forceinline__ BccSums bcc_sums(const Rep& R, const Planes& oth, uint32_t q, uint32_t ix, uint32_t iy,
+                                            uint32_t iz, int colour)
+{
+    const BccSteps st = bcc_steps(R, ix, iy, iz, colour);
+    f3 v[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v[b] = ld3(oth, bcc_index(st, q, b));
+    if (R.per != 7) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            if (!bcc_ok(st, b)) v[b] = f3{ 0.f, 0.f, 0.f };
+    }
+    // Sums over d = 0 and d = 1 per axis.
+    f3 x0{ 0.f, 0.f, 0.f }, x1 = x0, y0 = x0, y1 = x0, z0 = x0, z1 = x0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const f3 t = v[b];
+        f3& xs = (b >> 2) & 1 ? x1 : x0;
+        f3& ys = (b >> 1) & 1 ? y1 : y0;
+        f3& zs = b & 1 ? z1 : z0;
+        xs.x += t.x; xs.y += t.y; xs.z += t.z;
+        ys.x += t.x; ys.y += t.y; ys.z += t.z;
+        zs.x += t.x; zs.y += t.y; zs.z += t.z;
+    }
+    const float sg = colour == 0 ? 1.f : -1.f;
+    BccSums r;
+    r.sum = f3{ x0.x + x1.x, x0.y + x1.y, x0.z + x1.z };
+    r.ax = f3{ sg * (x0.x - x1.x), sg * (x0.y - x1.y), sg * (x0.z - x1.z) };
+    r.ay = f3{ sg * (y0.x - y1.x), sg * (y0.y - y1.y), sg * (y0.z - y1.z) };
+    r.az = f3{ sg * (z0.x - z1.x), sg * (z0.y - z1.y), sg * (z0.z - z1.z) };
+    return r;
+}
+
+__device__ __forceinline__ f3 bcc_field(const Lad& L, const BccSums& b)
+{
+    return field_from_sums(L, b.sum, b.ax, b.ay, b.az);
 }
 
 __device__ __forceinline__ f3 local_field(const Rep& R, const Lad& L, const Planes& oth, const Nbr& nb)
@@ -265,7 +353,8 @@ __device__ __forceinline__ float* spin_buffer(const SpinBuf& b, uint32_t fused_d
 }
 
 // Site number (ix*ly + iy)*lz + iz of the Philox counter, in 24-bit
-// multiplies when the replica has fewer than 2^24 sites.
+// multiplies when the replica has fewer than 2^24 sites (on bcc this is the
+// cell number; the site is 2*cell + atom).
 __device__ __forceinline__ uint32_t site_number(const Rep& R, uint32_t ix, uint32_t iy, uint32_t iz)
 {
     return R.n < (1 << 24) ? __umul24(__umul24(ix, (uint32_t)R.ly) + iy, (uint32_t)R.lz) + iz
@@ -371,14 +460,24 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
     const uint32_t h = fmodq(q, row, R.div_hx);
     const uint32_t iz = fdiv(row, R.div_ly);
     const uint32_t iy = fmodq(row, iz, R.div_ly);
-    const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
-    const uint32_t ix = 2u * h + p;
     const f3 s = nt ? f3{ ldf_nt(own.x, q), ldf_nt(own.y, q), ldf_nt(own.z, q) } : ld3(own, q);
     f3 nbr[6];
-    load_nbrs(R, oth, neighbours<false>(R, q, h, iy, iz, p), nbr);
-    const f3 hf = field_from(L, nbr);
+    f3 hf;
+    uint32_t site;
+    if (R.cell == ISING_CELL_BCC) {      // uniform per workgroup; never a learned ladder
+        hf = bcc_field(L, bcc_sums(R, oth, q, h, iy, iz, colour));
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nbr[k] = f3{ 0.f, 0.f, 0.f };
+        site = 2u * site_number(R, h, iy, iz) + (uint32_t)colour;
+    } else {
+        const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
+        const uint32_t ix = 2u * h + p;
+        load_nbrs(R, oth, neighbours<false>(R, q, h, iy, iz, p), nbr);
+        hf = field_from(L, nbr);
+        site = site_number(R, ix, iy, iz);
+    }
     bool accepted;
-    const f3 fs = metropolis_site<kLearned>(L, beta, s, nbr, hf, site_number(R, ix, iy, iz), sweep, seed_lo, seed_hi,
+    const f3 fs = metropolis_site<kLearned>(L, beta, s, nbr, hf, site, sweep, seed_lo, seed_hi,
                                             (uint32_t)slot, mlp, active, accepted);
     if (accepted) {
         if (nt) {
@@ -733,7 +832,36 @@ __global__ __launch_bounds__(kBlock) void k_measure(
     const Lad& L = lad_override ? *lad_override : lads[R.ladder];
     const uint32_t q = (uint32_t)(b - R.first_block) * kBlock + threadIdx.x;
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
-    if (q < R.half) {
+    if (q < R.half && R.cell == ISING_CELL_BCC) {
+        // Every bond once, from its atom-0 end (colour 0):
+        // -J S_i.S_j + d_b.(S_i x S_j), d_b = sum_a sgn_a(b) d_a (d_a scaled
+        // by 1/sqrt 3); on-site terms and spins of both atoms of the cell.
+        const uint32_t row = fdiv(q, R.div_hx);
+        const uint32_t ix = q - row * (uint32_t)R.hx;
+        const uint32_t iz = fdiv(row, R.div_ly);
+        const uint32_t iy = row - iz * (uint32_t)R.ly;
+        const Planes p0 = planes(spins, R, 0), p1 = planes(spins, R, 1);
+        const BccSteps st = bcc_steps(R, ix, iy, iz, 0);
+        const f3 af = ld3(p0, q);
+        const double sx = af.x, sy = af.y, sz = af.z;
+        double e = 0.0;
+        for (int b = 0; b < 8; ++b) {
+            if (!bcc_ok(st, b)) continue;
+            const f3 tf = ld3(p1, bcc_index(st, q, b));
+            const double tx = tf.x, ty = tf.y, tz = tf.z;
+            const double gx = ((b >> 2) & 1) ? -1.0 : 1.0, gy = ((b >> 1) & 1) ? -1.0 : 1.0, gz = (b & 1) ? -1.0 : 1.0;
+            const double dbx = gx * L.dd[0] + gy * L.dd[3] + gz * L.dd[6];
+            const double dby = gx * L.dd[1] + gy * L.dd[4] + gz * L.dd[7];
+            const double dbz = gx * L.dd[2] + gy * L.dd[5] + gz * L.dd[8];
+            const double cx = sy * tz - sz * ty, cy = sz * tx - sx * tz, cz = sx * ty - sy * tx;
+            e += -L.Jd * (sx * tx + sy * ty + sz * tz) + (dbx * cx + dby * cy + dbz * cz);
+        }
+        e -= L.zs * L.Bzd * sz + L.Kzd * sz * sz;
+        const f3 bf = ld3(p1, q);
+        const double ux = bf.x, uy = bf.y, uz = bf.z;
+        e -= L.zs * L.Bzd * uz + L.Kzd * uz * uz;
+        v[0] = e; v[1] = sx + ux; v[2] = sy + uy; v[3] = sz + uz;
+    } else if (q < R.half) {
         const uint32_t row = fdiv(q, R.div_hx);
         const uint32_t h = q - row * (uint32_t)R.hx;
         const uint32_t iz = fdiv(row, R.div_ly);
@@ -841,8 +969,13 @@ __device__ __forceinline__ double triangle_sigma(const double a[3], const double
     return 2.0 * atan2(im, re);
 }
 
+// Spin of the plaquette corner at cell (ix, iy, iz): the site on SC, atom 0
+// of the cell on bcc (the reference's derivativeList steps to the same atom
+// of the neighbouring cell, src/chainMesh.f90:931-1031).
 __device__ __forceinline__ f3 spin_at(const float* __restrict__ spins, const Rep& R, uint32_t ix, uint32_t iy, uint32_t iz)
 {
+    if (R.cell == ISING_CELL_BCC)
+        return ld3(planes(spins, R, 0), (iz * (uint32_t)R.ly + iy) * (uint32_t)R.hx + ix);
     const uint32_t k = (ix + iy + iz) & 1u;
     const uint32_t idx = (iz * (uint32_t)R.ly + iy) * (uint32_t)R.hx + (ix >> 1);
     return ld3(planes(spins, R, (int)k), idx);
@@ -1046,11 +1179,13 @@ __global__ __launch_bounds__(kBlock) void k_init(
     const uint32_t h = q - row * (uint32_t)R.hx;
     const uint32_t iz = fdiv(row, R.div_ly);
     const uint32_t iy = row - iz * (uint32_t)R.ly;
+    const bool bcc = R.cell == ISING_CELL_BCC;
     for (int colour = 0; colour < 2; ++colour) {
         const Planes own = planes(spins, R, colour);
         const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
-        const uint32_t ix = 2u * h + p;
-        const uint32_t site = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
+        const uint32_t ix = bcc ? h : 2u * h + p;
+        const uint32_t cellsite = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
+        const uint32_t site = bcc ? 2u * cellsite + (uint32_t)colour : cellsite;
         const u32x4 r = philox4x32_10(u32x4{ site, 0xFFFFFFFFu, kStreamInit, seed_hi }, seed_lo, (uint32_t)slot);
         float x, y, z;
         if (mode == 0) {
@@ -1078,19 +1213,21 @@ __global__ __launch_bounds__(kBlock) void k_delta_energy(
     if (e >= count) return;
     const Rep R = reps[slot];
     const Lad L = lads[R.ladder];
+    const bool bcc = R.cell == ISING_CELL_BCC;
     const uint32_t site = (uint32_t)sites[e];
-    const uint32_t iz = site % (uint32_t)R.lz;
-    const uint32_t iy = (site / (uint32_t)R.lz) % (uint32_t)R.ly;
-    const uint32_t ix = site / ((uint32_t)R.lz * (uint32_t)R.ly);
-    const uint32_t colour = (ix + iy + iz) & 1u;
+    const uint32_t cellsite = bcc ? site >> 1 : site;
+    const uint32_t iz = cellsite % (uint32_t)R.lz;
+    const uint32_t iy = (cellsite / (uint32_t)R.lz) % (uint32_t)R.ly;
+    const uint32_t ix = cellsite / ((uint32_t)R.lz * (uint32_t)R.ly);
+    const uint32_t colour = bcc ? site & 1u : (ix + iy + iz) & 1u;
     const uint32_t row = iz * (uint32_t)R.ly + iy;
-    const uint32_t h = ix >> 1;
+    const uint32_t h = bcc ? ix : ix >> 1;
     const uint32_t q = row * (uint32_t)R.hx + h;
     const Planes own = planes(spins, R, (int)colour);
     const Planes oth = planes(spins, R, (int)(colour ^ 1u));
-    const Nbr nb = neighbours<false>(R, q, h, iy, iz, ix & 1u);
     const f3 s = ld3(own, q);
-    const f3 hf = local_field(R, L, oth, nb);
+    const f3 hf = bcc ? bcc_field(L, bcc_sums(R, oth, q, ix, iy, iz, (int)colour))
+                      : local_field(R, L, oth, neighbours<false>(R, q, h, iy, iz, ix & 1u));
     const f3 sn{ props[3 * e], props[3 * e + 1], props[3 * e + 2] };
     out[e] = (double)delta_e(s, sn, hf, L.Kz);
 }
@@ -1225,13 +1362,19 @@ int dalloc(ising_ctx* c, T** p, size_t count)
     return ISING_OK;
 }
 
-Lad make_lad(const ising_hamiltonian& h, double cone)
+// Per-axis DMI vectors d_a (bulk D e_a; reference interfacial d_x = -D e_y,
+// d_y = D e_x). On bcc a bond b with unit vector sgn(b)/sqrt 3 carries
+// d_b = sum_a sgn_a(b) d_a / sqrt 3, so the stored vectors are scaled by
+// 1/sqrt 3 and the kernels combine them with the signs.
+Lad make_lad(const ising_hamiltonian& h, double cone, int cell)
 {
     Lad L;
     memset(&L, 0, sizeof L);
     double d[9] = { 0 };
     if (h.dmi_type == ISING_DMI_BULK) { d[0] = h.D; d[4] = h.D; d[8] = h.D; }
     else if (h.dmi_type == ISING_DMI_INTERFACIAL_REF) { d[1] = -h.D; d[3] = h.D; }
+    if (cell == ISING_CELL_BCC)
+        for (double& v : d) v /= std::sqrt(3.0);
     L.J = (float)h.J; L.Bz = (float)h.Bz; L.Kz = (float)h.Kz; L.r = (float)cone;
     L.lr8 = (float)(std::log2(cone) - 8.0);
     L.zsf = (float)h.zeeman_total_scale;
@@ -1244,14 +1387,21 @@ Lad make_lad(const ising_hamiltonian& h, double cone)
 int check_lattice(ising_ctx* c, const ising_lattice& l)
 {
     const int n[3] = { l.lx, l.ly, l.lz };
+    if (l.cell != ISING_CELL_SC && l.cell != ISING_CELL_BCC)
+        return set_err(c, ISING_ERR_BAD_DIMS, "unknown unit cell %d", l.cell);
+    const bool bcc = l.cell == ISING_CELL_BCC;
     for (int a = 0; a < 3; ++a) {
         if (n[a] < 1) return set_err(c, ISING_ERR_BAD_DIMS, "axis %d has %d cells", a, n[a]);
-        if (l.periodic[a] && (n[a] < 4 || (n[a] & 1)))
+        // bcc's sublattices are the two colours for any length; a periodic
+        // axis of one cell would bond an atom to the same neighbour twice
+        if (bcc && l.periodic[a] && n[a] < 2)
+            return set_err(c, ISING_ERR_BAD_DIMS, "periodic bcc axis %d needs >= 2 cells (got %d)", a, n[a]);
+        if (!bcc && l.periodic[a] && (n[a] < 4 || (n[a] & 1)))
             return set_err(c, ISING_ERR_BAD_DIMS,
                            "periodic axis %d needs an even length >= 4 for the two-colour sweep (got %d)", a, n[a]);
     }
-    if (l.lx & 1) return set_err(c, ISING_ERR_BAD_DIMS, "lx must be even (got %d)", l.lx);
-    const int64_t N = (int64_t)l.lx * l.ly * l.lz;
+    if (!bcc && (l.lx & 1)) return set_err(c, ISING_ERR_BAD_DIMS, "lx must be even (got %d)", l.lx);
+    const int64_t N = (int64_t)l.lx * l.ly * l.lz * (bcc ? 2 : 1);
     if (N >= (1ll << 31)) return set_err(c, ISING_ERR_BAD_DIMS, "lattice of %lld sites too large", (long long)N);
     return ISING_OK;
 }
@@ -1270,12 +1420,26 @@ int upload_lads(ising_ctx* c)
 {
     std::vector<Lad> l;
     for (size_t k = 0; k < c->ladders.size(); ++k) {
-        l.push_back(make_lad(c->ladders[k].ham, c->ladders[k].cone));
+        l.push_back(make_lad(c->ladders[k].ham, c->ladders[k].cone, c->ladders[k].lat.cell));
         l.back().learned = k < c->learned.size() ? c->learned[k] : 0;
     }
     HIP_TRY(c, hipMemcpyAsync(c->d_lads, l.data(), l.size() * sizeof(Lad), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return ISING_OK;
+}
+
+// Index colour*half + q in a replica's component arrays of the site with
+// reference index `site` (SC: (ix*ly + iy)*lz + iz; bcc: that cell number
+// times 2 plus the atom).
+int64_t dev_index(const Rep& r, int64_t site)
+{
+    const bool bcc = r.cell == ISING_CELL_BCC;
+    int64_t t = bcc ? site >> 1 : site;
+    const int64_t iz = t % r.lz; t /= r.lz;
+    const int64_t iy = t % r.ly;
+    const int64_t ix = t / r.ly;
+    const int64_t k = bcc ? (site & 1) : ((ix + iy + iz) & 1);
+    return k * r.half + (iz * r.ly + iy) * r.hx + (bcc ? ix : ix >> 1);
 }
 
 int check_replica(ising_ctx* c, int r)
@@ -1462,14 +1626,18 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
 int fused_setup(ising_ctx* c)
 {
     int pmax = 0, zsum = 0;
-    for (const Rep& r : c->reps) { pmax = std::max(pmax, r.plane); zsum += r.lz; }
+    bool all_sc = true;
+    for (const Rep& r : c->reps) {
+        pmax = std::max(pmax, r.plane); zsum += r.lz;
+        all_sc = all_sc && r.cell == ISING_CELL_SC;
+    }
     int lds_max = 0, optin = 0;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, c->device);
     lds_max = std::max(lds_max, optin);
     const size_t lds = (size_t)18 * pmax * sizeof(float);
     int threads = (int)((((pmax + kFusedPasses - 1) / kFusedPasses) + kWave - 1) / kWave * kWave);
-    c->fused_ok = pmax > 0 && threads <= kFusedMaxThreads && lds <= (size_t)lds_max;
+    c->fused_ok = all_sc && pmax > 0 && threads <= kFusedMaxThreads && lds <= (size_t)lds_max;
     c->fused_threads = std::max(threads, kWave);
     c->fused_wpb = c->fused_threads / kWave;
     c->fused_pstride = (uint32_t)pmax;
@@ -1615,10 +1783,12 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
         for (int k = 0; k < L.n; ++k) {
             Rep r;
             memset(&r, 0, sizeof r);
-            r.lx = L.lat.lx; r.ly = L.lat.ly; r.lz = L.lat.lz; r.hx = r.lx / 2;
+            r.lx = L.lat.lx; r.ly = L.lat.ly; r.lz = L.lat.lz;
+            r.cell = L.lat.cell;
+            r.hx = r.cell == ISING_CELL_BCC ? r.lx : r.lx / 2;
             r.per = (L.lat.periodic[0] ? 1 : 0) | (L.lat.periodic[1] ? 2 : 0) | (L.lat.periodic[2] ? 4 : 0);
             r.ladder = l;
-            r.n = (int64_t)r.lx * r.ly * r.lz;
+            r.n = (int64_t)r.lx * r.ly * r.lz * (r.cell == ISING_CELL_BCC ? 2 : 1);
             r.half = (uint32_t)(r.n / 2);
             r.off = off;
             off += 3 * r.n;
@@ -1801,18 +1971,14 @@ int ising_set_spins(ising_ctx* c, int replica, const double* soa, int64_t n)
     if (!soa || n != r.n) return set_err(c, ISING_ERR_INVALID_ARG, "spin buffer of %lld sites, replica has %lld",
                                          (long long)n, (long long)r.n);
     std::vector<float> h((size_t)3 * r.n);
-    for (int ix = 0; ix < r.lx; ++ix)
-        for (int iy = 0; iy < r.ly; ++iy)
-            for (int iz = 0; iz < r.lz; ++iz) {
-                const int64_t site = ((int64_t)ix * r.ly + iy) * r.lz + iz;
-                const double x = soa[site], y = soa[n + site], z = soa[2 * n + site];
-                const double nn = sqrt(x * x + y * y + z * z);
-                if (!std::isfinite(nn) || fabs(nn - 1.0) > 1e-3)
-                    return set_err(c, ISING_ERR_BAD_SPIN, "site %lld: |S| = %g (NaN, inf or not unit)", (long long)site, nn);
-                const int k = (ix + iy + iz) & 1;
-                const int64_t idx = (int64_t)k * r.half + ((int64_t)iz * r.ly + iy) * r.hx + (ix >> 1);
-                h[idx] = (float)x; h[r.n + idx] = (float)y; h[2 * r.n + idx] = (float)z;
-            }
+    for (int64_t site = 0; site < n; ++site) {
+        const double x = soa[site], y = soa[n + site], z = soa[2 * n + site];
+        const double nn = sqrt(x * x + y * y + z * z);
+        if (!std::isfinite(nn) || fabs(nn - 1.0) > 1e-3)
+            return set_err(c, ISING_ERR_BAD_SPIN, "site %lld: |S| = %g (NaN, inf or not unit)", (long long)site, nn);
+        const int64_t idx = dev_index(r, site);
+        h[idx] = (float)x; h[r.n + idx] = (float)y; h[2 * r.n + idx] = (float)z;
+    }
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpyAsync(cur_spins(c) + r.off, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1829,14 +1995,10 @@ int ising_get_spins(ising_ctx* c, int replica, double* soa, int64_t n)
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpyAsync(h.data(), cur_spins(c) + r.off, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    for (int ix = 0; ix < r.lx; ++ix)
-        for (int iy = 0; iy < r.ly; ++iy)
-            for (int iz = 0; iz < r.lz; ++iz) {
-                const int64_t site = ((int64_t)ix * r.ly + iy) * r.lz + iz;
-                const int k = (ix + iy + iz) & 1;
-                const int64_t idx = (int64_t)k * r.half + ((int64_t)iz * r.ly + iy) * r.hx + (ix >> 1);
-                soa[site] = h[idx]; soa[n + site] = h[r.n + idx]; soa[2 * n + site] = h[2 * r.n + idx];
-            }
+    for (int64_t site = 0; site < n; ++site) {
+        const int64_t idx = dev_index(r, site);
+        soa[site] = h[idx]; soa[n + site] = h[r.n + idx]; soa[2 * n + site] = h[2 * r.n + idx];
+    }
     return ISING_OK;
 }
 
@@ -1908,7 +2070,7 @@ int ising_energy_with(ising_ctx* c, int replica, const ising_hamiltonian* h, dou
     if ((rc = check_ham(c, *h))) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
     const Rep& r = c->reps[replica];
-    Lad L = make_lad(*h, c->ladders[r.ladder].cone);
+    Lad L = make_lad(*h, c->ladders[r.ladder].cone, r.cell);
     HIP_TRY(c, hipMemcpyAsync(c->d_lad_override, &L, sizeof L, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_measure, dim3(r.nblocks), dim3(kBlock), 0, c->stream, spin_buf(c), 0u, c->d_reps,
                        c->d_lads, (const Lad*)c->d_lad_override, c->d_block_rep, r.first_block, c->d_partial);
@@ -2217,7 +2379,8 @@ int ising_set_tuning(ising_ctx* c, uint32_t flags)
     const int lc = (int)((flags & kTuneFusedMask) >> kTuneFusedShift);
     if (lc > 0 && !c->fused_ok)
         return set_err(c, ISING_ERR_UNSUPPORTED,
-                       "fused sweep unavailable: a z plane of %u sites per colour does not fit in LDS", c->fused_pstride);
+                       "fused sweep unavailable: it needs simple-cubic replicas whose z plane of one colour "
+                       "(%u sites here) fits in LDS", c->fused_pstride);
     HIP_TRY(c, hipSetDevice(c->device));
     if (lc != fused_lc(c)) {
         int rc = fused_layout(c, lc);
@@ -2324,6 +2487,10 @@ int ising_mlp_enable(ising_ctx* c, int ladder, int on)
 {
     if (!c) return ISING_ERR_INVALID_ARG;
     if (ladder >= (int)c->ladders.size()) return set_err(c, ISING_ERR_INVALID_ARG, "ladder %d", ladder);
+    // The network's input is the six neighbours of a simple-cubic site.
+    for (int k = 0; k < (int)c->ladders.size(); ++k)
+        if (on && (ladder < 0 || k == ladder) && c->ladders[k].lat.cell != ISING_CELL_SC)
+            return set_err(c, ISING_ERR_UNSUPPORTED, "learned proposal: ladder %d is not simple cubic", k);
     HIP_TRY(c, hipSetDevice(c->device));
     for (int k = 0; k < (int)c->learned.size(); ++k)
         if (ladder < 0 || k == ladder) c->learned[k] = on ? 1 : 0;
@@ -2335,6 +2502,9 @@ int ising_mlp_enable(ising_ctx* c, int ladder, int on)
 int ising_mlp_collect(ising_ctx* c, int per_replica)
 {
     if (!c || per_replica < 1) return ISING_ERR_INVALID_ARG;
+    for (const LadHost& L : c->ladders)
+        if (L.lat.cell != ISING_CELL_SC)
+            return set_err(c, ISING_ERR_UNSUPPORTED, "training records: every ladder must be simple cubic");
     HIP_TRY(c, hipSetDevice(c->device));
     const int64_t add = (int64_t)c->R * per_replica;
     if (c->n_recs + add > (int64_t)UINT32_MAX) return set_err(c, ISING_ERR_INVALID_ARG, "too many training records");

@@ -40,15 +40,23 @@ class Lattice:
     ly: int
     lz: int
     periodic: tuple = (True, True, True)
+    cell: str = "sc"          # "sc" or "bcc" (two atoms per cell)
+
+    @property
+    def atoms_per_cell(self) -> int:
+        return 2 if self.cell == "bcc" else 1
 
     @property
     def sites(self) -> int:
-        return self.lx * self.ly * self.lz
+        return self.lx * self.ly * self.lz * self.atoms_per_cell
 
     def _c(self) -> _capi.ising_lattice:
+        if self.cell not in ("sc", "bcc"):
+            raise ValueError("cell must be 'sc' or 'bcc'")
         lat = _capi.ising_lattice(self.lx, self.ly, self.lz)
         for a in range(3):
             lat.periodic[a] = 1 if self.periodic[a] else 0
+        lat.cell = _capi.CELL_BCC if self.cell == "bcc" else _capi.CELL_SC
         return lat
 
 

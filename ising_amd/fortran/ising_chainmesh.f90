@@ -1,9 +1,10 @@
-! ising_chainmesh.f90 -- drop-in ChainMesh module for simple-cubic lattices.
+! ising_chainmesh.f90 -- drop-in ChainMesh module for simple-cubic and bcc lattices.
 !
 ! Keeps the fields of the reference's ChainMesh_t that the Monte Carlo path
 ! and the PT driver read (src/chainMesh.f90:17-47): numCellsX/Y/Z, numAtoms
 ! and atomSpins(numAtoms,3) real(8) in the reference's site order
-! (IndexFromCoordinatesInteger, src/chainMesh.f90:673-685: z fastest).
+! (IndexFromCoordinatesInteger, src/chainMesh.f90:673-685: z fastest; with a
+! two-atom bcc cell, atom a of cell c is site 2*(c-1) + a, src/chainMesh.f90:878).
 ! Each mesh owns one device context (one replica) created on first use;
 ! the spins stay host-owned and are copied in and out per call, exactly as
 ! the reference's routines mutate chainMesh%atomSpins in place.
@@ -16,6 +17,8 @@ module ChainMesh
                 integer :: numCellsX = 0, numCellsY = 0, numCellsZ = 0
                 integer :: numAtoms = 0
                 logical :: periodic(3) = [.true., .true., .true.]
+                integer :: cell = 0           ! ISING_CELL_SC or ISING_CELL_BCC
+                integer :: atomsPerCell = 1
                 real(kind=8) :: a = 1.0_8     ! lattice constant (positions in output files)
                 real(kind=8), allocatable :: atomSpins(:,:)
                 type(c_ptr) :: gpu = c_null_ptr
@@ -32,18 +35,22 @@ module ChainMesh
 
 contains
 
-        ! Simple-cubic mesh of numCellsX x numCellsY x numCellsZ cells with one
-        ! atom per cell (the reference builds general Bravais cells,
-        ! src/chainMesh.f90:729-911; only the SC case is provided here).
-        function makeChainMesh(numCellsX, numCellsY, numCellsZ, periodic) result(mesh)
+        ! Mesh of numCellsX x numCellsY x numCellsZ cubic cells, one atom per
+        ! cell (SC) or two (cell = ISING_CELL_BCC: atoms at (0,0,0) and
+        ! (1/2,1/2,1/2), src/unit_cells.f90:7-30). The reference builds general
+        ! Bravais cells (src/chainMesh.f90:729-911); these two are provided.
+        function makeChainMesh(numCellsX, numCellsY, numCellsZ, periodic, cell) result(mesh)
                 integer, intent(in) :: numCellsX, numCellsY, numCellsZ
                 logical, intent(in), optional :: periodic(3)
+                integer, intent(in), optional :: cell
                 type(ChainMesh_t) :: mesh
                 integer :: stat
                 mesh%numCellsX = numCellsX
                 mesh%numCellsY = numCellsY
                 mesh%numCellsZ = numCellsZ
-                mesh%numAtoms = numCellsX * numCellsY * numCellsZ
+                if (present(cell)) mesh%cell = cell
+                mesh%atomsPerCell = merge(2, 1, mesh%cell == ISING_CELL_BCC)
+                mesh%numAtoms = numCellsX * numCellsY * numCellsZ * mesh%atomsPerCell
                 if (present(periodic)) mesh%periodic = periodic
                 allocate(mesh%atomSpins(mesh%numAtoms, 3), stat=stat)
                 if (stat /= 0) error stop "Error: Failed to allocated atomSpins array"
@@ -83,6 +90,7 @@ contains
                 d(1)%lattice%lx = mesh%numCellsX
                 d(1)%lattice%ly = mesh%numCellsY
                 d(1)%lattice%lz = mesh%numCellsZ
+                d(1)%lattice%cell = int(mesh%cell, c_int32_t)
                 do a = 1, 3
                         d(1)%lattice%periodic(a) = merge(1_c_int32_t, 0_c_int32_t, mesh%periodic(a))
                 end do

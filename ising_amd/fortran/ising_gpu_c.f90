@@ -30,9 +30,13 @@ module ising_gpu_c
         integer(c_int), parameter :: ISING_MLP_PARAMS = 1793
         integer(c_int), parameter :: ISING_MLP_RECORD = 32
 
+        integer(c_int32_t), parameter :: ISING_CELL_SC = 0
+        integer(c_int32_t), parameter :: ISING_CELL_BCC = 1
+
         type, bind(C) :: ising_lattice
                 integer(c_int32_t) :: lx, ly, lz
                 integer(c_int32_t) :: periodic(3)
+                integer(c_int32_t) :: cell = 0
         end type ising_lattice
 
         type, bind(C) :: ising_hamiltonian

@@ -11,9 +11,10 @@
 ! Environment variables ISING_PT_ITERATIONS, ISING_PT_SWEEPS_PER_SWAP and
 ! ISING_PT_RELAX override the three counts (for short test runs).
 !
-! Differences from the reference, all deliberate: simple-cubic lattice (the
-! reference uses bcc cells, src/PT.f90:145); seeded random-sphere start (the
-! reference never sets the spins); swaps use the same beta = 1/(Kb T) as the
+! Lattice as the reference's: 30 x 30 x 6 bcc cells (src/PT.f90:36-38,145)
+! of edge a = 2 (bcc_unit_cell overwrites a_bravais, src/unit_cells.f90:21).
+! Differences from the reference, all deliberate: seeded random-sphere start
+! (the reference never sets the spins); swaps use the same beta = 1/(Kb T) as the
 ! sweeps and the textbook exponent (the reference uses 1/T and the opposite
 ! sign, src/PT.f90:256-259); one even/odd round of neighbour swaps per
 ! interval instead of numTemps random picks; no MPI ranks (one GPU holds all
@@ -30,7 +31,8 @@ program pt_gpu
         real(dp), parameter :: TMax = 20000_dp, TMin = 400_dp
         integer, parameter :: numTemps = 10
         integer, parameter :: numCellsX = 30, numCellsY = 30, numCellsZ = 6
-        real(dp), parameter :: a_bravais = 2.8_dp
+        real(dp), parameter :: a_bravais = 2.0_dp
+        integer, parameter :: atomsPerCell = 2
         real(dp), parameter :: cone_r = 0.2_dp
 
         integer :: numIterations, numMCSSweepsPerSwap, numRelax
@@ -50,7 +52,7 @@ program pt_gpu
         real(c_double), allocatable :: qr(:), mag(:), spins(:), density(:)
         type(ising_schedule) :: sched
         type(ising_series) :: series
-        integer :: fileHandle, nsites, ix, iy, iz, site
+        integer :: fileHandle, nsites, ix, iy, iz, site, atom
         real(dp) :: winding_middle, winding_spread
 
         if (command_argument_count() /= 10) then
@@ -87,7 +89,7 @@ program pt_gpu
                 do j = 1, numTemps
                         betas((i - 1) * numTemps + j) = 1.0_dp / (Kb * TemperatureArray(j))
                 end do
-                ladders(i)%lattice = ising_lattice(numCellsX, numCellsY, numCellsZ, [1, 1, 1])
+                ladders(i)%lattice = ising_lattice(numCellsX, numCellsY, numCellsZ, [1, 1, 1], ISING_CELL_BCC)
                 ! Reference parameter convention: +J S.S exchange, interfacial DMI.
                 ladders(i)%ham = ising_hamiltonian(-ParamArray(i, 1), ParamArray(i, 2), ISING_DMI_INTERFACIAL_REF, 0, &
                                                    ParamArray(i, 3), 0.0_dp, 2.0_dp)
@@ -113,7 +115,7 @@ program pt_gpu
         end if
 
         ! Statistics and files (src/PT.f90:297-364).
-        nsites = numCellsX * numCellsY * numCellsZ
+        nsites = numCellsX * numCellsY * numCellsZ * atomsPerCell
         allocate(slot_of_temp(NumParams * numTemps), q(NumParams * numTemps * numCellsZ), &
                  qr(NumParams * numTemps * numCellsZ), mag(3 * NumParams * numTemps), spins(3 * nsites))
         rc = ising_temperature_map(ctx, slot_of_temp)
@@ -162,10 +164,15 @@ program pt_gpu
                         do ix = 0, numCellsX - 1
                                 do iy = 0, numCellsY - 1
                                         do iz = 0, numCellsZ - 1
-                                                site = (ix * numCellsY + iy) * numCellsZ + iz + 1
-                                                write(fileHandle, '(ES22.14, 5(",", ES22.14))') a_bravais * ix, &
-                                                        a_bravais * iy, a_bravais * iz, spins(site), &
-                                                        spins(nsites + site), spins(2 * nsites + site)
+                                                do atom = 0, atomsPerCell - 1
+                                                        site = ((ix * numCellsY + iy) * numCellsZ + iz) * atomsPerCell &
+                                                               + atom + 1
+                                                        write(fileHandle, '(ES22.14, 5(",", ES22.14))') &
+                                                                a_bravais * (ix + 0.5_dp * atom), &
+                                                                a_bravais * (iy + 0.5_dp * atom), &
+                                                                a_bravais * (iz + 0.5_dp * atom), spins(site), &
+                                                                spins(nsites + site), spins(2 * nsites + site)
+                                                end do
                                         end do
                                 end do
                         end do

@@ -9,6 +9,9 @@
  *   orc_philox4x32_10       Random123 known-answer vectors (tests/test_oracle_rng.py)
  *   orc_ref_uniform         reference rand.f90 built into oracle/_ref (tests/test_oracle_ref.py)
  *   orc_neighbour           brute-force enumeration on tiny lattices (tests/test_oracle_geometry.py)
+ *   orc_bcc_bonds           brute-force distance enumeration of bcc positions (tests/test_oracle_bcc.py)
+ *   bcc energies            closed forms (ferromagnet, Neel, bulk and interfacial helices),
+ *                           ref restatement in reference mode (tests/test_oracle_bcc.py)
  *   orc_total_energy        closed forms: ferromagnet, bulk helix (tests/test_oracle_energy.py)
  *   orc_delta_energy        E(after) - E(before) of orc_total_energy, plus closed forms
  *   orc_local_field         via orc_delta_energy linearity in the proposal
@@ -126,6 +129,62 @@ int64_t orc_neighbour(const orc_lattice* L, int64_t site, int axis, int dir)
     return orc_site(L, c[0], c[1], c[2]);
 }
 
+/* bcc: site ((ix*ly + iy)*lz + iz)*2 + atom; atom 1 sits at (1/2,1/2,1/2) of
+ * its cell. */
+int64_t orc_bcc_site(const orc_lattice* L, int ix, int iy, int iz, int atom)
+{
+    return (((int64_t)ix * L->ly + iy) * L->lz + iz) * 2 + atom;
+}
+
+static inline void bcc_decode(const orc_lattice* L, int64_t s, int c[3], int* atom)
+{
+    *atom = (int)(s & 1);
+    s >>= 1;
+    c[2] = (int)(s % L->lz); s /= L->lz;
+    c[1] = (int)(s % L->ly); s /= L->ly;
+    c[0] = (int)s;
+}
+
+/* The eight nearest neighbours of a bcc site, at distance sqrt(3)/2 cells:
+ * atom 0 of cell c bonds to atom 1 of the cells c - (dx,dy,dz), dx,dy,dz in
+ * {0,1}, along (1/2 - dx, 1/2 - dy, 1/2 - dz); atom 1 bonds to atom 0 of the
+ * cells c + (dx,dy,dz) along (dx - 1/2, dy - 1/2, dz - 1/2). sgn[k] holds the
+ * signs of bond k's vector (site -> neighbour). Bonds across open faces are
+ * left out; returns the number of bonds. */
+int orc_bcc_bonds(const orc_lattice* L, int64_t site, int64_t nb[8], int sgn[8][3])
+{
+    int c[3], a;
+    bcc_decode(L, site, c, &a);
+    const int n[3] = { L->lx, L->ly, L->lz };
+    int k = 0;
+    for (int b = 0; b < 8; ++b) {
+        const int d[3] = { (b >> 2) & 1, (b >> 1) & 1, b & 1 };
+        int cc[3], ok = 1;
+        for (int ax = 0; ax < 3 && ok; ++ax) {
+            cc[ax] = a == 0 ? c[ax] - d[ax] : c[ax] + d[ax];
+            if (cc[ax] < 0 || cc[ax] >= n[ax]) {
+                if (!L->periodic[ax]) ok = 0;
+                else cc[ax] = (cc[ax] + n[ax]) % n[ax];
+            }
+        }
+        if (!ok) continue;
+        nb[k] = orc_bcc_site(L, cc[0], cc[1], cc[2], 1 - a);
+        for (int ax = 0; ax < 3; ++ax) sgn[k][ax] = a == 0 ? 1 - 2 * d[ax] : 2 * d[ax] - 1;
+        ++k;
+    }
+    return k;
+}
+
+/* Colour of a site in the two-colour sweep: (ix+iy+iz) mod 2 on SC, the atom
+ * of the cell on bcc (both nearest-neighbour graphs are bipartite). */
+static inline int site_colour(const orc_lattice* L, int64_t i)
+{
+    if (L->cell == ORC_CELL_BCC) return (int)(i & 1);
+    int64_t iz = i % L->lz, r = i / L->lz;
+    int64_t iy = r % L->ly, ix = r / L->ly;
+    return (int)((ix + iy + iz) & 1);
+}
+
 /* DMI vector d_a of the bond i -> i + r_a.  Bulk: D r_a.  Reference
  * interfacial form D_ij = Dz (c x r_{j->i}) with r_{j->i} = -r_a
  * (src/energyMin.f90:114,135-137). */
@@ -154,14 +213,49 @@ static inline void load(const double* s, int64_t N, int64_t i, double v[3])
 {
     v[0] = s[i]; v[1] = s[N + i]; v[2] = s[2 * N + i];
 }
-static inline int64_t nsites(const orc_lattice* L) { return (int64_t)L->lx * L->ly * L->lz; }
+static inline int64_t nsites(const orc_lattice* L)
+{
+    return (int64_t)L->lx * L->ly * L->lz * (L->cell == ORC_CELL_BCC ? 2 : 1);
+}
+int64_t orc_nsites(const orc_lattice* L) { return nsites(L); }
 
 /* ---- energies (build form) ----------------------------------------- */
 
 /* h_i = J sum_j S_j - sum_a (S_{i+a} - S_{i-a}) x d_a + Bz z.  The
  * anisotropy term is not part of h (it depends on the site's own spin). */
+/* bcc: h_i = J sum_b S_b - sum_b S_b x d_b + Bz z over the eight bonds, with
+ * d_b = (1/sqrt 3) sum_a sgn_a(b) d_a for the unit bond vector sgn(b)/sqrt 3
+ * (the per-axis vectors d_a of orc_dmi_vectors: bulk d_b = D r_b, interfacial
+ * d_b = D (z x r_{j->i})). */
+static void bcc_bond_dmi(const double d[3][3], const int sg[3], double db[3])
+{
+    const double inv = 1.0 / sqrt(3.0);
+    for (int m = 0; m < 3; ++m) db[m] = inv * (sg[0] * d[0][m] + sg[1] * d[1][m] + sg[2] * d[2][m]);
+}
+
+static void bcc_local_field(const orc_lattice* L, const orc_ham* h, const double* s, int64_t site, double out[3])
+{
+    const int64_t N = nsites(L);
+    double d[3][3];
+    orc_dmi_vectors(h, d);
+    int64_t nb[8];
+    int sg[8][3];
+    const int nbonds = orc_bcc_bonds(L, site, nb, sg);
+    double sum[3] = { 0, 0, 0 }, dm[3] = { 0, 0, 0 };
+    for (int k = 0; k < nbonds; ++k) {
+        double sj[3], db[3], c[3];
+        load(s, N, nb[k], sj);
+        bcc_bond_dmi(d, sg[k], db);
+        cross(sj, db, c);
+        for (int m = 0; m < 3; ++m) { sum[m] += sj[m]; dm[m] += c[m]; }
+    }
+    for (int m = 0; m < 3; ++m) out[m] = h->J * sum[m] - dm[m];
+    out[2] += h->Bz;
+}
+
 void orc_local_field(const orc_lattice* L, const orc_ham* h, const double* s, int64_t site, double out[3])
 {
+    if (L->cell == ORC_CELL_BCC) { bcc_local_field(L, h, s, site, out); return; }
     const int64_t N = nsites(L);
     double d[3][3];
     orc_dmi_vectors(h, d);
@@ -190,8 +284,39 @@ double orc_delta_energy(const orc_lattice* L, const orc_ham* h, const double* s,
     return -dot(del, hf) - h->Kz * del[2] * (sn[2] + so[2]);
 }
 
+/* bcc: every bond once, from its atom-0 end: -J S_i.S_j + d_b.(S_i x S_j),
+ * plus the on-site terms of every site. */
+static double bcc_total_energy(const orc_lattice* L, const orc_ham* h, const double* s)
+{
+    const int64_t N = nsites(L);
+    double d[3][3];
+    orc_dmi_vectors(h, d);
+    double e = 0.0;
+    #pragma omp parallel for reduction(+:e) schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        double si[3], sj[3], c[3], db[3];
+        load(s, N, i, si);
+        double ei = 0.0;
+        if ((i & 1) == 0) {
+            int64_t nb[8];
+            int sg[8][3];
+            const int nbonds = orc_bcc_bonds(L, i, nb, sg);
+            for (int k = 0; k < nbonds; ++k) {
+                load(s, N, nb[k], sj);
+                cross(si, sj, c);
+                bcc_bond_dmi(d, sg[k], db);
+                ei += -h->J * dot(si, sj) + dot(db, c);
+            }
+        }
+        ei -= h->zeeman_total_scale * h->Bz * si[2] + h->Kz * si[2] * si[2];
+        e += ei;
+    }
+    return e;
+}
+
 double orc_total_energy(const orc_lattice* L, const orc_ham* h, const double* s)
 {
+    if (L->cell == ORC_CELL_BCC) return bcc_total_energy(L, h, s);
     const int64_t N = nsites(L);
     double d[3][3];
     orc_dmi_vectors(h, d);
@@ -222,6 +347,26 @@ void orc_magnetisation(const orc_lattice* L, const double* s, double m[3])
     m[0] = mx / (double)N; m[1] = my / (double)N; m[2] = mz / (double)N;
 }
 
+/* Corners (i,j), (i+1,j), (i,j+1), (i+1,j+1) of the plaquette of layer z:
+ * the atom-0 sites of those cells on bcc (the reference's derivativeList
+ * steps to the same atom of the next cell, src/chainMesh.f90:931-1031);
+ * returns 0 when a corner lies across an open face. ids = {s1, s2, s3, s4}
+ * with s2 = +x, s3 = +y, s4 = +x+y. */
+static int plaquette(const orc_lattice* L, int i, int j, int z, int64_t ids[4])
+{
+    int i2 = i + 1, j2 = j + 1;
+    if (i2 == L->lx) { if (!L->periodic[0]) return 0; i2 = 0; }
+    if (j2 == L->ly) { if (!L->periodic[1]) return 0; j2 = 0; }
+    if (L->cell == ORC_CELL_BCC) {
+        ids[0] = orc_bcc_site(L, i, j, z, 0); ids[1] = orc_bcc_site(L, i2, j, z, 0);
+        ids[2] = orc_bcc_site(L, i, j2, z, 0); ids[3] = orc_bcc_site(L, i2, j2, z, 0);
+    } else {
+        ids[0] = orc_site(L, i, j, z); ids[1] = orc_site(L, i2, j, z);
+        ids[2] = orc_site(L, i, j2, z); ids[3] = orc_site(L, i2, j2, z);
+    }
+    return 1;
+}
+
 /* ---- reference restatements ----------------------------------------- */
 
 /* Minimum-image displacement p2 - p1 on a periodic SC lattice of unit cell
@@ -236,39 +381,60 @@ static void ref_min_image(const orc_lattice* L, const double p1[3], const double
     }
 }
 
-/* calculateHeisenbergEnergy (src/energyMin.f90:86-171) for an SC lattice with
- * J' = 0 and demag off: old/new local energy of site i for proposal sp.
- * For each nearest neighbour j: r = -(x_j - x_i)_min/|.|, D = Dz (c x r),
- * energy += J S.S_j + D.(S x S_j); then -B S_z. */
+/* Atom position in cell units (the bcc body atom at +1/2 on every axis,
+ * src/unit_cells.f90:27-28). */
+static void ref_position(const orc_lattice* L, int64_t site, double p[3])
+{
+    int c[3], a = 0;
+    if (L->cell == ORC_CELL_BCC) bcc_decode(L, site, c, &a);
+    else decode(L, site, c);
+    for (int k = 0; k < 3; ++k) p[k] = c[k] + 0.5 * a;
+}
+
+/* Nearest-neighbour list of a site of a periodic lattice, as the reference's
+ * atomShells(i,1) holds it (src/chainMesh.f90:495-591): 6 on SC (-x, +x,
+ * -y, +y, -z, +z), 8 on bcc. */
+static int ref_nn(const orc_lattice* L, int64_t site, int64_t nb[8])
+{
+    if (L->cell == ORC_CELL_BCC) {
+        int sg[8][3];
+        return orc_bcc_bonds(L, site, nb, sg);
+    }
+    int k = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int dir = -1; dir <= 1; dir += 2) nb[k++] = orc_neighbour(L, site, a, dir);
+    return k;
+}
+
+/* calculateHeisenbergEnergy (src/energyMin.f90:86-171) for a periodic SC or
+ * bcc lattice with J' = 0 and demag off: old/new local energy of site i for
+ * proposal sp. For each nearest neighbour j: r = -(x_j - x_i)_min/|.|,
+ * D = Dz (c x r), energy += J S.S_j + D.(S x S_j); then -B S_z. */
 void orc_ref_local_energy(const orc_lattice* L, double J_r, double Dz, double B, const double* s,
                           int64_t site, const double sp[3], double* old_e, double* new_e)
 {
     const int64_t N = nsites(L);
     const double chat[3] = { 0.0, 0.0, 1.0 };
     double S[3], p1[3];
-    int c[3];
     load(s, N, site, S);
-    decode(L, site, c);
-    p1[0] = c[0]; p1[1] = c[1]; p1[2] = c[2];
+    ref_position(L, site, p1);
     double tv[3] = { Dz * chat[0], Dz * chat[1], Dz * chat[2] };
     double oe = 0.0, ne = 0.0;
-    for (int a = 0; a < 3; ++a) {
-        for (int dir = -1; dir <= 1; dir += 2) {
-            int64_t j = orc_neighbour(L, site, a, dir);
-            double Sj[3], p2[3], r[3], D[3], x1[3], x2[3];
-            int cj[3];
-            load(s, N, j, Sj);
-            decode(L, j, cj);
-            p2[0] = cj[0]; p2[1] = cj[1]; p2[2] = cj[2];
-            ref_min_image(L, p1, p2, r);
-            double nr = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-            for (int k = 0; k < 3; ++k) r[k] = (-1.0) * r[k] / nr;
-            cross(tv, r, D);
-            cross(S, Sj, x1);
-            cross(sp, Sj, x2);
-            oe = oe + (J_r * dot(S, Sj) + dot(D, x1));
-            ne = ne + (J_r * dot(sp, Sj) + dot(D, x2));
-        }
+    int64_t nb[8];
+    const int nn = ref_nn(L, site, nb);
+    for (int k = 0; k < nn; ++k) {
+        const int64_t j = nb[k];
+        double Sj[3], p2[3], r[3], D[3], x1[3], x2[3];
+        load(s, N, j, Sj);
+        ref_position(L, j, p2);
+        ref_min_image(L, p1, p2, r);
+        double nr = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        for (int m = 0; m < 3; ++m) r[m] = (-1.0) * r[m] / nr;
+        cross(tv, r, D);
+        cross(S, Sj, x1);
+        cross(sp, Sj, x2);
+        oe = oe + (J_r * dot(S, Sj) + dot(D, x1));
+        ne = ne + (J_r * dot(sp, Sj) + dot(D, x2));
     }
     *old_e = oe - B * S[2];
     *new_e = ne - B * sp[2];
@@ -284,26 +450,22 @@ double orc_ref_total_energy(const orc_lattice* L, double J_r, double Dz, double 
     double total = 0.0;
     for (int64_t i = 0; i < N; ++i) {
         double S[3], p1[3], e = 0.0;
-        int c[3];
         const double tv[3] = { 0.0, 0.0, Dz };
         load(s, N, i, S);
-        decode(L, i, c);
-        p1[0] = c[0]; p1[1] = c[1]; p1[2] = c[2];
-        for (int a = 0; a < 3; ++a) {
-            for (int dir = -1; dir <= 1; dir += 2) {
-                int64_t j = orc_neighbour(L, i, a, dir);
-                double Sj[3], p2[3], r[3], D[3], x1[3];
-                int cj[3];
-                load(s, N, j, Sj);
-                decode(L, j, cj);
-                p2[0] = cj[0]; p2[1] = cj[1]; p2[2] = cj[2];
-                ref_min_image(L, p1, p2, r);
-                double nr = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-                for (int k = 0; k < 3; ++k) r[k] = (-1.0) * r[k] / nr;
-                cross(tv, r, D);
-                cross(S, Sj, x1);
-                e = e + 0.5 * (J_r * dot(S, Sj) + dot(D, x1));
-            }
+        ref_position(L, i, p1);
+        int64_t nb[8];
+        const int nn = ref_nn(L, i, nb);
+        for (int k = 0; k < nn; ++k) {
+            const int64_t j = nb[k];
+            double Sj[3], p2[3], r[3], D[3], x1[3];
+            load(s, N, j, Sj);
+            ref_position(L, j, p2);
+            ref_min_image(L, p1, p2, r);
+            double nr = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            for (int m = 0; m < 3; ++m) r[m] = (-1.0) * r[m] / nr;
+            cross(tv, r, D);
+            cross(S, Sj, x1);
+            e = e + 0.5 * (J_r * dot(S, Sj) + dot(D, x1));
         }
         e = e - (2.0 * 1.0 * B * S[2]);
         total = total + e;
@@ -348,12 +510,9 @@ double orc_ref_winding_number(const orc_lattice* L, const double* s, int zi)
     double wn = 0.0;
     for (int i = 0; i < L->lx; ++i) {
         for (int j = 0; j < L->ly; ++j) {
-            int64_t a1 = orc_site(L, i, j, zi - 1);
-            int64_t a2 = orc_neighbour(L, a1, 0, +1);
-            int64_t a4 = orc_neighbour(L, a2, 1, +1);
-            int64_t a3 = orc_neighbour(L, a1, 1, +1);
             double v[4][3];
-            int64_t ids[4] = { a1, a2, a3, a4 };
+            int64_t ids[4];
+            if (!plaquette(L, i, j, zi - 1, ids)) continue;
             for (int k = 0; k < 4; ++k) {
                 load(s, N, ids[k], v[k]);
                 double n = 0.0;
@@ -382,12 +541,9 @@ void orc_ref_winding_density(const orc_lattice* L, const double* s, int zi, doub
     const double pi_ref = (double)3.14159265358979323846f;
     for (int i = 0; i < L->lx; ++i) {
         for (int j = 0; j < L->ly; ++j) {
-            int64_t a1 = orc_site(L, i, j, zi - 1);
-            int64_t a2 = orc_neighbour(L, a1, 0, +1);
-            int64_t a4 = orc_neighbour(L, a2, 1, +1);
-            int64_t a3 = orc_neighbour(L, a1, 1, +1);
             double v[4][3];
-            int64_t ids[4] = { a1, a2, a3, a4 };
+            int64_t ids[4];
+            if (!plaquette(L, i, j, zi - 1, ids)) continue;
             for (int k = 0; k < 4; ++k) {
                 load(s, N, ids[k], v[k]);
                 double n = 0.0;
@@ -440,13 +596,9 @@ double orc_winding_layer(const orc_lattice* L, const double* s, int z, int64_t* 
     double acc = 0.0;
     for (int i = 0; i < L->lx; ++i) {
         for (int j = 0; j < L->ly; ++j) {
-            int64_t a1 = orc_site(L, i, j, z);
-            int64_t a2 = orc_neighbour(L, a1, 0, +1);
-            int64_t a3 = orc_neighbour(L, a1, 1, +1);
-            if (a2 < 0 || a3 < 0) continue;
-            int64_t a4 = orc_neighbour(L, a2, 1, +1);
             float v[4][3];
-            int64_t ids[4] = { a1, a2, a3, a4 };
+            int64_t ids[4];
+            if (!plaquette(L, i, j, z, ids)) continue;
             for (int k = 0; k < 4; ++k)
                 for (int m = 0; m < 3; ++m) v[k][m] = (float)s[m * N + ids[k]];
             acc += orc_triangle_sigma(v[0], v[1], v[3]) + orc_triangle_sigma(v[0], v[3], v[2]);
@@ -466,13 +618,9 @@ void orc_winding_density(const orc_lattice* L, const double* s, int z, double* o
     for (int i = 0; i < L->lx; ++i) {
         for (int j = 0; j < L->ly; ++j) {
             double d = 0.0;
-            int64_t a1 = orc_site(L, i, j, z);
-            int64_t a2 = orc_neighbour(L, a1, 0, +1);
-            int64_t a3 = orc_neighbour(L, a1, 1, +1);
-            if (a2 >= 0 && a3 >= 0) {
-                int64_t a4 = orc_neighbour(L, a2, 1, +1);
+            int64_t ids[4];
+            if (plaquette(L, i, j, z, ids)) {
                 float v[4][3];
-                int64_t ids[4] = { a1, a2, a3, a4 };
                 for (int k = 0; k < 4; ++k)
                     for (int m = 0; m < 3; ++m) v[k][m] = (float)s[m * N + ids[k]];
                 d = (orc_triangle_sigma(v[0], v[1], v[3]) + orc_triangle_sigma(v[0], v[3], v[2])) / (4.0 * M_PI);
@@ -515,9 +663,7 @@ int64_t orc_half_sweep(const orc_lattice* L, const orc_ham* h, double* s, double
     int64_t acc = 0;
     #pragma omp parallel for reduction(+:acc) schedule(static)
     for (int64_t i = 0; i < N; ++i) {
-        int c[3];
-        decode(L, i, c);
-        if (((c[0] + c[1] + c[2]) & 1) != colour) continue;
+        if (site_colour(L, i) != colour) continue;
         uint32_t rnd[4];
         orc_metropolis_random(seed, slot, (uint32_t)i, sweep, rnd);
         double so[3], sn[3], u;

@@ -22,7 +22,13 @@
 #define ISING_ORACLE_H
 #include <stdint.h>
 
-typedef struct { int32_t lx, ly, lz; int32_t periodic[3]; } orc_lattice;
+/* cell: ORC_CELL_SC, one atom per cell; ORC_CELL_BCC, atoms at (0,0,0) and
+ * (1/2,1/2,1/2) of each cell (src/unit_cells.f90:7-30), site index
+ * ((ix*Ly + iy)*Lz + iz)*2 + atom, the reference order for two-atom cells
+ * (src/chainMesh.f90:684,878). */
+typedef struct { int32_t lx, ly, lz; int32_t periodic[3]; int32_t cell; } orc_lattice;
+#define ORC_CELL_SC 0
+#define ORC_CELL_BCC 1
 typedef struct {
     double J, D; int32_t dmi_type; int32_t reserved;
     double Bz, Kz, zeeman_total_scale;
@@ -47,6 +53,9 @@ double  orc_ref_uniform(orc_ref_rng* r);
 /* ---- geometry ---- */
 int64_t orc_site(const orc_lattice* L, int ix, int iy, int iz);
 int64_t orc_neighbour(const orc_lattice* L, int64_t site, int axis, int dir);
+int64_t orc_bcc_site(const orc_lattice* L, int ix, int iy, int iz, int atom);
+int     orc_bcc_bonds(const orc_lattice* L, int64_t site, int64_t nb[8], int sgn[8][3]);
+int64_t orc_nsites(const orc_lattice* L);
 void    orc_dmi_vectors(const orc_ham* h, double d[3][3]);
 
 /* ---- energies (build form) ---- */

@@ -18,7 +18,11 @@ DMI = {"none": 0, "bulk": 1, "interfacial_ref": 2}
 
 
 class orc_lattice(C.Structure):
-    _fields_ = [("lx", C.c_int32), ("ly", C.c_int32), ("lz", C.c_int32), ("periodic", C.c_int32 * 3)]
+    _fields_ = [("lx", C.c_int32), ("ly", C.c_int32), ("lz", C.c_int32), ("periodic", C.c_int32 * 3),
+                ("cell", C.c_int32)]
+
+
+CELL_SC, CELL_BCC = 0, 1
 
 
 class orc_ham(C.Structure):
@@ -58,6 +62,7 @@ def lib() -> C.CDLL:
             "orc_ref_winding_number": (C.c_double, [lp, dp, C.c_int]),
             "orc_winding_layer": (C.c_double, [lp, dp, C.c_int, C.POINTER(C.c_int64)]),
             "orc_winding_density": (None, [lp, dp, C.c_int, dp]),
+            "orc_bcc_bonds": (C.c_int, [lp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
             "orc_ref_winding_density": (None, [lp, dp, C.c_int, dp]),
             "orc_propose": (C.c_int, [dp, C.c_double, u32p, dp, dp]),
             "orc_half_sweep": (C.c_int64, [lp, hp, dp, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
@@ -96,11 +101,28 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def lattice(lx, ly, lz, periodic=(True, True, True)) -> orc_lattice:
+def lattice(lx, ly, lz, periodic=(True, True, True), cell: str = "sc") -> orc_lattice:
     L = orc_lattice(lx, ly, lz)
     for a in range(3):
         L.periodic[a] = 1 if periodic[a] else 0
+    L.cell = {"sc": CELL_SC, "bcc": CELL_BCC}[cell]
     return L
+
+
+def nsites(L) -> int:
+    return L.lx * L.ly * L.lz * (2 if L.cell == CELL_BCC else 1)
+
+
+def bcc_site(L, ix, iy, iz, atom) -> int:
+    return ((ix * L.ly + iy) * L.lz + iz) * 2 + atom
+
+
+def bcc_bonds(L, site) -> tuple[np.ndarray, np.ndarray]:
+    """Nearest neighbours of a bcc site and the signs of the bond vectors."""
+    nb = (C.c_int64 * 8)()
+    sg = (C.c_int * 24)()
+    n = lib().orc_bcc_bonds(C.byref(L), site, nb, sg)
+    return np.array(nb[:n]), np.array(sg[:3 * n]).reshape(n, 3)
 
 
 def ham(J=1.0, D=0.0, dmi="bulk", Bz=0.0, Kz=0.0, zeeman_total_scale=1.0) -> orc_ham:
@@ -232,13 +254,13 @@ def sweeps(L, h, s, beta, radius, seed, slot, sweep0, n) -> int:
 
 
 def init_random(L, seed, slot) -> np.ndarray:
-    s = np.zeros((3, L.lx * L.ly * L.lz))
+    s = np.zeros((3, nsites(L)))
     lib().orc_init_random(C.byref(L), _dp(s), seed, slot)
     return s
 
 
 def init_ferro(L, seed, slot, frac) -> np.ndarray:
-    s = np.zeros((3, L.lx * L.ly * L.lz))
+    s = np.zeros((3, nsites(L)))
     lib().orc_init_ferro(C.byref(L), _dp(s), seed, slot, frac)
     return s
 

@@ -12,7 +12,7 @@ def engine(lat: Lattice, ham: Hamiltonian, betas, cone=0.5, seed=20261203) -> En
 
 
 def olat(lat: Lattice):
-    return orc.lattice(lat.lx, lat.ly, lat.lz, lat.periodic)
+    return orc.lattice(lat.lx, lat.ly, lat.lz, lat.periodic, cell=lat.cell)
 
 
 def random_unit(rng: np.random.Generator, n: int) -> np.ndarray:
@@ -27,11 +27,13 @@ def de_tolerance(lat_o, ham, s, sites, props) -> np.ndarray:
     tol = np.empty(len(sites))
     for k, i in enumerate(sites):
         acc = abs(ham.Bz)
-        for a in range(3):
-            for d in (-1, 1):
-                j = orc.neighbour(lat_o, int(i), a, d)
-                if j >= 0:
-                    acc += (abs(ham.J) + 2 * abs(ham.D)) * np.linalg.norm(s[:, j])
+        if lat_o.cell == orc.CELL_BCC:
+            nbrs = orc.bcc_bonds(lat_o, int(i))[0].tolist()
+        else:
+            nbrs = [orc.neighbour(lat_o, int(i), a, d) for a in range(3) for d in (-1, 1)]
+        for j in nbrs:
+            if j >= 0:
+                acc += (abs(ham.J) + 2 * abs(ham.D)) * np.linalg.norm(s[:, j])
         dl = props[k] - s[:, i]
         tol[k] = 1e-5 * (np.linalg.norm(dl) * acc + abs(ham.Kz) * abs(dl[2]) * abs(props[k][2] + s[2, i]))
     return tol

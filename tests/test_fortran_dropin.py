@@ -108,8 +108,11 @@ def test_pt_driver_reference_cli_and_csv(tmp_path):
     spins = sorted((tmp_path / "run_spins").glob("spins_*.csv"))
     assert len(spins) == 20
     data = np.loadtxt(spins[0], delimiter=",", skiprows=1)
-    assert data.shape == (30 * 30 * 6, 6)
+    # bcc cells of edge 2 (src/unit_cells.f90:21): body atoms at +1 on every axis
+    assert data.shape == (30 * 30 * 6 * 2, 6)
     assert np.allclose(np.linalg.norm(data[:, 3:], axis=1), 1.0, atol=1e-6)
+    assert np.allclose(data[0::2, :3] % 2.0, 0.0) and np.allclose(data[1::2, :3] % 2.0, 1.0)
+    assert np.allclose(data[1::2, :3] - data[0::2, :3], 1.0)
     # write_winding_number_density: x,y,z,Winding_Density per plaquette, layer
     # sums equal to the layer charge (middle layer in the data file, F0.4)
     dens = sorted((tmp_path / "run_density").glob("density_*.csv"))
@@ -118,8 +121,8 @@ def test_pt_driver_reference_cli_and_csv(tmp_path):
     for df, f in zip(dens, files):
         d = np.loadtxt(df, delimiter=",", skiprows=1)
         assert d.shape == (30 * 30 * 6, 4)
-        assert np.allclose(d[:, :3] / 2.8, np.rint(d[:, :3] / 2.8), atol=1e-6)
-        assert d[:, 2].min() == pytest.approx(2.8) and d[:, 2].max() == pytest.approx(6 * 2.8)
-        mid = d[np.isclose(d[:, 2], 3 * 2.8), 3].sum()
+        assert np.allclose(d[:, :3] / 2.0, np.rint(d[:, :3] / 2.0), atol=1e-6)
+        assert d[:, 2].min() == pytest.approx(2.0) and d[:, 2].max() == pytest.approx(6 * 2.0)
+        mid = d[np.isclose(d[:, 2], 3 * 2.0), 3].sum()
         row = [float(x) for x in list(csv.reader(f.open()))[1]]
         assert abs(mid - row[4]) < 1e-4 + 1e-6 * 900

@@ -1,0 +1,156 @@
+"""The oracle on body-centred cubic cells (src/unit_cells.f90:7-30, the cell
+the reference's PT driver uses, src/PT.f90:140): neighbour lists by brute
+force over positions, energies by closed forms, dE against energy
+differences, the reference restatement in reference mode, and the exact
+statistics of a one-bond dimer."""
+import itertools
+
+import numpy as np
+import pytest
+
+
+def positions(orc, L):
+    """Cell-unit positions of every site in oracle order."""
+    n = orc.nsites(L)
+    p = np.zeros((n, 3))
+    for ix, iy, iz, a in itertools.product(range(L.lx), range(L.ly), range(L.lz), range(2)):
+        p[orc.bcc_site(L, ix, iy, iz, a)] = [ix + 0.5 * a, iy + 0.5 * a, iz + 0.5 * a]
+    return p
+
+
+@pytest.mark.parametrize("dims,per", [((3, 4, 5), (True, True, True)), ((2, 2, 2), (True, True, True)),
+                                      ((3, 4, 5), (True, False, True)), ((2, 3, 2), (False, False, False))])
+def test_bcc_bonds_brute_force(orc, dims, per):
+    L = orc.lattice(*dims, per, cell="bcc")
+    p = positions(orc, L)
+    w = np.array(dims, dtype=float)
+    for i in range(len(p)):
+        d = p - p[i]
+        for ax in range(3):
+            if per[ax]:
+                d[:, ax] -= w[ax] * np.round(d[:, ax] / w[ax])
+        dist = np.linalg.norm(d, axis=1)
+        expect = set(np.flatnonzero(np.isclose(dist, np.sqrt(3) / 2)))
+        nb, sg = orc.bcc_bonds(L, i)
+        assert set(nb.tolist()) == expect and len(nb) == len(expect)
+        for j, s in zip(nb, sg):
+            assert np.array_equal(np.sign(d[j]).astype(int), s)     # bond vector site -> neighbour
+
+
+def test_bcc_closed_forms(orc):
+    lx, ly, lz = 4, 6, 8
+    L = orc.lattice(lx, ly, lz, cell="bcc")
+    nc = lx * ly * lz
+    p = positions(orc, L)
+    J, D, B, K = 1.3, 0.7, 0.4, 0.25
+    fm = np.zeros((3, 2 * nc)); fm[2] = 1.0
+    h = orc.ham(J, D, "bulk", B, K, zeeman_total_scale=2.0)
+    assert orc.total_energy(L, h, fm) == pytest.approx(-8 * J * nc - 2 * B * 2 * nc - K * 2 * nc, rel=1e-13)
+    neel = fm.copy(); neel[2, 1::2] = -1.0
+    assert orc.total_energy(L, h, neel) == pytest.approx(8 * J * nc - K * 2 * nc, rel=1e-13)
+    # bulk helix along z: S = (cos kz, sin kz, 0); each bond spans dz = +-1/2
+    h0 = orc.ham(J, D, "bulk", 0.0, 0.0)
+    for m in (1, 2, 3):
+        k = 2 * np.pi * m / lz
+        s = np.stack([np.cos(k * p[:, 2]), np.sin(k * p[:, 2]), 0 * p[:, 2]])
+        expect = 8 * nc * (-J * np.cos(k / 2) + D / np.sqrt(3) * np.sin(k / 2))
+        assert orc.total_energy(L, h0, s) == pytest.approx(expect, rel=1e-12)
+    # interfacial cycloid along x: S = (cos kx, 0, sin kx)
+    hi = orc.ham(J, D, "interfacial_ref", 0.0, 0.0)
+    for m in (1, 2):
+        k = 2 * np.pi * m / lx
+        s = np.stack([np.cos(k * p[:, 0]), 0 * p[:, 0], np.sin(k * p[:, 0])])
+        expect = 8 * nc * (-J * np.cos(k / 2) + D / np.sqrt(3) * np.sin(k / 2))
+        assert orc.total_energy(L, hi, s) == pytest.approx(expect, rel=1e-12)
+
+
+@pytest.mark.parametrize("dmi", ["bulk", "interfacial_ref"])
+@pytest.mark.parametrize("per", [(True, True, True), (True, True, False)])
+def test_bcc_delta_energy_is_energy_difference(orc, dmi, per):
+    L = orc.lattice(4, 3, 5, per, cell="bcc")
+    h = orc.ham(0.9, 0.6, dmi, 0.3, 0.2)
+    rng = np.random.default_rng(4)
+    n = orc.nsites(L)
+    s = rng.normal(size=(3, n)); s /= np.linalg.norm(s, axis=0)
+    e0 = orc.total_energy(L, h, s)
+    for i in rng.choice(n, 12, replace=False):
+        sn = rng.normal(size=3); sn /= np.linalg.norm(sn)
+        t = s.copy(); t[:, i] = sn
+        assert orc.delta_energy(L, h, s, int(i), sn) == pytest.approx(orc.total_energy(L, h, t) - e0, abs=1e-11)
+
+
+def test_bcc_reference_restatement(orc):
+    """calculateHeisenbergEnergy / TotalHeisenbergEnergy restated on bcc
+    positions against the build form in reference mode (J = -J_ref,
+    interfacial DMI, Zeeman coefficient 1 per site change, 2 in totals)."""
+    L = orc.lattice(4, 4, 3, cell="bcc")
+    Jr, Dz, B = 0.8, 0.45, 0.3
+    rng = np.random.default_rng(9)
+    n = orc.nsites(L)
+    s = rng.normal(size=(3, n)); s /= np.linalg.norm(s, axis=0)
+    h1 = orc.ham(-Jr, Dz, "interfacial_ref", B, 0.0, zeeman_total_scale=1.0)
+    for i in rng.choice(n, 10, replace=False):
+        sp = rng.normal(size=3); sp /= np.linalg.norm(sp)
+        oe, ne = orc.ref_local_energy(L, Jr, Dz, B, s, int(i), sp)
+        assert ne - oe == pytest.approx(orc.delta_energy(L, h1, s, int(i), sp), abs=1e-12)
+    h2 = orc.ham(-Jr, Dz, "interfacial_ref", B, 0.0, zeeman_total_scale=2.0)
+    assert orc.ref_total_energy(L, Jr, Dz, B, s) == pytest.approx(orc.total_energy(L, h2, s), abs=1e-10)
+
+
+def dimer_mean_energy(J, D, beta):
+    """Exact <E> of two free unit spins with E = -S1.M S2, M = J I + [d]x, |d| = D.
+    Integrating S2 out leaves 4 pi sinh(beta g)/(beta g) with
+    g^2 = |M^T S1|^2 = J^2 + D^2 (1 - c^2), c = cos(S1, d); then
+    <E> = -(1/beta) int [cosh(beta g) - sinh(beta g)/(beta g)] dc / int sinh(beta g)/(beta g) dc."""
+    from scipy.integrate import quad
+    g = lambda c: np.sqrt(J * J + D * D * (1 - c * c))
+    z = quad(lambda c: np.sinh(beta * g(c)) / (beta * g(c)), -1, 1)[0]
+    dz = quad(lambda c: np.cosh(beta * g(c)) - np.sinh(beta * g(c)) / (beta * g(c)), -1, 1)[0]
+    return -dz / (beta * z)
+
+
+def test_dimer_law_reduces_to_langevin():
+    J, beta = 1.3, 0.8
+    assert dimer_mean_energy(J, 0.0, beta) == pytest.approx(-J * (1 / np.tanh(beta * J) - 1 / (beta * J)), rel=1e-10)
+
+
+def test_bcc_one_bond_dimer(orc):
+    """An open 1x1x1 bcc cell has exactly one bond, along (1,1,1)/sqrt 3, so
+    with bulk DMI d = D (1,1,1)/sqrt 3 the chain samples the two-spin law of
+    dimer_mean_energy."""
+    orc.set_threads(1)
+    L = orc.lattice(1, 1, 1, (False, False, False), cell="bcc")
+    J, D, beta = 1.0, 0.9, 1.4
+    h = orc.ham(J, D, "bulk", 0.0, 0.0)
+    nb, sg = orc.bcc_bonds(L, 0)
+    assert nb.tolist() == [1] and sg.tolist() == [[1, 1, 1]]
+    s = orc.init_random(L, 6, 0)
+    vals = []
+    for k in range(60000):
+        orc.sweeps(L, h, s, beta, 1.2, 6, 0, k, 1)
+        vals.append(orc.total_energy(L, h, s))
+    vals = np.array(vals[1000:])
+    exact = dimer_mean_energy(J, D, beta)
+    from ising_amd.analysis import mean_error
+    m, err, _ = mean_error(vals)
+    assert abs(m - exact) < 4 * err, (m, exact, err)
+
+
+def test_bcc_winding_uses_atom0_planes(orc):
+    """Q of a bcc layer comes from the square lattice of atom-0 sites: a
+    skyrmion texture on atom 0 with the body atoms anywhere has |Q| = 1, and
+    the density sums to it."""
+    lx = ly = 20
+    L = orc.lattice(lx, ly, 2, cell="bcc")
+    rng = np.random.default_rng(3)
+    s = rng.normal(size=(3, orc.nsites(L))); s /= np.linalg.norm(s, axis=0)
+    for ix, iy, iz in itertools.product(range(lx), range(ly), range(2)):
+        dx, dy = ix - 9.7, iy - 10.2
+        th = np.pi * np.exp(-(np.hypot(dx, dy) / 3.2) ** 2)
+        ph = np.arctan2(dy, dx)
+        s[:, orc.bcc_site(L, ix, iy, iz, 0)] = [np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]
+    for z in range(2):
+        q, raw = orc.winding_layer(L, s, z)
+        assert abs(q) == 1 and abs(raw - q) < 1e-9
+        assert abs(orc.winding_density(L, s, z).sum() - raw) < 1e-12
+        assert abs(orc.ref_winding_number(L, s, z + 1) - raw) < 1e-5
