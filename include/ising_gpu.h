@@ -1,7 +1,8 @@
 /*
  * ising_gpu.h -- C ABI of libising_hip.so, the MI355X (gfx950) Metropolis +
  * parallel-tempering engine for classical Heisenberg spins on simple-cubic
- * lattices (exchange + DMI + Zeeman + uniaxial anisotropy).
+ * and body-centred-cubic lattices (exchange + DMI + Zeeman + uniaxial
+ * anisotropy + the reference's demagnetisation term).
  *
  * Plain C types only; no exceptions cross this boundary. Every function
  * returns ISING_OK (0) on success or a negative ISING_ERR_* code; the text of
@@ -23,6 +24,14 @@
  * with d_a = D r_a for bulk DMI and d_a = D (z x (-r_a)) for the reference's
  * interfacial form (src/energyMin.f90:114,135-137). Single-site energy
  * changes use Bz with coefficient 1 (src/energyMin.f90:161-162).
+ *
+ * With ising_hamiltonian.demag = 1 the reference's demagnetisation term is
+ * added: single-site changes -mu0 (S'-S).H_i and totals -mu0/2 sum S_i.H_i
+ * (src/energyMin.f90:113-116,163-166,223-226), mu0 = ISING_MU0_REF, H the
+ * field of calculate_demagnetisation_field (src/reciprocalSpaceProcesses.f90:
+ * 202-281) in lattice units: cloud-in-cell deposit onto one mesh point per
+ * cell centre, H(k) = -(k.M(k)) k / k^2 with the k = 0 mode removed, the
+ * same weights back to the atoms. It needs a fully periodic lattice.
  */
 #ifndef ISING_GPU_H
 #define ISING_GPU_H
@@ -40,7 +49,7 @@ extern "C" {
 #define ISING_ERR_BAD_SPIN      -3  /* NaN / inf / non-unit spin uploaded     */
 #define ISING_ERR_NO_DEVICE     -4  /* no HIP device or device index invalid  */
 #define ISING_ERR_HIP           -5  /* a HIP runtime call failed              */
-#define ISING_ERR_UNSUPPORTED   -6  /* e.g. demagnetisation requested         */
+#define ISING_ERR_UNSUPPORTED   -6  /* e.g. demagnetisation on an open lattice */
 #define ISING_ERR_OUT_OF_MEMORY -7
 #define ISING_ERR_STATE         -8  /* call not valid in the current state    */
 
@@ -50,6 +59,11 @@ extern "C" {
 #define ISING_DMI_INTERFACIAL_REF 2 /* d_a = D z x (-r_a), reference form    */
 
 typedef struct ising_ctx ising_ctx;
+
+/* The reference's constants::mu_0 (src/constants.f90:15): the literal
+ * 1.25663706127e-6 is default (single-precision) real, so its value is the
+ * float nearest to it. */
+#define ISING_MU0_REF 1.256637006008532e-06
 
 /* ---- unit cells ------------------------------------------------------- */
 #define ISING_CELL_SC  0  /* one atom per cell, 6 nearest neighbours          */
@@ -69,7 +83,7 @@ typedef struct {
     double  J;                   /* E = -J S_i.S_j per bond                  */
     double  D;                   /* DMI magnitude                            */
     int32_t dmi_type;            /* ISING_DMI_*                              */
-    int32_t reserved;
+    int32_t demag;               /* 1: demagnetisation term, see below; 0: off */
     double  Bz;                  /* Zeeman field along z                     */
     double  Kz;                  /* uniaxial anisotropy, E -= Kz Sz^2        */
     double  zeeman_total_scale;  /* Zeeman coefficient in total energies:
@@ -143,7 +157,8 @@ int  ising_init_ferro(ising_ctx* ctx, int replica, double flip_fraction);
  * n_sweeps exact checkerboard Metropolis sweeps at each replica's beta. */
 int  ising_sweep(ising_ctx* ctx, int n_sweeps);
 /* Replaces TotalHeisenbergEnergy (src/energyMin.f90:499-528): FP64 total
- * energy of every replica slot, e[R]. */
+ * energy of every replica slot, e[R]; ladders with demag = 1 first
+ * recompute the field from the current spins, as the reference does. */
 int  ising_energy(ising_ctx* ctx, double* e);
 /* Total energy of one replica under an arbitrary Hamiltonian (drop-in for
  * TotalHeisenbergEnergy's per-call J, Dz, B arguments). */
@@ -174,9 +189,31 @@ int  ising_delta_energy(ising_ctx* ctx, int replica, const int32_t* sites,
 int  ising_save_state(ising_ctx* ctx);
 int  ising_restore_state(ising_ctx* ctx);
 
+/* ---- demagnetisation -------------------------------------------------- */
+/* Replaces calculate_demagnetisation_field(chainMesh, outputArray)
+ * (src/reciprocalSpaceProcesses.f90:202-281): the field H of the current
+ * spins of `replica`, h[3*n] in the atomSpins layout (n = sites), FP64. */
+int  ising_demag_field(ising_ctx* ctx, int replica, double* h, int64_t n);
+/* The field a replica's sweeps and ising_delta_energy use now (the last one
+ * computed or set; zero before any), without recomputing it: the state of
+ * chainMesh%demagnetisation_array after Metropolis_mcs. */
+int  ising_get_demag_field(ising_ctx* ctx, int replica, double* h, int64_t n);
+/* Sets the field a replica's sweeps and ising_delta_energy use until the
+ * next recomputation (calculateHeisenbergEnergy takes the field as its
+ * demagnetisation_array argument, src/energyMin.f90:87,113-115). */
+int  ising_set_demag_field(ising_ctx* ctx, int replica, const double* h, int64_t n);
+/* ising_sweep on ladders with demag = 1 recomputes the field from the spins
+ * before its first sweep and after every sweep whose 1-based number in the
+ * call is a multiple of `sweeps` (0: only before the first), as
+ * Metropolis_mcs does with int(pi sqrt 5 / (2 r)) (src/energyMin.f90:258-259,
+ * 362-369). Default 1. */
+int  ising_set_demag_interval(ising_ctx* ctx, int sweeps);
+
 /* ---- parallel tempering ---------------------------------------------- */
 /* Replaces the PT iteration loop src/PT.f90:219-273 (sweeps + swaps) plus
- * the per-temperature statistics the reference takes at the end. */
+ * the per-temperature statistics the reference takes at the end. Ladders
+ * with demag = 1 are refused (the reference's PT runs without the term,
+ * src/PT.f90:229-230,251-254). */
 int  ising_pt_run(ising_ctx* ctx, const ising_schedule* sched, const ising_series* out);
 /* slot_of_temp[R]: storage slot currently at temperature position t
  * (the reference's TemperatureMeshArray, src/PT.f90:58,201,264-268). */

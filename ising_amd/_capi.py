@@ -31,6 +31,7 @@ TUNE_XCD_REMAP = 1
 TUNE_NT_OWN = 2
 TUNE_FUSED_SHIFT = 8
 TUNE_MAX_CANDIDATES = 16
+MU0_REF = 1.256637006008532e-06
 
 
 def tune_fused(planes: int) -> int:
@@ -45,7 +46,7 @@ class ising_lattice(C.Structure):
 
 
 class ising_hamiltonian(C.Structure):
-    _fields_ = [("J", C.c_double), ("D", C.c_double), ("dmi_type", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("J", C.c_double), ("D", C.c_double), ("dmi_type", C.c_int32), ("demag", C.c_int32),
                 ("Bz", C.c_double), ("Kz", C.c_double), ("zeeman_total_scale", C.c_double)]
 
 
@@ -95,6 +96,10 @@ SIGNATURES = {
     "ising_winding_all": (C.c_int, [_P, _i64p, _dp]),
     "ising_winding_density": (C.c_int, [_P, C.c_int, C.c_int, _dp]),
     "ising_delta_energy": (C.c_int, [_P, C.c_int, _i32p, _dp, C.c_int, _dp]),
+    "ising_demag_field": (C.c_int, [_P, C.c_int, _dp, C.c_int64]),
+    "ising_get_demag_field": (C.c_int, [_P, C.c_int, _dp, C.c_int64]),
+    "ising_set_demag_field": (C.c_int, [_P, C.c_int, _dp, C.c_int64]),
+    "ising_set_demag_interval": (C.c_int, [_P, C.c_int]),
     "ising_pt_run": (C.c_int, [_P, C.POINTER(ising_schedule), C.POINTER(ising_series)]),
     "ising_temperature_map": (C.c_int, [_P, _i32p]),
     "ising_save_state": (C.c_int, [_P]),

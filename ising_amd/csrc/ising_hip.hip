@@ -113,6 +113,9 @@ struct Lad {
     int32_t learned;        // cone radius from the network (mlp.hpp)
     double Jd, Bzd, Kzd, zs;
     double dd[9];
+    int32_t demag;          // demagnetisation term on (ising_hamiltonian.demag)
+    int32_t pad_;
+    double mu0;             // its coefficient (0 when off)
 };
 
 struct f3 { float x, y, z; };
@@ -417,6 +420,18 @@ __device__ __forceinline__ f3 metropolis_site(const Lad& L, float beta, const f3
     return fs;
 }
 
+// Demagnetising field of the site at colour-major index t (colour*half + q),
+// mu0 H from the FP64 per-site field in the spin layout, added to the local
+// field h (src/energyMin.f90:113-116,163-166).
+__device__ __forceinline__ void add_demag(const Lad& L, const Rep& R, const double* __restrict__ demag,
+                                          uint32_t t, f3& hf)
+{
+    const double* d = demag + R.off + t;
+    hf.x += (float)(L.mu0 * d[0]);
+    hf.y += (float)(L.mu0 * d[R.n]);
+    hf.z += (float)(L.mu0 * d[2 * R.n]);
+}
+
 // Energy of a colour-1 site after the sweep and of its on-site partner: its
 // six bonds, -S.(h - Bz z) with h formed from the updated colour-0
 // neighbours (each bond has exactly one colour-1 end), plus the on-site
@@ -440,7 +455,8 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
     const float* __restrict__ beta_slot, const int32_t* __restrict__ block_rep,
     const uint32_t* __restrict__ sweep_ctr, uint32_t sweep_off, int colour,
     uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count,
-    double* __restrict__ wave_part, uint32_t tune, const MlpDev* __restrict__ mlp)
+    double* __restrict__ wave_part, uint32_t tune, const MlpDev* __restrict__ mlp,
+    const double* __restrict__ demag)
 {
     const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
     const int slot = block_rep[blk];
@@ -476,6 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
         hf = field_from(L, nbr);
         site = site_number(R, ix, iy, iz);
     }
+    if (L.demag) add_demag(L, R, demag, (uint32_t)colour * R.half + q, hf);   // uniform per workgroup
     bool accepted;
     const f3 fs = metropolis_site<kLearned>(L, beta, s, nbr, hf, site, sweep, seed_lo, seed_hi,
                                             (uint32_t)slot, mlp, active, accepted);
@@ -822,7 +839,7 @@ __device__ __forceinline__ void block_sum4(double v[4], double* lds /* [4][kBloc
 __global__ __launch_bounds__(kBlock) void k_measure(
     SpinBuf sb, uint32_t sweep_off, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const Lad* __restrict__ lad_override, const int32_t* __restrict__ block_rep, int block_base,
-    double* __restrict__ partial)
+    double* __restrict__ partial, const double* __restrict__ demag)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
     const float* spins = spin_buffer(sb, *sb.ctr + sweep_off);
@@ -860,6 +877,12 @@ __global__ __launch_bounds__(kBlock) void k_measure(
         const f3 bf = ld3(p1, q);
         const double ux = bf.x, uy = bf.y, uz = bf.z;
         e -= L.zs * L.Bzd * uz + L.Kzd * uz * uz;
+        if (L.demag) {      // -mu0/2 S.H of both atoms (src/energyMin.f90:223-226)
+            const double* d = demag + R.off + q;
+            e -= 0.5 * L.mu0 * (sx * d[0] + sy * d[R.n] + sz * d[2 * R.n]);
+            d += R.half;
+            e -= 0.5 * L.mu0 * (ux * d[0] + uy * d[R.n] + uz * d[2 * R.n]);
+        }
         v[0] = e; v[1] = sx + ux; v[2] = sy + uy; v[3] = sz + uz;
     } else if (q < R.half) {
         const uint32_t row = fdiv(q, R.div_hx);
@@ -886,6 +909,10 @@ __global__ __launch_bounds__(kBlock) void k_measure(
                      + (L.dd[3 * a] * cx + L.dd[3 * a + 1] * cy + L.dd[3 * a + 2] * cz);
             }
             e -= L.zs * L.Bzd * sz + L.Kzd * sz * sz;
+            if (L.demag) {
+                const double* d = demag + R.off + (uint32_t)colour * R.half + q;
+                e -= 0.5 * L.mu0 * (sx * d[0] + sy * d[R.n] + sz * d[2 * R.n]);
+            }
             v[0] += e; v[1] += sx; v[2] += sy; v[3] += sz;
         }
     }
@@ -1207,7 +1234,8 @@ __global__ __launch_bounds__(kBlock) void k_init(
 // Single-site dE test hook: same local field and dE code as the sweep.
 __global__ __launch_bounds__(kBlock) void k_delta_energy(
     const float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads, int slot,
-    const int32_t* __restrict__ sites, const float* __restrict__ props, int count, double* __restrict__ out)
+    const int32_t* __restrict__ sites, const float* __restrict__ props, int count, double* __restrict__ out,
+    const double* __restrict__ demag)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= count) return;
@@ -1226,10 +1254,285 @@ __global__ __launch_bounds__(kBlock) void k_delta_energy(
     const Planes own = planes(spins, R, (int)colour);
     const Planes oth = planes(spins, R, (int)(colour ^ 1u));
     const f3 s = ld3(own, q);
-    const f3 hf = bcc ? bcc_field(L, bcc_sums(R, oth, q, ix, iy, iz, (int)colour))
-                      : local_field(R, L, oth, neighbours<false>(R, q, h, iy, iz, ix & 1u));
+    f3 hf = bcc ? bcc_field(L, bcc_sums(R, oth, q, ix, iy, iz, (int)colour))
+                : local_field(R, L, oth, neighbours<false>(R, q, h, iy, iz, ix & 1u));
+    if (L.demag) add_demag(L, R, demag, colour * R.half + q, hf);
     const f3 sn{ props[3 * e], props[3 * e + 1], props[3 * e + 2] };
     out[e] = (double)delta_e(s, sn, hf, L.Kz);
+}
+
+// ---------------------------------------------------------------------------
+// Demagnetising field, calculate_demagnetisation_field
+// (src/reciprocalSpaceProcesses.f90:202-281), FP64 throughout:
+//   k_demag_deposit    cloud-in-cell deposit onto one mesh point per cell
+//                      centre (interpolate_to_fft_array, :76-129);
+//   k_dft_r2c_x        forward real-to-half-complex transform along x;
+//   k_dft_c2c<axis>    complex transforms along y and z (both directions);
+//   k_demag_project    H(k) = -(k.M) k / k^2, k = 0 removed (:237-270);
+//   k_dft_c2r_x        half-complex-to-real inverse along x, reading only
+//                      the real parts of the DC and Nyquist terms (FFTW's
+//                      c2r convention), scaled by 1/(lx ly lz) (:271-275);
+//   k_demag_gather     the same weights back to the atoms
+//                      (interpolate_fft_to_atoms, :131-178).
+// Mesh arrays per replica: real [3][lx][ly][lz] (cell (i*ly + j)*lz + k, the
+// reference's cell order), complex [3][lx/2+1][ly][lz]. Transforms are direct
+// sums over each axis with a twiddle table e^{-2 pi i m/n} per length, one
+// thread per output: any length, exact FP64 twiddles. Every kernel runs on
+// grid (x blocks, replica slot); a slot is skipped unless it is fully
+// periodic and selected (demag_slot).
+struct DemagRep {
+    int64_t moff;        // doubles: real mesh of this slot
+    int64_t koff;        // complex elements: half-complex mesh of this slot
+    int32_t tw[3];       // first entry of the twiddle table per axis length
+    int32_t hx;          // lx/2 + 1
+};
+
+// only >= 0: that slot; kDemagLadders: slots whose ladder has the term on;
+// kDemagAll: every fully periodic slot.
+constexpr int kDemagLadders = -1;
+constexpr int kDemagAll = -2;
+
+__device__ __forceinline__ bool demag_slot(const Rep& R, const Lad* __restrict__ lads, int only)
+{
+    if (R.per != 7) return false;
+    if (only >= 0) return (int)blockIdx.y == only;
+    return only == kDemagAll || lads[R.ladder].demag;
+}
+
+// One step of getNeighboringCells (src/chainMesh.f90:98-117): n -> 0, -1 -> n-1.
+__device__ __forceinline__ int wrap_cell(int i, int n) { return i == n ? 0 : (i == -1 ? n - 1 : i); }
+
+// Weight factor of one axis: d = atom - centre in lattice units, minimum
+// image d - n nint(d/n) (nint half away from zero; distance_points_vec_bravais,
+// src/chainMesh.f90:268-270), then max(1 - |d|, 0).
+__device__ __forceinline__ double cic_axis(int atom_cell, double frac, int centre_cell, int n)
+{
+    double d = ((double)atom_cell + frac) - ((double)centre_cell + 0.5);
+    d -= (double)n * round(d / (double)n);
+    return fmax(1.0 - fabs(d), 0.0);
+}
+
+// Colour-major index colour*half + q of atom a of cell (i, j, k).
+__device__ __forceinline__ uint32_t atom_index(const Rep& R, int i, int j, int k, int a)
+{
+    const uint32_t row = (uint32_t)k * (uint32_t)R.ly + (uint32_t)j;
+    if (R.cell == ISING_CELL_BCC) return (uint32_t)a * R.half + row * (uint32_t)R.hx + (uint32_t)i;
+    return (uint32_t)((i + j + k) & 1) * R.half + row * (uint32_t)R.hx + ((uint32_t)i >> 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_demag_deposit(
+    const float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only, double* __restrict__ mesh)
+{
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const int nx = R.lx, ny = R.ly, nz = R.lz;
+    const int64_t Nc = (int64_t)nx * ny * nz;
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= Nc) return;
+    const int k = (int)(cell % nz), j = (int)((cell / nz) % ny), i = (int)(cell / ((int64_t)nz * ny));
+    const int na = R.cell == ISING_CELL_BCC ? 2 : 1;
+    const float* sp = spins + R.off;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int di = -1; di <= 1; ++di) {
+        const int ni = wrap_cell(i + di, nx);
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int nj = wrap_cell(j + dj, ny);
+            for (int dk = -1; dk <= 1; ++dk) {
+                const int nk = wrap_cell(k + dk, nz);
+                for (int a = 0; a < na; ++a) {
+                    const double f = 0.5 * a;
+                    const double w = cic_axis(ni, f, i, nx) * cic_axis(nj, f, j, ny) * cic_axis(nk, f, k, nz);
+                    if (w == 0.0) continue;
+                    const uint32_t t = atom_index(R, ni, nj, nk, a);
+                    m0 += w * (double)sp[t];
+                    m1 += w * (double)sp[R.n + t];
+                    m2 += w * (double)sp[2 * R.n + t];
+                }
+            }
+        }
+    }
+    double* o = mesh + dreps[blockIdx.y].moff;
+    o[cell] = m0;
+    o[Nc + cell] = m1;
+    o[2 * Nc + cell] = m2;
+}
+
+// X[c][kx][j][k] = sum_n x[c][n][j][k] e^{-2 pi i kx n / lx}, kx <= lx/2.
+__global__ __launch_bounds__(kBlock) void k_dft_r2c_x(
+    const double* __restrict__ mesh, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw, double2* __restrict__ out)
+{
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const DemagRep D = dreps[blockIdx.y];
+    const int nx = R.lx;
+    const int64_t plane = (int64_t)R.ly * R.lz, Nc = plane * nx, Nk = plane * D.hx;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= 3 * Nk) return;
+    const int64_t c = e / Nk, r = e - c * Nk;
+    const int kx = (int)(r / plane);
+    const int64_t jk = r - (int64_t)kx * plane;
+    const double* x = mesh + D.moff + c * Nc + jk;
+    const double2* t = tw + D.tw[0];
+    double re = 0.0, im = 0.0;
+    int m = 0;
+    for (int n = 0; n < nx; ++n) {
+        const double v = x[(int64_t)n * plane];
+        re += v * t[m].x;
+        im += v * t[m].y;
+        m += kx;
+        if (m >= nx) m -= nx;
+    }
+    out[D.koff + e] = double2{ re, im };
+}
+
+// Complex transform along axis 1 (y) or 2 (z) of [3][hx][ly][lz]; sign -1
+// forward (e^{-2 pi i k n / len}), +1 inverse (unnormalised).
+template <int kAxis>
+__global__ __launch_bounds__(kBlock) void k_dft_c2c(
+    const double2* __restrict__ in, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw, int sign, double2* __restrict__ out)
+{
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const DemagRep D = dreps[blockIdx.y];
+    const int ny = R.ly, nz = R.lz;
+    const int64_t Nk = (int64_t)D.hx * ny * nz;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= 3 * Nk) return;
+    const int kz = (int)(e % nz);
+    const int ky = (int)((e / nz) % ny);
+    const int len = kAxis == 1 ? ny : nz;
+    const int kk = kAxis == 1 ? ky : kz;
+    const int64_t stride = kAxis == 1 ? nz : 1;
+    const double2* x = in + D.koff + e - (int64_t)kk * stride;
+    const double2* t = tw + D.tw[kAxis];
+    const double sg = sign < 0 ? 1.0 : -1.0;   // the table holds e^{-...}; conjugate for +1
+    double re = 0.0, im = 0.0;
+    int m = 0;
+    for (int n = 0; n < len; ++n) {
+        const double2 v = x[(int64_t)n * stride];
+        const double tr = t[m].x, ti = sg * t[m].y;
+        re += v.x * tr - v.y * ti;
+        im += v.x * ti + v.y * tr;
+        m += kk;
+        if (m >= len) m -= len;
+    }
+    out[D.koff + e] = double2{ re, im };
+}
+
+// H(k) = -(k.M(k)) k / k^2 on the half-complex mesh, in place; wave indices
+// wx = kx, wy and wz folded to (-n/2, n/2]; k = 2 pi w / n per axis (cubic
+// cells in lattice units, so the reference's cutoff |k_a| > pi/a never
+// removes a mode).
+__global__ __launch_bounds__(kBlock) void k_demag_project(
+    double2* __restrict__ F, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only)
+{
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const DemagRep D = dreps[blockIdx.y];
+    const int nx = R.lx, ny = R.ly, nz = R.lz;
+    const int64_t Nk = (int64_t)D.hx * ny * nz;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= Nk) return;
+    const int kz = (int)(e % nz), ky = (int)((e / nz) % ny), kx = (int)(e / ((int64_t)nz * ny));
+    double2* f = F + D.koff + e;
+    if (kx == 0 && ky == 0 && kz == 0) {
+        f[0] = double2{ 0.0, 0.0 }; f[Nk] = double2{ 0.0, 0.0 }; f[2 * Nk] = double2{ 0.0, 0.0 };
+        return;
+    }
+    const int wy = ky > ny / 2 ? ky - ny : ky, wz = kz > nz / 2 ? kz - nz : kz;
+    const double two_pi = 6.283185307179586476925;
+    const double k0 = two_pi * ((double)kx / (double)nx);
+    const double k1 = two_pi * ((double)wy / (double)ny);
+    const double k2 = two_pi * ((double)wz / (double)nz);
+    const double2 a = f[0], b = f[Nk], c = f[2 * Nk];
+    const double inv = 1.0 / (k0 * k0 + k1 * k1 + k2 * k2);
+    const double dr = (k0 * a.x + k1 * b.x + k2 * c.x) * inv;
+    const double di = (k0 * a.y + k1 * b.y + k2 * c.y) * inv;
+    f[0] = double2{ -dr * k0, -di * k0 };
+    f[Nk] = double2{ -dr * k1, -di * k1 };
+    f[2 * Nk] = double2{ -dr * k2, -di * k2 };
+}
+
+// x[c][i][j][k] = (Re X_0 + sum_{0<kx<lx/2} 2 Re(X_kx e^{2 pi i kx i/lx})
+//                 [+ (-1)^i Re X_{lx/2}]) / (lx ly lz).
+__global__ __launch_bounds__(kBlock) void k_dft_c2r_x(
+    const double2* __restrict__ in, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw, double* __restrict__ mesh)
+{
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const DemagRep D = dreps[blockIdx.y];
+    const int nx = R.lx;
+    const int64_t plane = (int64_t)R.ly * R.lz, Nc = plane * nx, Nk = plane * D.hx;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= 3 * Nc) return;
+    const int64_t c = e / Nc, cell = e - c * Nc;
+    const int i = (int)(cell / plane);
+    const int64_t jk = cell - (int64_t)i * plane;
+    const double2* X = in + D.koff + c * Nk + jk;
+    const double2* t = tw + D.tw[0];
+    const int nyq = (nx & 1) ? -1 : nx / 2;
+    double acc = X[0].x;
+    int m = 0;
+    for (int kx = 1; kx < D.hx; ++kx) {
+        m += i;
+        if (m >= nx) m -= nx;
+        const double2 v = X[(int64_t)kx * plane];
+        if (kx == nyq) acc += (i & 1) ? -v.x : v.x;
+        else acc += 2.0 * (v.x * t[m].x + v.y * t[m].y);   // Re(v conj(t)), t = e^{-...}
+    }
+    mesh[D.moff + e] = acc / (double)Nc;
+}
+
+// H of every atom: the mesh field of the 27 cells around the atom's cell with
+// the deposit's weights; written in the spin layout (colour-major t).
+__global__ __launch_bounds__(kBlock) void k_demag_gather(
+    const double* __restrict__ mesh, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only, double* __restrict__ H)
+{
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const int64_t tt = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (tt >= R.n) return;
+    const uint32_t t = (uint32_t)tt;
+    const uint32_t colour = t >= R.half ? 1u : 0u;
+    const uint32_t q = t - colour * R.half;
+    const uint32_t row = q / (uint32_t)R.hx, h = q - row * (uint32_t)R.hx;
+    const int iz = (int)(row / (uint32_t)R.ly), iy = (int)(row - (uint32_t)iz * (uint32_t)R.ly);
+    const bool bcc = R.cell == ISING_CELL_BCC;
+    const int ix = bcc ? (int)h : (int)(2u * h + (((uint32_t)(iy + iz) + colour) & 1u));
+    const double f = bcc && colour ? 0.5 : 0.0;
+    const int nx = R.lx, ny = R.ly, nz = R.lz;
+    const int64_t Nc = (int64_t)nx * ny * nz;
+    const double* m = mesh + dreps[blockIdx.y].moff;
+    double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+    for (int di = -1; di <= 1; ++di) {
+        const int ci = wrap_cell(ix + di, nx);
+        const double wx = cic_axis(ix, f, ci, nx);
+        if (wx == 0.0) continue;
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int cj = wrap_cell(iy + dj, ny);
+            const double wy = cic_axis(iy, f, cj, ny);
+            if (wy == 0.0) continue;
+            for (int dk = -1; dk <= 1; ++dk) {
+                const int ck = wrap_cell(iz + dk, nz);
+                const double w = wx * wy * cic_axis(iz, f, ck, nz);
+                if (w == 0.0) continue;
+                const int64_t cell = ((int64_t)ci * ny + cj) * nz + ck;
+                h0 += w * m[cell];
+                h1 += w * m[Nc + cell];
+                h2 += w * m[2 * Nc + cell];
+            }
+        }
+    }
+    double* o = H + R.off + t;
+    o[0] = h0;
+    o[R.n] = h1;
+    o[2 * R.n] = h2;
 }
 
 } // namespace ising
@@ -1268,6 +1571,18 @@ struct ising_ctx {
     std::vector<int32_t> fblock_rep;
     bool any_learned = false;    // some ladder takes its cone radius from the network
     std::vector<int> learned;    // per ladder
+    // demagnetising field (allocated on first use)
+    bool any_demag = false;      // some ladder has the demagnetisation term on
+    int demag_interval = 1;      // sweeps between field recomputations in ising_sweep
+    bool demag_ready = false;
+    std::vector<DemagRep> dreps;
+    int64_t dmax_cells = 0, dmax_modes = 0, dmax_sites = 0;
+    double* d_demag = nullptr;   // FP64 field per site, spin layout (offsets of d_spins)
+    double* d_dmesh = nullptr;   // real mesh [3][cells] per slot
+    double2* d_dk1 = nullptr;    // half-complex meshes [3][modes] per slot (ping-pong)
+    double2* d_dk2 = nullptr;
+    double2* d_tw = nullptr;     // twiddle tables e^{-2 pi i m/n}
+    DemagRep* d_dreps = nullptr;
 
     // learned proposal
     MlpDev* d_mlp = nullptr;
@@ -1381,6 +1696,8 @@ Lad make_lad(const ising_hamiltonian& h, double cone, int cell)
     for (int k = 0; k < 9; ++k) { L.d[k] = (float)d[k]; L.dd[k] = d[k]; }
     L.dmi = h.dmi_type;
     L.Jd = h.J; L.Bzd = h.Bz; L.Kzd = h.Kz; L.zs = h.zeeman_total_scale;
+    L.demag = h.demag ? 1 : 0;
+    L.mu0 = h.demag ? ISING_MU0_REF : 0.0;
     return L;
 }
 
@@ -1413,13 +1730,25 @@ int check_ham(ising_ctx* c, const ising_hamiltonian& h)
     const double v[5] = { h.J, h.D, h.Bz, h.Kz, h.zeeman_total_scale };
     for (double x : v)
         if (!std::isfinite(x)) return set_err(c, ISING_ERR_INVALID_ARG, "non-finite Hamiltonian parameter");
+    if (h.demag != 0 && h.demag != 1) return set_err(c, ISING_ERR_INVALID_ARG, "demag must be 0 or 1 (got %d)", h.demag);
+    return ISING_OK;
+}
+
+// The demagnetising field is periodic (a discrete Fourier transform of the
+// whole lattice, src/chainMesh.f90:717-728).
+int check_demag_lattice(ising_ctx* c, const ising_lattice& l)
+{
+    if (!l.periodic[0] || !l.periodic[1] || !l.periodic[2])
+        return set_err(c, ISING_ERR_UNSUPPORTED, "the demagnetising field needs a fully periodic lattice");
     return ISING_OK;
 }
 
 int upload_lads(ising_ctx* c)
 {
     std::vector<Lad> l;
+    c->any_demag = false;
     for (size_t k = 0; k < c->ladders.size(); ++k) {
+        c->any_demag = c->any_demag || c->ladders[k].ham.demag != 0;
         l.push_back(make_lad(c->ladders[k].ham, c->ladders[k].cone, c->ladders[k].lat.cell));
         l.back().learned = k < c->learned.size() ? c->learned[k] : 0;
     }
@@ -1461,13 +1790,19 @@ float* cur_spins(const ising_ctx* c) { return c->d_spins + (int64_t)(c->host_fus
 // z planes per fused-sweep workgroup (0: two half-sweep launches per sweep).
 int fused_lc(const ising_ctx* c) { return (int)((c->tune >> 8) & 0xFFu); }
 
+// Sweeps run as fused launches (which swap the spin buffers): the fused flag
+// is set and no ladder has the demagnetisation term, which only the
+// in-place half sweeps carry.
+bool fused_active(const ising_ctx* c) { return fused_lc(c) > 0 && !c->any_demag; }
+
 template <bool kMeasure, bool kLearned>
 void launch_half_kernel(ising_ctx* c, uint32_t off, int colour)
 {
     hipLaunchKernelGGL((k_half_sweep<kMeasure, kLearned>), dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
                        spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
                        (const uint32_t*)c->d_ctr, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
-                       kMeasure ? c->d_wave_part : (double*)nullptr, c->tune & 0xFFu, (const MlpDev*)c->d_mlp);
+                       kMeasure ? c->d_wave_part : (double*)nullptr, c->tune & 0xFFu, (const MlpDev*)c->d_mlp,
+                       (const double*)c->d_demag);
 }
 
 template <bool kMeasure, bool kLearned>
@@ -1487,7 +1822,7 @@ void launch_fused_kernel(ising_ctx* c, uint32_t off)
 int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false)
 {
     const bool learned = c->any_learned;
-    if (fused_lc(c) > 0) {
+    if (fused_active(c)) {
         if (measure) {
             if (learned) launch_fused_kernel<true, true>(c, off);
             else launch_fused_kernel<true, false>(c, off);
@@ -1521,14 +1856,95 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false)
 }
 
 // Buffer offset of the state after `done` sweeps of a period.
-uint32_t buf_off(const ising_ctx* c, int done) { return fused_lc(c) > 0 ? (uint32_t)done : 0u; }
+uint32_t buf_off(const ising_ctx* c, int done) { return fused_active(c) ? (uint32_t)done : 0u; }
+
+// Mesh, twiddle and field buffers of the demagnetising field, allocated on
+// first use (the field starts at zero).
+int demag_setup(ising_ctx* c)
+{
+    if (c->demag_ready) return ISING_OK;
+    std::vector<double2> tw;
+    std::vector<std::pair<int, int>> tables;   // (length, first entry)
+    auto table = [&](int n) {
+        for (auto& t : tables)
+            if (t.first == n) return t.second;
+        const int first = (int)tw.size();
+        for (int m = 0; m < n; ++m) {
+            const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+            tw.push_back(double2{ (double)cosl(a), -(double)sinl(a) });
+        }
+        tables.emplace_back(n, first);
+        return first;
+    };
+    c->dreps.clear();
+    int64_t moff = 0, koff = 0;
+    for (const Rep& r : c->reps) {
+        DemagRep d;
+        const int64_t cells = (int64_t)r.lx * r.ly * r.lz;
+        d.hx = r.lx / 2 + 1;
+        const int64_t modes = (int64_t)d.hx * r.ly * r.lz;
+        d.moff = moff; d.koff = koff;
+        d.tw[0] = table(r.lx); d.tw[1] = table(r.ly); d.tw[2] = table(r.lz);
+        moff += 3 * cells; koff += 3 * modes;
+        c->dmax_cells = std::max(c->dmax_cells, cells);
+        c->dmax_modes = std::max(c->dmax_modes, modes);
+        c->dmax_sites = std::max(c->dmax_sites, r.n);
+        c->dreps.push_back(d);
+    }
+    int rc;
+    if ((rc = dalloc(c, &c->d_demag, (size_t)c->total_floats)) || (rc = dalloc(c, &c->d_dmesh, (size_t)moff)) ||
+        (rc = dalloc(c, &c->d_dk1, (size_t)koff)) || (rc = dalloc(c, &c->d_dk2, (size_t)koff)) ||
+        (rc = dalloc(c, &c->d_tw, tw.size())) || (rc = dalloc(c, &c->d_dreps, c->dreps.size())))
+        return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_dreps, c->dreps.data(), c->dreps.size() * sizeof(DemagRep), hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_demag, 0, (size_t)c->total_floats * sizeof(double), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->demag_ready = true;
+    return ISING_OK;
+}
+
+// Recomputes the field of the selected slots (demag_slot) from the current
+// spins, on the context's stream.
+int launch_demag(ising_ctx* c, int only)
+{
+    int rc = demag_setup(c);
+    if (rc) return rc;
+    const unsigned R = (unsigned)c->R;
+    auto grid = [&](int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock), R); };
+    hipStream_t s = c->stream;
+    const DemagRep* dr = c->d_dreps;
+    const double2* tw = c->d_tw;
+    hipLaunchKernelGGL(k_demag_deposit, grid(c->dmax_cells), dim3(kBlock), 0, s, (const float*)cur_spins(c),
+                       c->d_reps, c->d_lads, dr, only, c->d_dmesh);
+    hipLaunchKernelGGL(k_dft_r2c_x, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double*)c->d_dmesh,
+                       c->d_reps, c->d_lads, dr, only, tw, c->d_dk1);
+    hipLaunchKernelGGL(k_dft_c2c<1>, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double2*)c->d_dk1,
+                       c->d_reps, c->d_lads, dr, only, tw, -1, c->d_dk2);
+    hipLaunchKernelGGL(k_dft_c2c<2>, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double2*)c->d_dk2,
+                       c->d_reps, c->d_lads, dr, only, tw, -1, c->d_dk1);
+    hipLaunchKernelGGL(k_demag_project, grid(c->dmax_modes), dim3(kBlock), 0, s, c->d_dk1, c->d_reps, c->d_lads,
+                       dr, only);
+    hipLaunchKernelGGL(k_dft_c2c<2>, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double2*)c->d_dk1,
+                       c->d_reps, c->d_lads, dr, only, tw, +1, c->d_dk2);
+    hipLaunchKernelGGL(k_dft_c2c<1>, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double2*)c->d_dk2,
+                       c->d_reps, c->d_lads, dr, only, tw, +1, c->d_dk1);
+    hipLaunchKernelGGL(k_dft_c2r_x, grid(3 * c->dmax_cells), dim3(kBlock), 0, s, (const double2*)c->d_dk1,
+                       c->d_reps, c->d_lads, dr, only, tw, c->d_dmesh);
+    hipLaunchKernelGGL(k_demag_gather, grid(c->dmax_sites), dim3(kBlock), 0, s, (const double*)c->d_dmesh,
+                       c->d_reps, c->d_lads, dr, only, c->d_demag);
+    LAUNCH_CHECK(c);
+    return ISING_OK;
+}
 
 // Exact FP64 E and M of every slot, for the state after `done` sweeps of the
 // period.
 int launch_measure_all(ising_ctx* c, int done = 0)
 {
     hipLaunchKernelGGL(k_measure, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
-                       spin_buf(c), buf_off(c, done), c->d_reps, c->d_lads, (const Lad*)nullptr, c->d_block_rep, 0, c->d_partial);
+                       spin_buf(c), buf_off(c, done), c->d_reps, c->d_lads, (const Lad*)nullptr, c->d_block_rep, 0, c->d_partial,
+                       (const double*)c->d_demag);
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_measure_finalize, dim3(c->R), dim3(kBlock), 0, c->stream,
                        c->d_reps, c->d_partial, 0, c->d_energy, c->d_mag);
@@ -1567,7 +1983,7 @@ int launch_swap(ising_ctx* c)
 // once they are enqueued.
 int launch_advance(ising_ctx* c, uint32_t n)
 {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_ctr, n, fused_lc(c) > 0 ? n : 0u);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_ctr, n, fused_active(c) ? n : 0u);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
@@ -1575,7 +1991,7 @@ int launch_advance(ising_ctx* c, uint32_t n)
 void advance_host(ising_ctx* c, uint32_t n)
 {
     c->host_sweep += n;
-    if (fused_lc(c) > 0) c->host_fused += n;
+    if (fused_active(c)) c->host_fused += n;
 }
 
 struct SeriesDev {
@@ -1733,7 +2149,8 @@ void ising_destroy(ising_ctx* c)
                      c->d_ctr, c->d_acc, c->d_acc_wave, c->d_wave_part, c->d_mlp, c->d_mlp_prm, c->d_mlp_m1,
                      c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_grad_part, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de,
                      c->ck_spins, c->ck_sot, c->ck_bslot, c->ck_btemp, c->ck_ctr,
-                     c->d_fblock_rep, c->d_acc_wave_f, c->d_acc_base, c->d_wave_part_f };
+                     c->d_fblock_rep, c->d_acc_wave_f, c->d_acc_base, c->d_wave_part_f,
+                     c->d_demag, c->d_dmesh, c->d_dk1, c->d_dk2, c->d_tw, c->d_dreps };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1764,6 +2181,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
         const ising_ladder_desc& d = ladders[l];
         if ((rc = check_lattice(c, d.lattice))) return fail(rc);
         if ((rc = check_ham(c, d.ham))) return fail(rc);
+        if (d.ham.demag && (rc = check_demag_lattice(c, d.lattice))) return fail(rc);
         if (d.n_replicas < 1 || !d.betas) return fail(set_err(c, ISING_ERR_INVALID_ARG, "ladder %d: n_replicas/betas", l));
         if (!(d.cone_radius >= 0.0) || !std::isfinite(d.cone_radius))
             return fail(set_err(c, ISING_ERR_INVALID_ARG, "ladder %d: cone radius", l));
@@ -1902,6 +2320,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
         hipMemsetAsync(c->d_swap_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess)
         return fail(set_err(c, ISING_ERR_HIP, "hipMemsetAsync"));
     if ((rc = upload_lads(c))) return fail(rc);
+    if (c->any_demag && (rc = demag_setup(c))) return fail(rc);
     hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, s, (const float*)c->d_mlp_prm, c->d_mlp);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
         return fail(set_err(c, ISING_ERR_HIP, "network initialisation"));
@@ -1915,7 +2334,14 @@ int ising_set_hamiltonian(ising_ctx* c, int ladder, const ising_hamiltonian* h)
     if (ladder < 0 || ladder >= (int)c->ladders.size()) return set_err(c, ISING_ERR_INVALID_ARG, "ladder %d", ladder);
     int rc = check_ham(c, *h);
     if (rc) return rc;
+    if (h->demag) {
+        if ((rc = check_demag_lattice(c, c->ladders[ladder].lat))) return rc;
+        if (c->learned[ladder])
+            return set_err(c, ISING_ERR_UNSUPPORTED, "ladder %d takes its radius from the network, whose input has "
+                           "no demagnetising field", ladder);
+    }
     HIP_TRY(c, hipSetDevice(c->device));
+    if (h->demag && (rc = demag_setup(c))) return rc;
     c->ladders[ladder].ham = *h;
     return upload_lads(c);
 }
@@ -2032,8 +2458,16 @@ int ising_sweep(ising_ctx* c, int n_sweeps)
     if (n_sweeps == 0) return ISING_OK;
     HIP_TRY(c, hipSetDevice(c->device));
     int rc;
-    for (int k = 0; k < n_sweeps; ++k)
+    // Metropolis_mcs computes the field before the first sweep and after
+    // every demag_update_interval-th sweep (src/energyMin.f90:258-259,362-369).
+    const bool demag = c->any_demag;
+    if (demag && (rc = launch_demag(c, kDemagLadders))) return rc;
+    for (int k = 0; k < n_sweeps; ++k) {
         if ((rc = launch_sweep(c, (uint32_t)k))) return rc;
+        if (demag && c->demag_interval > 0 && (k + 1) % c->demag_interval == 0 &&
+            (rc = launch_demag(c, kDemagLadders)))
+            return rc;
+    }
     if ((rc = launch_advance(c, (uint32_t)n_sweeps))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     advance_host(c, (uint32_t)n_sweeps);
@@ -2044,7 +2478,9 @@ int ising_energy(ising_ctx* c, double* e)
 {
     if (!c || !e) return ISING_ERR_INVALID_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
-    int rc = launch_measure_all(c);
+    int rc;
+    if (c->any_demag && (rc = launch_demag(c, kDemagLadders))) return rc;   // TotalHeisenbergEnergy recomputes it
+    rc = launch_measure_all(c);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(e, c->d_energy, c->R * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -2068,12 +2504,15 @@ int ising_energy_with(ising_ctx* c, int replica, const ising_hamiltonian* h, dou
     if (rc) return rc;
     if (!h || !e) return set_err(c, ISING_ERR_INVALID_ARG, "null argument");
     if ((rc = check_ham(c, *h))) return rc;
+    if (h->demag && (rc = check_demag_lattice(c, c->ladders[c->reps[replica].ladder].lat))) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
+    if (h->demag && (rc = launch_demag(c, replica))) return rc;
     const Rep& r = c->reps[replica];
     Lad L = make_lad(*h, c->ladders[r.ladder].cone, r.cell);
     HIP_TRY(c, hipMemcpyAsync(c->d_lad_override, &L, sizeof L, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_measure, dim3(r.nblocks), dim3(kBlock), 0, c->stream, spin_buf(c), 0u, c->d_reps,
-                       c->d_lads, (const Lad*)c->d_lad_override, c->d_block_rep, r.first_block, c->d_partial);
+                       c->d_lads, (const Lad*)c->d_lad_override, c->d_block_rep, r.first_block, c->d_partial,
+                       (const double*)c->d_demag);
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_measure_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->d_reps, c->d_partial, replica,
                        c->d_energy, c->d_mag);
@@ -2156,7 +2595,7 @@ int ising_delta_energy(ising_ctx* c, int replica, const int32_t* sites, const do
     HIP_TRY(c, hipMemcpyAsync(c->d_props, pf.data(), pf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_delta_energy, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
                        (const float*)cur_spins(c), c->d_reps, c->d_lads, replica, (const int32_t*)c->d_sites,
-                       (const float*)c->d_props, n, c->d_de);
+                       (const float*)c->d_props, n, c->d_de, (const double*)c->d_demag);
     LAUNCH_CHECK(c);
     HIP_TRY(c, hipMemcpyAsync(de, c->d_de, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -2166,6 +2605,9 @@ int ising_delta_energy(ising_ctx* c, int replica, const int32_t* sites, const do
 int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* out)
 {
     if (!c || !sched) return ISING_ERR_INVALID_ARG;
+    if (c->any_demag)
+        return set_err(c, ISING_ERR_UNSUPPORTED, "parallel tempering with the demagnetisation term (the reference's "
+                       "PT runs without it)");
     const ising_schedule s = *sched;
     if (s.n_sweeps < 0 || s.swap_every < 0 || s.measure_every < 0 || s.winding_every < 0)
         return set_err(c, ISING_ERR_INVALID_ARG, "negative schedule entry");
@@ -2377,6 +2819,8 @@ int ising_set_tuning(ising_ctx* c, uint32_t flags)
     if (flags & ~(kTuneXcdRemap | kTuneNtOwn | kTuneFusedMask))
         return set_err(c, ISING_ERR_INVALID_ARG, "unknown tuning flags %u", flags);
     const int lc = (int)((flags & kTuneFusedMask) >> kTuneFusedShift);
+    if (lc > 0 && c->any_demag)
+        return set_err(c, ISING_ERR_UNSUPPORTED, "fused sweep: a ladder has the demagnetisation term");
     if (lc > 0 && !c->fused_ok)
         return set_err(c, ISING_ERR_UNSUPPORTED,
                        "fused sweep unavailable: it needs simple-cubic replicas whose z plane of one colour "
@@ -2404,7 +2848,7 @@ int ising_get_tuning(const ising_ctx* c, uint32_t* flags)
 static std::vector<uint32_t> tuning_candidates(const ising_ctx* c)
 {
     std::vector<uint32_t> v = { 0u, kTuneXcdRemap, kTuneNtOwn, kTuneXcdRemap | kTuneNtOwn };
-    if (c->fused_ok) {
+    if (c->fused_ok && !c->any_demag) {
         std::vector<int> lcs;
         for (int lc : { 8, 16, 32, c->zmax }) {
             lc = std::min(std::min(lc, c->zmax), 255);
@@ -2489,8 +2933,12 @@ int ising_mlp_enable(ising_ctx* c, int ladder, int on)
     if (ladder >= (int)c->ladders.size()) return set_err(c, ISING_ERR_INVALID_ARG, "ladder %d", ladder);
     // The network's input is the six neighbours of a simple-cubic site.
     for (int k = 0; k < (int)c->ladders.size(); ++k)
+    {
         if (on && (ladder < 0 || k == ladder) && c->ladders[k].lat.cell != ISING_CELL_SC)
             return set_err(c, ISING_ERR_UNSUPPORTED, "learned proposal: ladder %d is not simple cubic", k);
+        if (on && (ladder < 0 || k == ladder) && c->ladders[k].ham.demag)
+            return set_err(c, ISING_ERR_UNSUPPORTED, "learned proposal: ladder %d has the demagnetisation term", k);
+    }
     HIP_TRY(c, hipSetDevice(c->device));
     for (int k = 0; k < (int)c->learned.size(); ++k)
         if (ladder < 0 || k == ladder) c->learned[k] = on ? 1 : 0;
@@ -2503,8 +2951,9 @@ int ising_mlp_collect(ising_ctx* c, int per_replica)
 {
     if (!c || per_replica < 1) return ISING_ERR_INVALID_ARG;
     for (const LadHost& L : c->ladders)
-        if (L.lat.cell != ISING_CELL_SC)
-            return set_err(c, ISING_ERR_UNSUPPORTED, "training records: every ladder must be simple cubic");
+        if (L.lat.cell != ISING_CELL_SC || L.ham.demag)
+            return set_err(c, ISING_ERR_UNSUPPORTED,
+                           "training records: every ladder must be simple cubic, without the demagnetisation term");
     HIP_TRY(c, hipSetDevice(c->device));
     const int64_t add = (int64_t)c->R * per_replica;
     if (c->n_recs + add > (int64_t)UINT32_MAX) return set_err(c, ISING_ERR_INVALID_ARG, "too many training records");
@@ -2681,6 +3130,80 @@ int ising_accept_counts(ising_ctx* c, int64_t* accepted)
     LAUNCH_CHECK(c);
     HIP_TRY(c, hipMemcpyAsync(accepted, c->d_acc, c->R * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+// ---- demagnetisation ---------------------------------------------------------
+
+static int demag_replica_args(ising_ctx* c, int replica, const void* h, int64_t n)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    if (!h) return set_err(c, ISING_ERR_INVALID_ARG, "null field");
+    const Rep& r = c->reps[replica];
+    if (n != r.n) return set_err(c, ISING_ERR_INVALID_ARG, "n = %lld, replica has %lld sites", (long long)n, (long long)r.n);
+    return check_demag_lattice(c, c->ladders[r.ladder].lat);
+}
+
+static int copy_out_demag(ising_ctx* c, int replica, double* h)
+{
+    const Rep& r = c->reps[replica];
+    std::vector<double> dev((size_t)3 * r.n);
+    HIP_TRY(c, hipMemcpyAsync(dev.data(), c->d_demag + r.off, dev.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < r.n; ++i) {
+        const int64_t t = dev_index(r, i);
+        for (int k = 0; k < 3; ++k) h[k * r.n + i] = dev[(size_t)k * r.n + t];
+    }
+    return ISING_OK;
+}
+
+int ising_demag_field(ising_ctx* c, int replica, double* h, int64_t n)
+{
+    int rc = demag_replica_args(c, replica, h, n);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = launch_demag(c, replica))) return rc;
+    return copy_out_demag(c, replica, h);
+}
+
+int ising_get_demag_field(ising_ctx* c, int replica, double* h, int64_t n)
+{
+    int rc = demag_replica_args(c, replica, h, n);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = demag_setup(c))) return rc;
+    return copy_out_demag(c, replica, h);
+}
+
+int ising_set_demag_field(ising_ctx* c, int replica, const double* h, int64_t n)
+{
+    int rc = demag_replica_args(c, replica, h, n);
+    if (rc) return rc;
+    const Rep& r = c->reps[replica];
+    std::vector<double> dev((size_t)3 * r.n);
+    for (int64_t i = 0; i < r.n; ++i) {
+        const int64_t t = dev_index(r, i);
+        for (int k = 0; k < 3; ++k) {
+            const double v = h[k * r.n + i];
+            if (!std::isfinite(v)) return set_err(c, ISING_ERR_INVALID_ARG, "non-finite field at site %lld", (long long)i);
+            dev[(size_t)k * r.n + t] = v;
+        }
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = demag_setup(c))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->d_demag + r.off, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_set_demag_interval(ising_ctx* c, int sweeps)
+{
+    if (!c) return ISING_ERR_INVALID_ARG;
+    if (sweeps < 0) return set_err(c, ISING_ERR_INVALID_ARG, "interval %d", sweeps);
+    c->demag_interval = sweeps;
     return ISING_OK;
 }
 

@@ -9,6 +9,8 @@ replicas, sweeps, swaps) and map one to one onto the C entry points:
     Engine.winding          -> ising_winding         (calculate_winding_number2, src/reciprocalSpaceProcesses.f90:320-364)
     Engine.magnetisation    -> ising_magnetisation   (src/PT-Utils.f90:59-65)
     Engine.pt_run           -> ising_pt_run          (PT loop, src/PT.f90:219-273)
+    Engine.demag_field      -> ising_demag_field     (calculate_demagnetisation_field,
+                                                      src/reciprocalSpaceProcesses.f90:202-281)
 
 Spin arrays use the reference's atomSpins(numAtoms,3) column-major layout,
 i.e. numpy arrays of shape (3, N), sites in the order (ix*Ly + iy)*Lz + iz.
@@ -69,19 +71,22 @@ class Hamiltonian:
     Bz: float = 0.0
     Kz: float = 0.0
     zeeman_total_scale: float = 1.0
+    demag: bool = False       # the reference's demagnetisation term (mu0 = MU0_REF)
 
     @classmethod
-    def reference(cls, J: float, Dz: float, B: float, total_energy_convention: bool = True) -> "Hamiltonian":
+    def reference(cls, J: float, Dz: float, B: float, total_energy_convention: bool = True,
+                  demag: bool = False) -> "Hamiltonian":
         """The reference's parameters: exchange +J S.S (src/energyMin.f90:139),
         interfacial DMI Dz (c x r) (:114,137), Zeeman -B S_z in single-site
-        changes (:161) and -2 B S_z in total energies (:222)."""
+        changes (:161) and -2 B S_z in total energies (:222), optional
+        demagnetisation term -mu0 S.H (:113-116,163-166)."""
         return cls(J=-J, D=Dz, dmi="interfacial_ref", Bz=B, Kz=0.0,
-                   zeeman_total_scale=2.0 if total_energy_convention else 1.0)
+                   zeeman_total_scale=2.0 if total_energy_convention else 1.0, demag=demag)
 
     def _c(self) -> _capi.ising_hamiltonian:
         if self.dmi not in _DMI:
             raise ValueError(f"dmi must be one of {sorted(_DMI)}")
-        return _capi.ising_hamiltonian(J=self.J, D=self.D, dmi_type=_DMI[self.dmi], reserved=0,
+        return _capi.ising_hamiltonian(J=self.J, D=self.D, dmi_type=_DMI[self.dmi], demag=1 if self.demag else 0,
                                        Bz=self.Bz, Kz=self.Kz, zeeman_total_scale=self.zeeman_total_scale)
 
 
@@ -250,6 +255,32 @@ class Engine:
                                                  p.ctypes.data_as(C.POINTER(C.c_double)), len(s),
                                                  out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+    # -- demagnetisation -----------------------------------------------------
+    def demag_field(self, replica: int) -> np.ndarray:
+        """Field H of the current spins, shape (3, N) (the reference's
+        demagnetisation_array, without mu0)."""
+        n = self.sites(replica)
+        h = np.empty((3, n), dtype=np.float64)
+        self._check(self._lib.ising_demag_field(self._h, replica, h.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return h
+
+    def stored_demag_field(self, replica: int) -> np.ndarray:
+        """The field the replica's sweeps use now (last computed or set)."""
+        n = self.sites(replica)
+        h = np.empty((3, n), dtype=np.float64)
+        self._check(self._lib.ising_get_demag_field(self._h, replica, h.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return h
+
+    def set_demag_field(self, replica: int, h: np.ndarray) -> None:
+        n = self.sites(replica)
+        a = np.ascontiguousarray(h, dtype=np.float64)
+        if a.shape != (3, n):
+            raise ValueError(f"field must have shape (3, {n})")
+        self._check(self._lib.ising_set_demag_field(self._h, replica, a.ctypes.data_as(C.POINTER(C.c_double)), n))
+
+    def set_demag_interval(self, sweeps: int) -> None:
+        self._check(self._lib.ising_set_demag_interval(self._h, int(sweeps)))
 
     # -- parallel tempering -------------------------------------------------
     def pt_run(self, n_sweeps: int, swap_every: int = 10, measure_every: int = 1, winding_every: int = 0,

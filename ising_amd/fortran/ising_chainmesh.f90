@@ -21,6 +21,9 @@ module ChainMesh
                 integer :: atomsPerCell = 1
                 real(kind=8) :: a = 1.0_8     ! lattice constant (positions in output files)
                 real(kind=8), allocatable :: atomSpins(:,:)
+                ! the last demagnetising field computed (src/chainMesh.f90:38),
+                ! filled by the demag = .true. paths of EnergyMin
+                real(kind=8), allocatable :: demagnetisation_array(:,:)
                 type(c_ptr) :: gpu = c_null_ptr
                 integer(c_int64_t) :: seed = 0
         end type ChainMesh_t
@@ -56,6 +59,9 @@ contains
                 if (stat /= 0) error stop "Error: Failed to allocated atomSpins array"
                 mesh%atomSpins = 0.0_8
                 mesh%atomSpins(:, 3) = 1.0_8
+                allocate(mesh%demagnetisation_array(mesh%numAtoms, 3), stat=stat)
+                if (stat /= 0) error stop "Error: Failed to allocate demagnetisation_array"
+                mesh%demagnetisation_array = 0.0_8
         end function makeChainMesh
 
         subroutine deallocateChainMesh(mesh)
@@ -63,6 +69,7 @@ contains
                 if (c_associated(mesh%gpu)) call ising_destroy(mesh%gpu)
                 mesh%gpu = c_null_ptr
                 if (allocated(mesh%atomSpins)) deallocate(mesh%atomSpins)
+                if (allocated(mesh%demagnetisation_array)) deallocate(mesh%demagnetisation_array)
         end subroutine deallocateChainMesh
 
         ! 1-based cell index, z fastest (src/chainMesh.f90:673-685).
@@ -94,7 +101,7 @@ contains
                 do a = 1, 3
                         d(1)%lattice%periodic(a) = merge(1_c_int32_t, 0_c_int32_t, mesh%periodic(a))
                 end do
-                d(1)%ham = ising_hamiltonian(0.0_c_double, 0.0_c_double, ISING_DMI_NONE, 0, &
+                d(1)%ham = ising_hamiltonian(0.0_c_double, 0.0_c_double, ISING_DMI_NONE, 0_c_int32_t, &
                                              0.0_c_double, 0.0_c_double, 1.0_c_double)
                 d(1)%n_replicas = 1
                 d(1)%reserved = 0

@@ -30,6 +30,9 @@ module ising_gpu_c
         integer(c_int), parameter :: ISING_MLP_PARAMS = 1793
         integer(c_int), parameter :: ISING_MLP_RECORD = 32
 
+        ! the reference's constants::mu_0 (src/constants.f90:15), single-precision literal
+        real(c_double), parameter :: ISING_MU0_REF = 1.256637006008532e-06_c_double
+
         integer(c_int32_t), parameter :: ISING_CELL_SC = 0
         integer(c_int32_t), parameter :: ISING_CELL_BCC = 1
 
@@ -41,7 +44,7 @@ module ising_gpu_c
 
         type, bind(C) :: ising_hamiltonian
                 real(c_double) :: J, D
-                integer(c_int32_t) :: dmi_type, reserved
+                integer(c_int32_t) :: dmi_type, demag
                 real(c_double) :: Bz, Kz, zeeman_total_scale
         end type ising_hamiltonian
 
@@ -376,6 +379,36 @@ module ising_gpu_c
                         type(c_ptr), value :: ctx
                         real(c_float), intent(in) :: a(*), b(*)
                         real(c_float), intent(out) :: d(*)
+                        integer(c_int) :: rc
+                end function
+                function ising_demag_field(ctx, replica, h, n) bind(C, name="ising_demag_field") result(rc)
+                        import :: c_ptr, c_int, c_int64_t, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: replica
+                        real(c_double), intent(out) :: h(*)
+                        integer(c_int64_t), value :: n
+                        integer(c_int) :: rc
+                end function
+                function ising_get_demag_field(ctx, replica, h, n) bind(C, name="ising_get_demag_field") result(rc)
+                        import :: c_ptr, c_int, c_int64_t, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: replica
+                        real(c_double), intent(out) :: h(*)
+                        integer(c_int64_t), value :: n
+                        integer(c_int) :: rc
+                end function
+                function ising_set_demag_field(ctx, replica, h, n) bind(C, name="ising_set_demag_field") result(rc)
+                        import :: c_ptr, c_int, c_int64_t, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: replica
+                        real(c_double), intent(in) :: h(*)
+                        integer(c_int64_t), value :: n
+                        integer(c_int) :: rc
+                end function
+                function ising_set_demag_interval(ctx, sweeps) bind(C, name="ising_set_demag_interval") result(rc)
+                        import :: c_ptr, c_int
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: sweeps
                         integer(c_int) :: rc
                 end function
                 function c_strlen(s) bind(C, name="strlen") result(n)

@@ -62,6 +62,20 @@ contains
                 close(unit)
         end subroutine write_winding_number_density
 
+        ! Replaces calculate_demagnetisation_field (src/reciprocalSpaceProcesses.f90:202-281):
+        ! the demagnetising field of the current spins, computed on the device
+        ! (cloud-in-cell deposit, discrete Fourier transform, -(k.M) k / k^2,
+        ! inverse, weights back to the atoms), outputArray(numAtoms, 3).
+        subroutine calculate_demagnetisation_field(chainMesh, outputArray)
+                type(ChainMesh_t), intent(inout) :: chainMesh
+                real(kind=8), allocatable, dimension(:,:), intent(out) :: outputArray
+                integer(c_int) :: rc
+                allocate(outputArray(chainMesh%numAtoms, 3))
+                call chainmesh_push_spins(chainMesh)
+                rc = ising_demag_field(chainMesh%gpu, 0_c_int, outputArray, int(chainMesh%numAtoms, c_int64_t))
+                call ising_check(rc, chainMesh%gpu, "calculate_demagnetisation_field")
+        end subroutine calculate_demagnetisation_field
+
         subroutine write_density_line(unit, x, y, z, d)
                 integer, intent(in) :: unit
                 real(kind=8), intent(in) :: x, y, z, d

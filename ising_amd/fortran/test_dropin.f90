@@ -1,7 +1,8 @@
 ! test_dropin.f90 -- exercises the drop-in Fortran entry points the way the
 ! reference's PT driver calls them (src/PT.f90:229-230, 251-254, 316) and
 ! prints one "name value" line per check; tests/test_fortran_dropin.py
-! compares the values with the FP64 oracle and closed forms.
+! compares the values with the FP64 oracle and closed forms. With a directory
+! argument it also writes the demagnetisation check's spins and fields there.
 program test_dropin
         use iso_c_binding
         use omp_lib, only: OMP_LOCK_KIND
@@ -10,14 +11,16 @@ program test_dropin
         use PT_Utils
         use reciprocal_space_processes
         implicit none
-        type(ChainMesh_t) :: mesh, fm
+        type(ChainMesh_t) :: mesh, fm, dm
         integer(kind=OMP_LOCK_KIND), allocatable :: lockArray(:)
         real(kind=8) :: e0, e1, eold, enew, winding_middle, winding_spread, m(3), sp(3), s0(3)
         real(kind=8) :: J, Dz, B, beta, pi
         integer :: skyr, i, ix, iy, iz, k
         real(kind=8) :: q, th
-        real(kind=8), allocatable :: density(:,:)
+        real(kind=8), allocatable :: density(:,:), hfield(:,:)
         integer :: peak(2)
+        character(len=512) :: outdir
+        integer :: unit, nargs
 
         pi = acos(-1.0_8)
         J = -1.0_8; Dz = 0.6_8; B = 0.3_8; beta = 2.0_8
@@ -82,7 +85,50 @@ program test_dropin
         print '(A,ES24.16)', "sk_density_max ", maxval(abs(density))
         peak = maxloc(abs(density))
         print '(A,I0,A,I0)', "sk_density_peak ", peak(1), " ", peak(2)
+        ! Demagnetisation: bcc mesh 4 x 3 x 3 (periodic), a conical texture.
+        nargs = command_argument_count()
+        dm = makeChainMesh(4, 3, 3, cell=ISING_CELL_BCC)
+        do i = 1, dm%numAtoms
+                th = 0.37_8 * i
+                dm%atomSpins(i, :) = [0.6_8 * cos(th), 0.6_8 * sin(th), 0.8_8]
+        end do
+        call calculate_demagnetisation_field(dm, hfield)
+        call TotalHeisenbergEnergy(dm, J, 0.0_8, Dz, 0.0_8, B, lockArray, e0, .true.)
+        call TotalHeisenbergEnergy(dm, J, 0.0_8, Dz, 0.0_8, B, lockArray, e1, .false.)
+        print '(A,ES24.16)', "demag_energy_term ", e0 - e1
+        print '(A,ES24.16)', "demag_array_vs_field ", maxval(abs(dm%demagnetisation_array - hfield))
+        ! calculateHeisenbergEnergy takes the field as an argument; one large
+        ! enough for mu_0 H to show at the FP32 resolution of the local field
+        sp = [0.0_8, 0.6_8, -0.8_8]
+        allocate(density(dm%numAtoms, 3))
+        density(:, 1) = 2.0e5_8
+        density(:, 2) = -1.0e5_8
+        density(:, 3) = 3.0e5_8
+        call calculateHeisenbergEnergy(dm, 5, J, 0.0_8, Dz, 0.0_8, B, lockArray, sp, eold, enew, density, .true.)
+        print '(A,2ES24.16)', "demag_local ", eold, enew
+        call calculateHeisenbergEnergy(dm, 5, J, 0.0_8, Dz, 0.0_8, B, lockArray, sp, e0, e1, density, .false.)
+        print '(A,2ES24.16)', "nodemag_local ", e0, e1
+        deallocate(density)
+        if (nargs >= 1) then
+                call get_command_argument(1, outdir)
+                open(newunit=unit, file=trim(outdir)//"/demag_in.txt", status="replace", action="write")
+                do i = 1, dm%numAtoms
+                        write(unit, '(3ES24.16)') dm%atomSpins(i, :)
+                end do
+                close(unit)
+                open(newunit=unit, file=trim(outdir)//"/demag_field.txt", status="replace", action="write")
+                do i = 1, dm%numAtoms
+                        write(unit, '(3ES24.16)') hfield(i, :)
+                end do
+                close(unit)
+        end if
+        ! r = 0.5: the field is recomputed after sweep int(pi sqrt 5) = 7, the last one
+        call Metropolis_mcs(dm, beta, 7, J, 0.0_8, Dz, 0.0_8, B, 0.5_8, lockArray, demag=.true.)
+        call calculate_demagnetisation_field(dm, hfield)
+        print '(A,ES24.16)', "demag_after_sweeps ", maxval(abs(dm%demagnetisation_array - hfield))
+        print '(A,ES24.16)', "demag_norm_dev ", maxval(abs(sqrt(sum(dm%atomSpins**2, dim=2)) - 1.0_8))
         call deallocateChainMesh(mesh)
         call deallocateChainMesh(fm)
+        call deallocateChainMesh(dm)
         print '(A)', "done"
 end program test_dropin

@@ -27,6 +27,10 @@
  *   orc_sweeps              single-spin Langevin <Sz>, two-spin dimer quadrature,
  *                           high-temperature series (tests/test_oracle_metropolis.py)
  *   orc_pt_swap_round       two-replica detailed balance (tests/test_oracle_pt.py)
+ *   orc_delta_energy_field  orc_delta_energy plus the field term; the field itself is
+ *                           pinned in demag_oracle.c (tests/test_oracle_demag.py)
+ *   orc_sweeps_demag        field schedule of Metropolis_mcs; single spin in a fixed
+ *                           external field follows the Langevin law (tests/test_oracle_demag.py)
  */
 #include "ising_oracle.h"
 #include <math.h>
@@ -656,8 +660,11 @@ int orc_propose(const double so[3], double radius, const uint32_t rnd[4], double
 /* One colour of the exact checkerboard sweep: every site with
  * (ix+iy+iz) % 2 == colour gets one proposal, accepted iff u < exp(-beta dE).
  * Same-colour sites share no bond, so the order inside a colour is free. */
-int64_t orc_half_sweep(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
-                       uint64_t seed, uint32_t slot, uint32_t sweep, int colour)
+/* One colour of one sweep; H (may be NULL) is an external per-site field
+ * entering dE as -mu0 (S' - S).H_i, the reference's demagnetisation term
+ * (src/energyMin.f90:113-116,163-166). */
+static int64_t half_sweep_field(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
+                                uint64_t seed, uint32_t slot, uint32_t sweep, int colour, const double* H, double mu0)
 {
     const int64_t N = nsites(L);
     int64_t acc = 0;
@@ -669,13 +676,19 @@ int64_t orc_half_sweep(const orc_lattice* L, const orc_ham* h, double* s, double
         double so[3], sn[3], u;
         load(s, N, i, so);
         if (!orc_propose(so, radius, rnd, sn, &u)) continue;
-        double de = orc_delta_energy(L, h, s, i, sn);
+        double de = H ? orc_delta_energy_field(L, h, s, i, sn, H, mu0) : orc_delta_energy(L, h, s, i, sn);
         if (u < exp(-beta * de)) {
             s[i] = sn[0]; s[N + i] = sn[1]; s[2 * N + i] = sn[2];
             acc++;
         }
     }
     return acc;
+}
+
+int64_t orc_half_sweep(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
+                       uint64_t seed, uint32_t slot, uint32_t sweep, int colour)
+{
+    return half_sweep_field(L, h, s, beta, radius, seed, slot, sweep, colour, NULL, 0.0);
 }
 
 int64_t orc_sweeps(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
@@ -689,8 +702,35 @@ int64_t orc_sweeps(const orc_lattice* L, const orc_ham* h, double* s, double bet
     return acc;
 }
 
-/* Seeded uniform-on-sphere start: z = 2U - 1, phi = 2 pi U' from the init
- * stream (cos(theta) uniform, as tests/random_test.py:33-39 settles). */
+double orc_delta_energy_field(const orc_lattice* L, const orc_ham* h, const double* s, int64_t site,
+                              const double sn[3], const double* H, double mu0)
+{
+    const int64_t N = nsites(L);
+    double so[3], hd[3];
+    load(s, N, site, so);
+    load(H, N, site, hd);
+    const double del[3] = { sn[0] - so[0], sn[1] - so[1], sn[2] - so[2] };
+    return orc_delta_energy(L, h, s, site, sn) - mu0 * dot(del, hd);
+}
+
+/* Metropolis_mcs with demagnetisation (src/energyMin.f90:258-259,362-369):
+ * the field is computed from the spins before the first sweep and again
+ * after every sweep whose 1-based number is a multiple of `interval`
+ * (interval <= 0: only before the first); H receives the last field. */
+int64_t orc_sweeps_demag(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
+                         uint64_t seed, uint32_t slot, uint32_t sweep0, int n_sweeps, double mu0, int interval,
+                         double* H)
+{
+    if (orc_ref_demag_field(L, s, H)) return -1;
+    int64_t acc = 0;
+    for (int k = 0; k < n_sweeps; ++k) {
+        acc += half_sweep_field(L, h, s, beta, radius, seed, slot, sweep0 + (uint32_t)k, 0, H, mu0);
+        acc += half_sweep_field(L, h, s, beta, radius, seed, slot, sweep0 + (uint32_t)k, 1, H, mu0);
+        if (interval > 0 && (k + 1) % interval == 0) orc_ref_demag_field(L, s, H);
+    }
+    return acc;
+}
+
 void orc_init_random(const orc_lattice* L, double* s, uint64_t seed, uint32_t slot)
 {
     const int64_t N = nsites(L);

@@ -86,6 +86,22 @@ int64_t orc_sweeps(const orc_lattice* L, const orc_ham* h, double* s, double bet
 void    orc_init_random(const orc_lattice* L, double* s, uint64_t seed, uint32_t slot);
 void    orc_init_ferro(const orc_lattice* L, double* s, uint64_t seed, uint32_t slot, double frac);
 
+/* ---- demagnetisation (demag_oracle.c) ----
+ * H: double[3*N] in the spin layout (the reference's demagnetisation_array);
+ * mesh arrays: double[3*Nc], cell (i*Ly + j)*Lz + k. mu0 is the reference's
+ * constants::mu_0 (a single-precision literal, 1.256637006008532e-6). */
+#define ORC_MU0_REF 1.256637006008532e-06
+int     orc_ref_demag_field(const orc_lattice* L, const double* s, double* H);   /* -1: not fully periodic */
+void    orc_demag_deposit(const orc_lattice* L, const double* s, double* mesh);
+void    orc_demag_mesh_field(const orc_lattice* L, const double* mesh, double* hmesh);
+void    orc_demag_gather(const orc_lattice* L, const double* hmesh, double* H);
+double  orc_demag_energy(const orc_lattice* L, const double* s, const double* H, double mu0);
+double  orc_delta_energy_field(const orc_lattice* L, const orc_ham* h, const double* s, int64_t site,
+                               const double sn[3], const double* H, double mu0);
+int64_t orc_sweeps_demag(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
+                         uint64_t seed, uint32_t slot, uint32_t sweep0, int n_sweeps, double mu0, int interval,
+                         double* H);
+
 /* ---- parallel tempering ---- */
 double  orc_swap_uniform(uint64_t seed, uint32_t ladder, uint32_t pair, uint32_t round);
 int     orc_pt_swap_round(int n_temps, const double* beta_of_temp, const double* e_of_slot,

@@ -74,6 +74,14 @@ def lib() -> C.CDLL:
             "orc_swap_uniform": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
             "orc_pt_swap_round": (C.c_int, [C.c_int, dp, dp, C.POINTER(C.c_int32), C.c_uint64, C.c_uint32,
                                             C.c_uint32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+            "orc_ref_demag_field": (C.c_int, [lp, dp, dp]),
+            "orc_demag_deposit": (None, [lp, dp, dp]),
+            "orc_demag_mesh_field": (None, [lp, dp, dp]),
+            "orc_demag_gather": (None, [lp, dp, dp]),
+            "orc_demag_energy": (C.c_double, [lp, dp, dp, C.c_double]),
+            "orc_delta_energy_field": (C.c_double, [lp, hp, dp, C.c_int64, dp, dp, C.c_double]),
+            "orc_sweeps_demag": (C.c_int64, [lp, hp, dp, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                                             C.c_uint32, C.c_int, C.c_double, C.c_int, dp]),
             "orc_num_threads": (C.c_int, []),
             "orc_set_threads": (None, [C.c_int]),
             "orc_mlp_forward": (C.c_double, [dp, dp, C.c_int]),
@@ -319,3 +327,65 @@ def mlp_loss_grad(params, recs, uvecs, quantize=True, lo=MLP_LOG_RMIN, hi=MLP_LO
     g = np.zeros(MLP_NPARAM)
     loss = lib().orc_mlp_loss_grad(_dp(p), _dp(r), _dp(u), r.shape[0], 1 if quantize else 0, lo, hi, _dp(g))
     return loss, g
+
+
+# ---- demagnetisation ------------------------------------------------------
+MU0_REF = 1.256637006008532e-06   # the reference's constants::mu_0 (single-precision literal)
+
+
+def ncells(L) -> int:
+    return L.lx * L.ly * L.lz
+
+
+def demag_field(L, s) -> np.ndarray:
+    """calculate_demagnetisation_field restated: H, shape (3, N)."""
+    s = _spins(s)
+    H = np.zeros_like(s)
+    rc = lib().orc_ref_demag_field(C.byref(L), _dp(s), _dp(H))
+    if rc:
+        raise ValueError("demagnetisation needs a fully periodic lattice")
+    return H
+
+
+def demag_deposit(L, s) -> np.ndarray:
+    """Cloud-in-cell mesh, shape (3, lx, ly, lz)."""
+    s = _spins(s)
+    m = np.zeros(3 * ncells(L))
+    lib().orc_demag_deposit(C.byref(L), _dp(s), _dp(m))
+    return m.reshape(3, L.lx, L.ly, L.lz)
+
+
+def demag_mesh_field(L, mesh) -> np.ndarray:
+    mesh = np.ascontiguousarray(mesh, dtype=np.float64)
+    h = np.zeros(3 * ncells(L))
+    lib().orc_demag_mesh_field(C.byref(L), _dp(mesh), _dp(h))
+    return h.reshape(3, L.lx, L.ly, L.lz)
+
+
+def demag_gather(L, hmesh) -> np.ndarray:
+    hmesh = np.ascontiguousarray(hmesh, dtype=np.float64)
+    H = np.zeros((3, nsites(L)))
+    lib().orc_demag_gather(C.byref(L), _dp(hmesh), _dp(H))
+    return H
+
+
+def demag_energy(L, s, H, mu0=MU0_REF) -> float:
+    s, H = _spins(s), _spins(H)
+    return lib().orc_demag_energy(C.byref(L), _dp(s), _dp(H), mu0)
+
+
+def delta_energy_field(L, h, s, i, sn, H, mu0=MU0_REF) -> float:
+    s, H = _spins(s), _spins(H)
+    sn = np.ascontiguousarray(sn, dtype=np.float64)
+    return lib().orc_delta_energy_field(C.byref(L), C.byref(h), _dp(s), i, _dp(sn), _dp(H), mu0)
+
+
+def sweeps_demag(L, h, s, beta, radius, seed, slot, sweep0, n, mu0=MU0_REF, interval=1) -> tuple[int, np.ndarray]:
+    """Metropolis_mcs with demagnetisation, in place on s; returns (accepts, last field)."""
+    assert s.flags.c_contiguous and s.dtype == np.float64
+    H = np.zeros_like(s)
+    acc = lib().orc_sweeps_demag(C.byref(L), C.byref(h), _dp(s), beta, radius, seed, slot, sweep0, n, mu0,
+                                 interval, _dp(H))
+    if acc < 0:
+        raise ValueError("demagnetisation needs a fully periodic lattice")
+    return acc, H

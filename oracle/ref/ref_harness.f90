@@ -5,7 +5,7 @@
 ! restatements with these.
 module ref_harness
         use iso_c_binding
-        use constants, only: pi
+        use constants, only: pi, mu_0
         use Rand, only: random, makeRandom, algor_uniform_random
         use vecNd
         implicit none
@@ -29,6 +29,13 @@ contains
                 real(c_double) :: p
                 p = pi
         end function ref_pi
+
+        ! The value of constants::mu_0 (src/constants.f90:15), the
+        ! permeability the demagnetisation energy is scaled by.
+        function ref_mu0() bind(C, name="ref_mu0") result(p)
+                real(c_double) :: p
+                p = mu_0
+        end function ref_mu0
 
         ! a .x. b and a * b of vecNd (src/vecNd.f90:213-237, 413-422).
         subroutine ref_cross_dot(a, b, c, d) bind(C, name="ref_cross_dot")

@@ -4,7 +4,7 @@ Run in a container where /root/reference is mounted, after
 `make -C oracle/ref` has compiled constants.f90, rand.f90 and vecNd.f90 from
 /root/reference/src into oracle/_ref/libref_leaf.so. The fixture holds only
 numbers (draws of algor_uniform_random, constants::pi, vecNd cross and dot
-products, the imaginary part of a kind-less cmplx()).
+products, the imaginary part of a kind-less cmplx(), constants::mu_0).
 """
 import ctypes as C
 import json
@@ -19,6 +19,7 @@ LIB = ROOT / "oracle" / "_ref" / "libref_leaf.so"
 def main():
     L = C.CDLL(str(LIB))
     L.ref_pi.restype = C.c_double
+    L.ref_mu0.restype = C.c_double
     L.ref_cmplx_default_kind_imag.restype = C.c_double
     L.ref_cmplx_default_kind_imag.argtypes = [C.c_double]
     out = {"source": "reference src/constants.f90, src/rand.f90, src/vecNd.f90 built by oracle/ref/Makefile"}
@@ -29,6 +30,7 @@ def main():
         streams[str(seed)] = [float(v) for v in buf]
     out["uniform_streams"] = streams
     out["pi"] = L.ref_pi()
+    out["mu0"] = L.ref_mu0()
     rng = np.random.default_rng(5)
     cd = []
     for _ in range(16):

@@ -92,6 +92,37 @@ def test_dropin_entry_points_on_gpu():
 
 @pytest.mark.gpu
 @needs_fortran
+def test_dropin_demagnetisation_on_gpu(tmp_path, orc):
+    """calculate_demagnetisation_field, TotalHeisenbergEnergy and
+    calculateHeisenbergEnergy with demag = .true., and Metropolis_mcs's field
+    schedule, against the oracle's restatement (bcc 4 x 3 x 3)."""
+    p = run([str(LIB / "test_dropin"), str(tmp_path)])
+    assert p.returncode == 0, p.stdout + p.stderr
+    v = parse(p.stdout)
+    s = np.loadtxt(tmp_path / "demag_in.txt").T.astype(np.float32).astype(np.float64)
+    H = np.loadtxt(tmp_path / "demag_field.txt").T
+    L = orc.lattice(4, 3, 3, cell="bcc")
+    want = orc.demag_field(L, np.ascontiguousarray(s))
+    assert np.max(np.abs(H - want)) < 1e-12 * max(1.0, np.max(np.abs(want)))
+    assert np.max(np.abs(want)) > 1e-3
+    term = -0.5 * orc.MU0_REF * np.sum(s * want)
+    assert abs(v["demag_energy_term"][0] - term) < 1e-6 * abs(term) + 1e-12
+    assert v["demag_array_vs_field"][0] < 1e-13
+    # local energies with the field argument H = (2e5, -1e5, 3e5) differ by
+    # -mu0 S.H of atom 5 (1-based); FP32 local field: 1e-5 of its scale
+    sp = np.array([0.0, 0.6, -0.8])
+    hb = np.array([2e5, -1e5, 3e5])
+    d_old = v["demag_local"][0] - v["nodemag_local"][0]
+    d_new = v["demag_local"][1] - v["nodemag_local"][1]
+    assert abs(d_old - (-orc.MU0_REF * s[:, 4] @ hb)) < 1e-5 * 10
+    assert abs(d_new - (-orc.MU0_REF * sp @ hb)) < 1e-5 * 10
+    assert abs(d_new - d_old) > 0.1
+    assert v["demag_after_sweeps"][0] < 1e-13
+    assert v["demag_norm_dev"][0] < 1e-6
+
+
+@pytest.mark.gpu
+@needs_fortran
 def test_pt_driver_reference_cli_and_csv(tmp_path):
     out = tmp_path / "run"
     env = {"ISING_PT_ITERATIONS": "4", "ISING_PT_SWEEPS_PER_SWAP": "10", "ISING_PT_RELAX": "20"}
