@@ -16,7 +16,7 @@ import pytest
 
 from gpu_util import engine, olat, random_unit
 from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
-from ising_amd.analysis import mean_error, tau_int
+from ising_amd.analysis import mean_error, specific_heat, specific_heat_error, tau_int
 from ising_amd.learned import initial_params
 from oracle import pyoracle as orc
 
@@ -143,9 +143,10 @@ def test_learned_sampler_exact_on_paramagnet():
 
 
 def test_learned_matches_fixed_observables():
-    """Interacting 12^3 lattice: <E> and <|M|> of the learned sampler (a random
-    network whose radius varies site by site around 0.4) within 3 sigma of the
-    fixed sampler's; both start from the same equilibrated configuration."""
+    """Interacting 12^3 lattice: <E>, <|M|>, C_v and the middle layer's <Q> of
+    the learned sampler (a random network whose radius varies site by site
+    around 0.4) within 3 sigma of the fixed sampler's; both start from the
+    same equilibrated configuration."""
     lat = Lattice(12, 12, 12)
     betas = [2.0, 0.9]
     rng = np.random.default_rng(6)
@@ -163,7 +164,8 @@ def test_learned_matches_fixed_observables():
                 e.set_spins(r, state[r])
             e.mlp_enable(learned)
             e.sweep(100)
-            res[learned] = e.pt_run(n_sweeps=4000, swap_every=0, measure_every=2)
+            res[learned] = e.pt_run(n_sweeps=4000, swap_every=0, measure_every=2, winding_every=2,
+                                    winding_layer=6)
     for k in range(2):
         a = mean_error(res[False]["energy"][:, k])
         b = mean_error(res[True]["energy"][:, k])
@@ -171,6 +173,13 @@ def test_learned_matches_fixed_observables():
         ma = mean_error(np.linalg.norm(res[False]["magnetisation"][:, k], axis=1))
         mb = mean_error(np.linalg.norm(res[True]["magnetisation"][:, k], axis=1))
         assert abs(ma[0] - mb[0]) < 3 * np.hypot(ma[1], mb[1]) + 1e-9, (k, ma, mb)
+        qa = mean_error(res[False]["winding"][:, k, 6].astype(float))
+        qb = mean_error(res[True]["winding"][:, k, 6].astype(float))
+        assert abs(qa[0] - qb[0]) < 3 * np.hypot(qa[1], qb[1]) + 1e-9, (k, qa, qb)
+        ea, eb = res[False]["energy"][:, k], res[True]["energy"][:, k]
+        ca, cea = specific_heat(ea, betas[k], lat.sites), specific_heat_error(ea, betas[k], lat.sites)
+        cb, ceb = specific_heat(eb, betas[k], lat.sites), specific_heat_error(eb, betas[k], lat.sites)
+        assert abs(ca - cb) < 3 * np.hypot(cea, ceb), (k, ca, cea, cb, ceb)
 
 
 def test_training_improves_and_decorrelates_faster():
