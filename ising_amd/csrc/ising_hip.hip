@@ -442,14 +442,77 @@ __device__ __forceinline__ float pair_energy(const Lad& L, const f3& fs, const f
            - L.zsf * L.Bz * (fs.z + c0.z) - L.Kz * (fs.z * fs.z + c0.z * c0.z);
 }
 
+// The same energy in FP64 from FP32 spins (the exact energy of the swap
+// exponent): the site's bonds -J S.sum_j T_j + sum_a d_a.(S x A_a), with
+// sum and the signed sums A_a of the neighbour spins formed in FP64, plus the
+// on-site terms of the site and of its colour-0 partner c0. d_a.(S x A_a)
+// keeps only the nonzero components of d_a (one for bulk DMI, one per
+// in-plane axis for the reference's interfacial form).
+__device__ __forceinline__ double pair_energy_exact(const Lad& L, const f3& s, const double (&sum)[3],
+                                                    const double (&A)[3][3], const f3& c0)
+{
+    const double sx = s.x, sy = s.y, sz = s.z;
+    double e = -L.Jd * (sx * sum[0] + sy * sum[1] + sz * sum[2]);
+    if (L.dmi == ISING_DMI_BULK) {
+        e += L.dd[0] * (sy * A[0][2] - sz * A[0][1]) + L.dd[4] * (sz * A[1][0] - sx * A[1][2])
+             + L.dd[8] * (sx * A[2][1] - sy * A[2][0]);
+    } else if (L.dmi == ISING_DMI_INTERFACIAL_REF) {
+        e += L.dd[1] * (sz * A[0][0] - sx * A[0][2]) + L.dd[3] * (sy * A[1][2] - sz * A[1][1]);
+    }
+    const double cz = c0.z;
+    return e - L.zs * L.Bzd * (sz + cz) - L.Kzd * (sz * sz + cz * cz);
+}
+
+// FP64 neighbour sums of a simple-cubic site (v: +x, -x, +y, -y, +z, -z,
+// zero across open faces): sum_j T_j and A_a = T_{+a} - T_{-a}.
+__device__ __forceinline__ void sc_sums64(const f3 (&v)[6], double (&sum)[3], double (&A)[3][3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const f3 p = v[2 * a], m = v[2 * a + 1];
+        A[a][0] = (double)p.x - (double)m.x;
+        A[a][1] = (double)p.y - (double)m.y;
+        A[a][2] = (double)p.z - (double)m.z;
+    }
+    sum[0] = (((double)v[0].x + (double)v[1].x) + ((double)v[2].x + (double)v[3].x)) + ((double)v[4].x + (double)v[5].x);
+    sum[1] = (((double)v[0].y + (double)v[1].y) + ((double)v[2].y + (double)v[3].y)) + ((double)v[4].y + (double)v[5].y);
+    sum[2] = (((double)v[0].z + (double)v[1].z) + ((double)v[2].z + (double)v[3].z)) + ((double)v[4].z + (double)v[5].z);
+}
+
+// FP64 neighbour sums of a bcc site: the eight bonds of bcc_sums, reloaded.
+__device__ __forceinline__ void bcc_sums64(const Rep& R, const Planes& oth, uint32_t q, uint32_t ix, uint32_t iy,
+                                           uint32_t iz, int colour, double (&sum)[3], double (&A)[3][3])
+{
+    const BccSteps st = bcc_steps(R, ix, iy, iz, colour);
+    const double sg = colour == 0 ? 1.0 : -1.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { sum[c] = 0.0; A[0][c] = 0.0; A[1][c] = 0.0; A[2][c] = 0.0; }
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        if (!bcc_ok(st, b)) continue;
+        const f3 t = ld3(oth, bcc_index(st, q, b));
+        const double v[3] = { t.x, t.y, t.z };
+        const double gx = ((b >> 2) & 1) ? -sg : sg, gy = ((b >> 1) & 1) ? -sg : sg, gz = (b & 1) ? -sg : sg;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            sum[c] += v[c];
+            A[0][c] += gx * v[c];
+            A[1][c] += gy * v[c];
+            A[2][c] += gz * v[c];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Half-sweep: one Metropolis proposal for every site of one colour of every
 // replica, one thread per site, in place in the current buffer (accepted
 // moves are stored).
 //
 // kMeasure (colour-1 launches only): per-site pair_energy and spins in FP32,
-// summed in FP64 per wavefront into wave_part[wave][4] = (E, Mx, My, Mz).
-template <bool kMeasure, bool kLearned>
+// summed in FP64 per wavefront into wave_part[wave][4] = (E, Mx, My, Mz);
+// with kExact the per-site energies are FP64 (pair_energy_exact), the exact
+// energies the replica exchange takes.
+template <bool kMeasure, bool kLearned, bool kExact = false>
 __global__ __launch_bounds__(kBlock) void k_half_sweep(
     SpinBuf sb, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ block_rep,
@@ -504,11 +567,19 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
         }
     }
     if constexpr (kMeasure) {
-        float me = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+        double me = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
         if (active) {
             const f3 c0 = ld3(planes(spins, R, 0), q);
-            me = pair_energy(L, fs, hf, c0);
-            mx = fs.x + c0.x; my = fs.y + c0.y; mz = fs.z + c0.z;
+            if constexpr (kExact) {
+                double sum[3], A[3][3];
+                if (R.cell == ISING_CELL_BCC) bcc_sums64(R, oth, q, h, iy, iz, colour, sum, A);
+                else sc_sums64(nbr, sum, A);
+                me = pair_energy_exact(L, fs, sum, A, c0);
+                mx = (double)fs.x + (double)c0.x; my = (double)fs.y + (double)c0.y; mz = (double)fs.z + (double)c0.z;
+            } else {
+                me = pair_energy(L, fs, hf, c0);
+                mx = fs.x + c0.x; my = fs.y + c0.y; mz = fs.z + c0.z;
+            }
         }
         double v[4] = { me, mx, my, mz };
 #pragma unroll
@@ -810,6 +881,34 @@ __global__ __launch_bounds__(kBlock) void k_acc_fold(
 }
 
 // ---------------------------------------------------------------------------
+// One record of a measurement series, written by the kernel that finishes
+// the value: entry *ctr + local of an array of cap entries indexed by
+// temperature position. ctr advances once per period (k_advance), local
+// counts this series' records earlier in the period, so one captured period
+// replays correctly. a: energies [cap][R] or FP64 Q [cap][R][zmax];
+// b: magnetisation [cap][R][3]; qi: integer Q [cap][R][zmax].
+struct SeriesRec {
+    const uint32_t* ctr;
+    uint32_t local, cap;
+    double* a;
+    double* b;
+    long long* qi;
+};
+
+// Temperature position of `slot` (the inverse of slot_of_temp), found by the
+// whole workgroup; -1 if absent. Every thread must call it.
+__device__ __forceinline__ int temp_of_slot(const int32_t* __restrict__ slot_of_temp, int R, int slot)
+{
+    __shared__ int tpos;
+    if (threadIdx.x == 0) tpos = -1;
+    __syncthreads();
+    for (int t = threadIdx.x; t < R; t += blockDim.x)
+        if (slot_of_temp[t] == slot) tpos = t;
+    __syncthreads();
+    return tpos;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 reductions.
 
 
@@ -904,9 +1003,15 @@ __global__ __launch_bounds__(kBlock) void k_measure(
                 if (!nb.ok[k]) continue;
                 const f3 tf = ld3(oth, nb.i[k]);
                 const double tx = tf.x, ty = tf.y, tz = tf.z;
-                const double cx = sy * tz - sz * ty, cy = sz * tx - sx * tz, cz = sx * ty - sy * tx;
-                e += -L.Jd * (sx * tx + sy * ty + sz * tz)
-                     + (L.dd[3 * a] * cx + L.dd[3 * a + 1] * cy + L.dd[3 * a + 2] * cz);
+                // d_a.(S x T): one component of S x T for bulk DMI (d_a = D e_a),
+                // one for the reference's interfacial form (d_x = -D e_y,
+                // d_y = D e_x, d_z = 0); the other terms of the dot are zeros
+                double dm = 0.0;
+                if (L.dmi == ISING_DMI_BULK)
+                    dm = L.dd[4 * a] * (a == 0 ? sy * tz - sz * ty : a == 1 ? sz * tx - sx * tz : sx * ty - sy * tx);
+                else if (L.dmi == ISING_DMI_INTERFACIAL_REF)
+                    dm = a == 0 ? L.dd[1] * (sz * tx - sx * tz) : a == 1 ? L.dd[3] * (sy * tz - sz * ty) : 0.0;
+                e += -L.Jd * (sx * tx + sy * ty + sz * tz) + dm;
             }
             e -= L.zs * L.Bzd * sz + L.Kzd * sz * sz;
             if (L.demag) {
@@ -929,7 +1034,8 @@ __global__ __launch_bounds__(kBlock) void k_measure(
 // wpb_f wavefronts per workgroup.
 __global__ __launch_bounds__(kBlock) void k_wave_finalize(
     const Rep* __restrict__ reps, const double* __restrict__ wave_part, int wpb_f,
-    double* __restrict__ energy, double* __restrict__ mag)
+    double* __restrict__ energy, double* __restrict__ mag, const int32_t* __restrict__ slot_of_temp, int nrep,
+    SeriesRec rec)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
     const int slot = blockIdx.x;
@@ -942,10 +1048,19 @@ __global__ __launch_bounds__(kBlock) void k_wave_finalize(
         v[0] += base[4 * j]; v[1] += base[4 * j + 1]; v[2] += base[4 * j + 2]; v[3] += base[4 * j + 3];
     }
     block_sum4(v, lds);
+    const double inv = 1.0 / (double)R.n;
     if (threadIdx.x == 0) {
         energy[slot] = v[0];
-        const double inv = 1.0 / (double)R.n;
         mag[3 * slot] = v[1] * inv; mag[3 * slot + 1] = v[2] * inv; mag[3 * slot + 2] = v[3] * inv;
+    }
+    if (rec.a) {       // uniform
+        const int t = temp_of_slot(slot_of_temp, nrep, slot);
+        const uint32_t m = *rec.ctr + rec.local;
+        if (threadIdx.x == 0 && m < rec.cap && t >= 0) {
+            const int64_t e = (int64_t)m * nrep + t;
+            rec.a[e] = v[0];
+            rec.b[3 * e] = v[1] * inv; rec.b[3 * e + 1] = v[2] * inv; rec.b[3 * e + 2] = v[3] * inv;
+        }
     }
 }
 
@@ -1081,24 +1196,33 @@ __global__ __launch_bounds__(kBlock) void k_winding_density(
 // one-layer mode (layer >= 0) the other layers read 0.
 __global__ __launch_bounds__(kBlock) void k_winding_finalize(
     const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax, int layer,
-    double* __restrict__ qraw, long long* __restrict__ qint)
+    double* __restrict__ qraw, long long* __restrict__ qint, const int32_t* __restrict__ slot_of_temp, int nrep,
+    SeriesRec rec)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
     const int slot = blockIdx.x / zmax;
     const int z = blockIdx.x - slot * zmax;
     const Rep R = reps[slot];
-    if (z >= R.lz || (layer >= 0 && z != layer)) {
-        if (threadIdx.x == 0) { qraw[blockIdx.x] = 0.0; qint[blockIdx.x] = 0; }
-        return;
+    double q = 0.0;
+    if (z < R.lz && (layer < 0 || z == layer)) {      // uniform
+        double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+        const int base = layer < 0 ? R.q_first_block + z * R.q_blocks_layer : R.l_first_block;
+        for (int j = threadIdx.x; j < R.q_blocks_layer; j += kBlock) v[0] += qpartial[base + j];
+        block_sum4(v, lds);
+        q = v[0] / (4.0 * 3.14159265358979323846);
     }
-    double v[4] = { 0.0, 0.0, 0.0, 0.0 };
-    const int base = layer < 0 ? R.q_first_block + z * R.q_blocks_layer : R.l_first_block;
-    for (int j = threadIdx.x; j < R.q_blocks_layer; j += kBlock) v[0] += qpartial[base + j];
-    block_sum4(v, lds);
     if (threadIdx.x == 0) {
-        const double q = v[0] / (4.0 * 3.14159265358979323846);
         qraw[blockIdx.x] = q;
         qint[blockIdx.x] = llrint(q);
+    }
+    if (rec.a) {       // uniform
+        const int t = temp_of_slot(slot_of_temp, nrep, slot);
+        const uint32_t m = *rec.ctr + rec.local;
+        if (threadIdx.x == 0 && m < rec.cap && t >= 0) {
+            const int64_t e = ((int64_t)m * nrep + t) * zmax + z;
+            rec.a[e] = q;
+            rec.qi[e] = llrint(q);
+        }
     }
 }
 
@@ -1112,7 +1236,8 @@ __global__ __launch_bounds__(kBlock) void k_swap(
     const int32_t* __restrict__ lad_of_pos, const int32_t* __restrict__ lad_start,
     const int32_t* __restrict__ lad_n, int R, uint32_t* __restrict__ round_ctr,
     uint32_t seed_lo, uint32_t seed_hi,
-    unsigned long long* __restrict__ attempts, unsigned long long* __restrict__ accepts)
+    unsigned long long* __restrict__ attempts, unsigned long long* __restrict__ accepts,
+    const double* __restrict__ mag, SeriesRec rec)
 {
     const uint32_t round = *round_ctr;
     __syncthreads();
@@ -1136,42 +1261,16 @@ __global__ __launch_bounds__(kBlock) void k_swap(
     }
     __syncthreads();
     if (threadIdx.x == 0) *round_ctr = round + 1u;
-}
-
-// Series records (single workgroup): value at temperature position t.
-__global__ __launch_bounds__(kBlock) void k_record(
-    const double* __restrict__ energy, const double* __restrict__ mag, const int32_t* __restrict__ slot_of_temp,
-    int R, uint32_t* __restrict__ meas_ctr, uint32_t cap, double* __restrict__ se, double* __restrict__ sm)
-{
-    const uint32_t m = *meas_ctr;
-    __syncthreads();
-    if (m < cap) {
+    // E and M after the round, at each temperature (configurations stay put)
+    const uint32_t m = rec.a ? *rec.ctr + rec.local : 0u;
+    if (rec.a && m < rec.cap) {
         for (int t = threadIdx.x; t < R; t += blockDim.x) {
             const int s = slot_of_temp[t];
-            se[(int64_t)m * R + t] = energy[s];
-            for (int c = 0; c < 3; ++c) sm[((int64_t)m * R + t) * 3 + c] = mag[3 * s + c];
+            const int64_t e = (int64_t)m * R + t;
+            rec.a[e] = energy[s];
+            rec.b[3 * e] = mag[3 * s]; rec.b[3 * e + 1] = mag[3 * s + 1]; rec.b[3 * e + 2] = mag[3 * s + 2];
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) *meas_ctr = m + 1u;
-}
-
-__global__ __launch_bounds__(kBlock) void k_record_q(
-    const double* __restrict__ qraw, const long long* __restrict__ qint, const int32_t* __restrict__ slot_of_temp,
-    int R, int zmax, uint32_t* __restrict__ q_ctr, uint32_t cap, double* __restrict__ sqr, long long* __restrict__ sqi)
-{
-    const uint32_t m = *q_ctr;
-    __syncthreads();
-    if (m < cap) {
-        for (int e = threadIdx.x; e < R * zmax; e += blockDim.x) {
-            const int t = e / zmax, z = e - t * zmax;
-            const int s = slot_of_temp[t];
-            sqr[(int64_t)m * R * zmax + e] = qraw[s * zmax + z];
-            sqi[(int64_t)m * R * zmax + e] = qint[s * zmax + z];
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) *q_ctr = m + 1u;
 }
 
 // Plain copy (16-byte vectors, grid-stride): the bandwidth and traffic probe.
@@ -1181,10 +1280,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(const float4* __restrict_
         dst[i] = src[i];
 }
 
-// Advances the sweep counter by `sweeps` and the fused-sweep counter by `fused`.
-__global__ void k_advance(uint32_t* __restrict__ ctr, uint32_t sweeps, uint32_t fused)
+// Advances the sweep counter by `sweeps`, the E/M and Q record counters by
+// the records of the period and the fused-sweep counter by `fused`.
+__global__ void k_advance(uint32_t* __restrict__ ctr, uint32_t sweeps, uint32_t meas, uint32_t q, uint32_t fused)
 {
-    if (threadIdx.x == 0) { ctr[0] += sweeps; ctr[4] += fused; }
+    if (threadIdx.x == 0) { ctr[0] += sweeps; ctr[1] += meas; ctr[2] += q; ctr[4] += fused; }
 }
 __global__ void k_set_u32(uint32_t* __restrict__ p, int n, uint32_t v)
 {
@@ -1795,10 +1895,10 @@ int fused_lc(const ising_ctx* c) { return (int)((c->tune >> 8) & 0xFFu); }
 // in-place half sweeps carry.
 bool fused_active(const ising_ctx* c) { return fused_lc(c) > 0 && !c->any_demag; }
 
-template <bool kMeasure, bool kLearned>
+template <bool kMeasure, bool kLearned, bool kExact = false>
 void launch_half_kernel(ising_ctx* c, uint32_t off, int colour)
 {
-    hipLaunchKernelGGL((k_half_sweep<kMeasure, kLearned>), dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
+    hipLaunchKernelGGL((k_half_sweep<kMeasure, kLearned, kExact>), dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
                        spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
                        (const uint32_t*)c->d_ctr, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
                        kMeasure ? c->d_wave_part : (double*)nullptr, c->tune & 0xFFu, (const MlpDev*)c->d_mlp,
@@ -1819,7 +1919,12 @@ void launch_fused_kernel(ising_ctx* c, uint32_t off)
 // colour 0 then colour 1 of every replica, from the buffer of the current
 // parity into the other one; with `measure`, E and M of the new state are
 // recorded by the same launches.
-int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false)
+constexpr SeriesRec kNoRecord{ nullptr, 0u, 0u, nullptr, nullptr, nullptr };
+
+// exact (half sweeps only): the colour-1 launch records FP64 per-site
+// energies (pair_energy_exact) instead of FP32 ones.
+int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesRec& rec = kNoRecord,
+                 bool exact = false)
 {
     const bool learned = c->any_learned;
     if (fused_active(c)) {
@@ -1833,7 +1938,8 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false)
         LAUNCH_CHECK(c);
         if (measure) {
             hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
-                               (const double*)c->d_wave_part_f, c->fused_wpb, c->d_energy, c->d_mag);
+                               (const double*)c->d_wave_part_f, c->fused_wpb, c->d_energy, c->d_mag,
+                               (const int32_t*)c->d_slot_of_temp, c->R, rec);
             LAUNCH_CHECK(c);
         }
         return ISING_OK;
@@ -1842,11 +1948,17 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false)
     else launch_half_kernel<false, false>(c, off, 0);
     LAUNCH_CHECK(c);
     if (measure) {
-        if (learned) launch_half_kernel<true, true>(c, off, 1);
-        else launch_half_kernel<true, false>(c, off, 1);
+        if (exact) {
+            if (learned) launch_half_kernel<true, true, true>(c, off, 1);
+            else launch_half_kernel<true, false, true>(c, off, 1);
+        } else {
+            if (learned) launch_half_kernel<true, true>(c, off, 1);
+            else launch_half_kernel<true, false>(c, off, 1);
+        }
         LAUNCH_CHECK(c);
         hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
-                           (const double*)c->d_wave_part, 0, c->d_energy, c->d_mag);
+                           (const double*)c->d_wave_part, 0, c->d_energy, c->d_mag,
+                           (const int32_t*)c->d_slot_of_temp, c->R, rec);
     } else {
         if (learned) launch_half_kernel<false, true>(c, off, 1);
         else launch_half_kernel<false, false>(c, off, 1);
@@ -1952,7 +2064,7 @@ int launch_measure_all(ising_ctx* c, int done = 0)
     return ISING_OK;
 }
 
-int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0)
+int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0, const SeriesRec& rec = kNoRecord)
 {
     const uint32_t off = buf_off(c, done);
     if (layer < 0) {
@@ -1964,26 +2076,28 @@ int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0)
     }
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_winding_finalize, dim3(c->R * c->zmax), dim3(kBlock), 0, c->stream,
-                       c->d_reps, c->d_qpartial, c->zmax, layer, c->d_qraw, c->d_qint);
+                       c->d_reps, c->d_qpartial, c->zmax, layer, c->d_qraw, c->d_qint,
+                       (const int32_t*)c->d_slot_of_temp, c->R, rec);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
 
-int launch_swap(ising_ctx* c)
+int launch_swap(ising_ctx* c, const SeriesRec& rec = kNoRecord)
 {
     hipLaunchKernelGGL(k_swap, dim3(1), dim3(kBlock), 0, c->stream,
                        (const double*)c->d_energy, (const double*)c->d_beta_temp, c->d_slot_of_temp, c->d_beta_slot,
                        c->d_lad_of_pos, c->d_lad_start, c->d_lad_n, c->R, c->d_ctr + 3,
-                       (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_swap_att, c->d_swap_acc);
+                       (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_swap_att, c->d_swap_acc,
+                       (const double*)c->d_mag, rec);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
 
 // Device counters after n sweeps (graph-capturable); advance_host follows
 // once they are enqueued.
-int launch_advance(ising_ctx* c, uint32_t n)
+int launch_advance(ising_ctx* c, uint32_t n, uint32_t meas = 0u, uint32_t q = 0u)
 {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_ctr, n, fused_active(c) ? n : 0u);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_ctr, n, meas, q, fused_active(c) ? n : 0u);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
@@ -2006,33 +2120,33 @@ struct SeriesDev {
 int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const SeriesDev& sd)
 {
     int rc;
+    uint32_t nm = 0, nq = 0;      // records of each series so far in this period
     for (int k = 0; k < count; ++k) {
         const int done = k + 1;
         const bool do_swap = s.swap_every > 0 && done % s.swap_every == 0;
         const bool do_meas = s.measure_every > 0 && done % s.measure_every == 0;
         const bool do_q = s.winding_every > 0 && done % s.winding_every == 0;
-        // Measurements ride on the sweep launches, except before a swap, whose
-        // exponent takes the per-site FP64 energies of the measure kernel.
-        if ((rc = launch_sweep(c, (uint32_t)k, do_meas && !do_swap))) return rc;
-        if (do_swap)
+        const SeriesRec rm = (do_meas && sd.e) ? SeriesRec{ c->d_ctr + 1, nm, sd.n_meas, sd.e, sd.m, nullptr } : kNoRecord;
+        // Measurements ride on the sweep launches. Before a swap, whose
+        // exponent takes FP64 per-site energies, the half sweep's colour-1
+        // launch computes them (the fused sweep leaves them to the measure
+        // kernel); the swap kernel then records E and M at the new
+        // temperatures.
+        const bool exact = do_swap && !fused_active(c);
+        if ((rc = launch_sweep(c, (uint32_t)k, (do_meas && !do_swap) || exact, do_swap ? kNoRecord : rm, exact)))
+            return rc;
+        if (do_swap && !exact)
             if ((rc = launch_measure_all(c, done))) return rc;
         if (do_swap)
-            if ((rc = launch_swap(c))) return rc;
-        if (do_meas && sd.e) {
-            hipLaunchKernelGGL(k_record, dim3(1), dim3(kBlock), 0, c->stream,
-                               (const double*)c->d_energy, (const double*)c->d_mag, (const int32_t*)c->d_slot_of_temp,
-                               c->R, c->d_ctr + 1, sd.n_meas, sd.e, sd.m);
-            LAUNCH_CHECK(c);
-        }
+            if ((rc = launch_swap(c, rm))) return rc;
         if (do_q && sd.qr) {
-            if ((rc = launch_winding_all(c, s.winding_layer - 1, done))) return rc;
-            hipLaunchKernelGGL(k_record_q, dim3(1), dim3(kBlock), 0, c->stream,
-                               (const double*)c->d_qraw, (const long long*)c->d_qint, (const int32_t*)c->d_slot_of_temp,
-                               c->R, c->zmax, c->d_ctr + 2, sd.n_q, sd.qr, sd.qi);
-            LAUNCH_CHECK(c);
+            const SeriesRec rq{ c->d_ctr + 2, nq, sd.n_q, sd.qr, nullptr, sd.qi };
+            if ((rc = launch_winding_all(c, s.winding_layer - 1, done, rq))) return rc;
         }
+        nm += do_meas ? 1u : 0u;
+        nq += do_q ? 1u : 0u;
     }
-    return launch_advance(c, (uint32_t)count);
+    return launch_advance(c, (uint32_t)count, nm, nq);
 }
 
 // Fused-sweep resources: available when every replica's z plane of one

@@ -347,6 +347,29 @@ def test_fused_measurement_matches_exact(lname, hname):
             assert np.allclose(o["magnetisation"][-1], mag, atol=1e-6)
 
 
+SWAP_LATS = dict(LATS, bcc_periodic_4x3x3=Lattice(4, 3, 3, cell="bcc"),
+                 bcc_open_z_3x4x3=Lattice(3, 4, 3, (True, True, False), cell="bcc"))
+
+
+@pytest.mark.parametrize("hname", sorted(HAMS))
+@pytest.mark.parametrize("lname", sorted(SWAP_LATS))
+def test_swap_sweep_energies_are_exact(lname, hname):
+    """The sweep before a swap records FP64 per-site energies from its
+    colour-1 launch (the swap exponent's energies): they equal the FP64
+    measure kernel's on the same configuration to 1e-12 relative, and the
+    magnetisation to 1e-12."""
+    lat, ham = SWAP_LATS[lname], HAMS[hname]
+    with engine(lat, ham, [1.5, 0.6, 0.3]) as e:
+        e.init_random()
+        for _ in range(3):
+            o = e.pt_run(n_sweeps=2, swap_every=2, measure_every=2, winding_every=0, use_graph=False)
+            sot = e.temperature_map()
+            exact = e.energy()[sot]
+            mag = e.magnetisation()[sot]
+            assert np.all(np.abs(o["energy"][-1] - exact) <= 1e-12 * np.maximum(np.abs(exact), lat.sites))
+            assert np.allclose(o["magnetisation"][-1], mag, rtol=0, atol=1e-12)
+
+
 def test_checkpoint_restore_replays_exactly():
     """ising_save_state / ising_restore_state: a restored run repeats the
     same random numbers, swaps and records bit for bit."""
