@@ -17,8 +17,9 @@ r = 0.5, and learned cone.
 value  = site updates/s of the fixed-cone run (every replica's sites x sweeps
 / wall time); also reported per sampler: tau_int-corrected independent
 samples/s of E and of Q (middle layer) and their learned/fixed ratio, the
-half-sweep kernel's HBM roofline fraction from hipEvent timing, and the FP64
-CPU oracle's rate on this host's cores.
+half-sweep kernel's HBM roofline fraction from hipEvent pairs around its
+launches inside the same schedule (ising_time_pt_sweeps, 200 sweeps after
+each timed run), and the FP64 CPU oracle's rate on this host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -53,6 +54,7 @@ MLP_FLOPS_PER_UPDATE = 2 * (21 * 32 + 32 * 32 + 32)
 CONE = 0.5
 SEED = 20261203
 BYTES_PER_UPDATE = 24  # FP32 (x,y,z) read once + written once per site update
+LAUNCH_TIMING_SWEEPS = 200   # sweeps of the schedule whose plain half-sweep launches are timed
 HBM_PEAK_GBS = 8000.0
 
 
@@ -74,6 +76,11 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
     dt = time.perf_counter() - t0
     return {"value": L ** 3 * n / dt, "unit": "site updates/s", "cores": cores, "kind": "port",
             "sample": f"FP64 C/OpenMP oracle, 1 replica 64^3, T=0.5, r=0.5, {n} sweeps in {dt:.1f} s"}
+
+
+def progress(msg: str) -> None:
+    """One line per finished stage on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def counters_from_profiles() -> dict:
@@ -136,10 +143,13 @@ def main() -> int:
     # 2000 plain sweeps that equilibrate and collect training records; Adam
     # training of the cone-width network on them.
     tune_flags, tune_ms = eng.autotune(10)
+    progress(f"autotune: flags {tune_flags}")
     t0 = time.perf_counter()
     n_records = generate_data(eng, sweeps=TRAIN_SWEEPS, collect_every=COLLECT_EVERY, per_replica=RECORDS_PER_REPLICA,
                               swap_every=SWAP_EVERY, use_graph=not args.eager)
+    progress(f"training data: {n_records} records")
     loss = train(eng, steps=TRAIN_STEPS, batch=TRAIN_BATCH, lr=TRAIN_LR)
+    progress(f"training: loss {float(loss[:10].mean()):.4f} -> {float(loss[-10:].mean()):.4f}")
     # Further equilibration (the trained sampler relaxes the cold replicas
     # fastest), then one device checkpoint that both timed runs start from
     # with the same random-number counters.
@@ -147,6 +157,7 @@ def main() -> int:
     eng.pt_run(EQUIL_SWEEPS, SWAP_EVERY, 0, 0, use_graph=not args.eager)
     eng.save_state()
     prep_s = time.perf_counter() - t0
+    progress("equilibrated and checkpointed")
 
     n_sweeps = args.steps * SWAP_EVERY
 
@@ -167,7 +178,9 @@ def main() -> int:
         q_series = out["winding_raw"][:, :, L // 2]               # middle layer, [n_q, R]
         n_e, tau_e = independent_samples(e_series)
         n_q, tau_q = independent_samples(q_series)
-        ms_launch = eng.time_sweeps(50)
+        # the plain half-sweep kernel timed inside this schedule (hipEvents)
+        ms_launch = eng.time_pt_sweeps(LAUNCH_TIMING_SWEEPS, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY,
+                                       winding_layer=L // 2)
         return {"wall": wall, "updates_per_s": n_sites * n_sweeps / wall, "ms_per_launch": ms_launch,
                 "samples_per_s_E": n_e / wall, "samples_per_s_Q": n_q / wall,
                 "tau_int_E_sweeps": (tau_e * MEASURE_EVERY).tolist(), "tau_int_Q_sweeps": (tau_q * WINDING_EVERY).tolist(),
@@ -175,7 +188,9 @@ def main() -> int:
                 "mean_absM": np.linalg.norm(out["magnetisation"], axis=2).mean(axis=0).tolist()}
 
     fixed = timed_run(False)
+    progress(f"fixed: {fixed['wall'] * 1e3 / args.steps:.3f} ms/step")
     learned = timed_run(True)
+    progress(f"learned: {learned['wall'] * 1e3 / args.steps:.3f} ms/step")
     att, acc = eng.swap_stats()
 
     bytes_launch = BYTES_PER_UPDATE * n_sites / 2
@@ -186,6 +201,8 @@ def main() -> int:
     mlp_tflops = MLP_FLOPS_PER_UPDATE * n_sites / 2 / (learned["ms_per_launch"] * 1e-3) / 1e12
 
     cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        progress(f"cpu baseline: {cpu['value']:.3g} updates/s on {cpu['cores']} threads")
 
     def summary(r: dict) -> dict:
         te, tq = np.array(r["tau_int_E_sweeps"]), np.array(r["tau_int_Q_sweeps"])

@@ -231,6 +231,14 @@ int  ising_accept_counts(ising_ctx* ctx, int64_t* accepted);
  * sweep, or one fused launch); *ms_per_launch = elapsed / (2*n_sweeps), the
  * time per half sweep (one update of every site of one colour). */
 int  ising_time_sweeps(ising_ctx* ctx, int n_sweeps, double* ms_per_launch);
+/* Runs n_sweeps sweeps of a PT schedule like ising_pt_run (swaps, E/M and
+ * Q measurements; records are computed, not stored), launching every period
+ * eagerly; each plain (non-measuring) sweep launch carries start/stop
+ * events in its dispatch packet (hipExtLaunchKernel), and *ms_per_launch =
+ * their mean device time per half sweep: the time of the dominant kernel
+ * inside the workload it belongs to (back-to-back launches alone run at a
+ * lower clock under the board's power limit). */
+int  ising_time_pt_sweeps(ising_ctx* ctx, const ising_schedule* sched, int n_sweeps, double* ms_per_launch);
 /* hipDeviceSynchronize on the context's device. */
 int  ising_synchronize(ising_ctx* ctx);
 /* Plain device-to-device copy kernel over `bytes` bytes (two scratch buffers

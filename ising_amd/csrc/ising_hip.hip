@@ -15,6 +15,7 @@
 // workgroup to (replica, chunk), so replicas of different sizes share one
 // launch.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 #include <string.h>
 #include <stdio.h>
@@ -555,7 +556,10 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
         hf = field_from(L, nbr);
         site = site_number(R, ix, iy, iz);
     }
-    if (L.demag) add_demag(L, R, demag, (uint32_t)colour * R.half + q, hf);   // uniform per workgroup
+    // uniform per workgroup; a context with the learned proposal has no
+    // ladder with the demagnetisation term (the API refuses the mix)
+    if constexpr (!kLearned)
+        if (L.demag) add_demag(L, R, demag, (uint32_t)colour * R.half + q, hf);
     bool accepted;
     const f3 fs = metropolis_site<kLearned>(L, beta, s, nbr, hf, site, sweep, seed_lo, seed_hi,
                                             (uint32_t)slot, mlp, active, accepted);
@@ -1895,9 +1899,20 @@ int fused_lc(const ising_ctx* c) { return (int)((c->tune >> 8) & 0xFFu); }
 // in-place half sweeps carry.
 bool fused_active(const ising_ctx* c) { return fused_lc(c) > 0 && !c->any_demag; }
 
+// t0/t1: start/stop events in the dispatch packet (timed launches only;
+// graph-captured launches take the plain path).
 template <bool kMeasure, bool kLearned, bool kExact = false>
-void launch_half_kernel(ising_ctx* c, uint32_t off, int colour)
+void launch_half_kernel(ising_ctx* c, uint32_t off, int colour, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr)
 {
+    if (t0) {
+        hipExtLaunchKernelGGL((k_half_sweep<kMeasure, kLearned, kExact>), dim3(c->n_blocks), dim3(kBlock), 0,
+                              c->stream, t0, t1, 0u, spin_buf(c), c->d_reps, c->d_lads, (const float*)c->d_beta_slot,
+                              (const int32_t*)c->d_block_rep, (const uint32_t*)c->d_ctr, off, colour,
+                              (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
+                              kMeasure ? c->d_wave_part : (double*)nullptr, c->tune & 0xFFu, (const MlpDev*)c->d_mlp,
+                              (const double*)c->d_demag);
+        return;
+    }
     hipLaunchKernelGGL((k_half_sweep<kMeasure, kLearned, kExact>), dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
                        spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
                        (const uint32_t*)c->d_ctr, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
@@ -1906,8 +1921,17 @@ void launch_half_kernel(ising_ctx* c, uint32_t off, int colour)
 }
 
 template <bool kMeasure, bool kLearned>
-void launch_fused_kernel(ising_ctx* c, uint32_t off)
+void launch_fused_kernel(ising_ctx* c, uint32_t off, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr)
 {
+    if (t0) {
+        hipExtLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned>), dim3(c->n_fblocks), dim3(c->fused_threads),
+                              (uint32_t)c->fused_lds, c->stream, t0, t1, 0u, spin_buf(c), c->d_reps, c->d_lads,
+                              (const float*)c->d_beta_slot, (const int32_t*)c->d_fblock_rep, (const uint32_t*)c->d_ctr,
+                              off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
+                              c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu,
+                              (const MlpDev*)c->d_mlp);
+        return;
+    }
     hipLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned>), dim3(c->n_fblocks), dim3(c->fused_threads), c->fused_lds,
                        c->stream, spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_fblock_rep,
                        (const uint32_t*)c->d_ctr, off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
@@ -1921,19 +1945,51 @@ void launch_fused_kernel(ising_ctx* c, uint32_t off)
 // recorded by the same launches.
 constexpr SeriesRec kNoRecord{ nullptr, 0u, 0u, nullptr, nullptr, nullptr };
 
+// Start/stop events carried by the dispatch packets of the plain
+// (non-measuring) sweep launches of an eagerly enqueued schedule
+// (hipExtLaunchKernel): the kernels' own execution time inside the workload
+// they are part of, without the barrier packets of separate event records
+// (ising_time_pt_sweeps).
+struct LaunchTimer {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    int launches_per_pair = 1;   // half sweeps covered by one timed launch (2 for a fused sweep)
+    ~LaunchTimer() { for (hipEvent_t e : ev) (void)hipEventDestroy(e); }
+    hipError_t next_pair(hipEvent_t& a, hipEvent_t& b)
+    {
+        while (ev.size() < used + 2) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ev.push_back(e);
+        }
+        a = ev[used]; b = ev[used + 1];
+        used += 2;
+        return hipSuccess;
+    }
+};
+
 // exact (half sweeps only): the colour-1 launch records FP64 per-site
 // energies (pair_energy_exact) instead of FP32 ones.
 int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesRec& rec = kNoRecord,
-                 bool exact = false)
+                 bool exact = false, LaunchTimer* timer = nullptr)
 {
     const bool learned = c->any_learned;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    auto pair = [&]() -> int {
+        if (timer && timer->next_pair(t0, t1) != hipSuccess) return set_err(c, ISING_ERR_HIP, "hipEventCreate");
+        return ISING_OK;
+    };
+    int rc;
     if (fused_active(c)) {
         if (measure) {
             if (learned) launch_fused_kernel<true, true>(c, off);
             else launch_fused_kernel<true, false>(c, off);
         } else {
-            if (learned) launch_fused_kernel<false, true>(c, off);
-            else launch_fused_kernel<false, false>(c, off);
+            if (timer) timer->launches_per_pair = 2;
+            if ((rc = pair())) return rc;
+            if (learned) launch_fused_kernel<false, true>(c, off, t0, t1);
+            else launch_fused_kernel<false, false>(c, off, t0, t1);
         }
         LAUNCH_CHECK(c);
         if (measure) {
@@ -1944,8 +2000,9 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
         }
         return ISING_OK;
     }
-    if (learned) launch_half_kernel<false, true>(c, off, 0);
-    else launch_half_kernel<false, false>(c, off, 0);
+    if ((rc = pair())) return rc;
+    if (learned) launch_half_kernel<false, true>(c, off, 0, t0, t1);
+    else launch_half_kernel<false, false>(c, off, 0, t0, t1);
     LAUNCH_CHECK(c);
     if (measure) {
         if (exact) {
@@ -1960,8 +2017,9 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
                            (const double*)c->d_wave_part, 0, c->d_energy, c->d_mag,
                            (const int32_t*)c->d_slot_of_temp, c->R, rec);
     } else {
-        if (learned) launch_half_kernel<false, true>(c, off, 1);
-        else launch_half_kernel<false, false>(c, off, 1);
+        if ((rc = pair())) return rc;
+        if (learned) launch_half_kernel<false, true>(c, off, 1, t0, t1);
+        else launch_half_kernel<false, false>(c, off, 1, t0, t1);
     }
     LAUNCH_CHECK(c);
     return ISING_OK;
@@ -2117,7 +2175,7 @@ struct SeriesDev {
 // sweep k+1 that the schedule names. Offsets are relative to the device sweep
 // counter, which is advanced by `count` at the end, so the same launch
 // sequence can be captured once and replayed.
-int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const SeriesDev& sd)
+int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const SeriesDev& sd, LaunchTimer* timer = nullptr)
 {
     int rc;
     uint32_t nm = 0, nq = 0;      // records of each series so far in this period
@@ -2133,14 +2191,14 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
         // kernel); the swap kernel then records E and M at the new
         // temperatures.
         const bool exact = do_swap && !fused_active(c);
-        if ((rc = launch_sweep(c, (uint32_t)k, (do_meas && !do_swap) || exact, do_swap ? kNoRecord : rm, exact)))
+        if ((rc = launch_sweep(c, (uint32_t)k, (do_meas && !do_swap) || exact, do_swap ? kNoRecord : rm, exact, timer)))
             return rc;
         if (do_swap && !exact)
             if ((rc = launch_measure_all(c, done))) return rc;
         if (do_swap)
             if ((rc = launch_swap(c, rm))) return rc;
-        if (do_q && sd.qr) {
-            const SeriesRec rq{ c->d_ctr + 2, nq, sd.n_q, sd.qr, nullptr, sd.qi };
+        if (do_q) {
+            const SeriesRec rq = sd.qr ? SeriesRec{ c->d_ctr + 2, nq, sd.n_q, sd.qr, nullptr, sd.qi } : kNoRecord;
             if ((rc = launch_winding_all(c, s.winding_layer - 1, done, rq))) return rc;
         }
         nm += do_meas ? 1u : 0u;
@@ -2450,9 +2508,9 @@ int ising_set_hamiltonian(ising_ctx* c, int ladder, const ising_hamiltonian* h)
     if (rc) return rc;
     if (h->demag) {
         if ((rc = check_demag_lattice(c, c->ladders[ladder].lat))) return rc;
-        if (c->learned[ladder])
-            return set_err(c, ISING_ERR_UNSUPPORTED, "ladder %d takes its radius from the network, whose input has "
-                           "no demagnetising field", ladder);
+        if (c->any_learned)
+            return set_err(c, ISING_ERR_UNSUPPORTED, "a ladder of this context takes its radius from the network "
+                           "(the learned sweep kernel carries no demagnetising field)");
     }
     HIP_TRY(c, hipSetDevice(c->device));
     if (h->demag && (rc = demag_setup(c))) return rc;
@@ -2889,6 +2947,44 @@ int ising_time_sweeps(ising_ctx* c, int n_sweeps, double* ms_per_launch)
     return ISING_OK;
 }
 
+int ising_time_pt_sweeps(ising_ctx* c, const ising_schedule* sched, int n_sweeps, double* ms_per_launch)
+{
+    if (!c || !sched || !ms_per_launch || n_sweeps < 1) return ISING_ERR_INVALID_ARG;
+    if (c->any_demag)
+        return set_err(c, ISING_ERR_UNSUPPORTED, "parallel tempering with the demagnetisation term");
+    const ising_schedule s = *sched;
+    if (s.swap_every < 0 || s.measure_every < 0 || s.winding_every < 0 || s.winding_layer < 0 || s.winding_layer > c->zmax)
+        return set_err(c, ISING_ERR_INVALID_ARG, "bad schedule");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int64_t period = 1;
+    for (int v : { s.swap_every, s.measure_every, s.winding_every })
+        if (v > 0) period = period / gcd64(period, v) * v;
+    LaunchTimer timer;
+    SeriesDev none;                      // records are computed, not stored
+    int rc = ISING_OK, done = 0;
+    // The record counters restart as in ising_pt_run (they only index series).
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, c->d_ctr + 1, 2, 0u);
+    LAUNCH_CHECK(c);
+    while (done < n_sweeps && !rc) {
+        const int cnt = (int)std::min<int64_t>(period, n_sweeps - done);
+        rc = enqueue_period(c, s, cnt, none, &timer);
+        done += cnt;
+    }
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    advance_host(c, (uint32_t)n_sweeps);
+    double total = 0.0;
+    for (size_t k = 0; k + 1 < timer.used; k += 2) {
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, timer.ev[k], timer.ev[k + 1]));
+        total += ms;
+    }
+    const size_t pairs = timer.used / 2;
+    if (pairs == 0) return set_err(c, ISING_ERR_STATE, "the schedule has no plain sweep launch");
+    *ms_per_launch = total / ((double)pairs * timer.launches_per_pair);
+    return ISING_OK;
+}
+
 int ising_synchronize(ising_ctx* c)
 {
     if (!c) return ISING_ERR_INVALID_ARG;
@@ -3050,8 +3146,9 @@ int ising_mlp_enable(ising_ctx* c, int ladder, int on)
     {
         if (on && (ladder < 0 || k == ladder) && c->ladders[k].lat.cell != ISING_CELL_SC)
             return set_err(c, ISING_ERR_UNSUPPORTED, "learned proposal: ladder %d is not simple cubic", k);
-        if (on && (ladder < 0 || k == ladder) && c->ladders[k].ham.demag)
-            return set_err(c, ISING_ERR_UNSUPPORTED, "learned proposal: ladder %d has the demagnetisation term", k);
+        if (on && c->ladders[k].ham.demag)
+            return set_err(c, ISING_ERR_UNSUPPORTED, "learned proposal: ladder %d of this context has the "
+                           "demagnetisation term", k);
     }
     HIP_TRY(c, hipSetDevice(c->device));
     for (int k = 0; k < (int)c->learned.size(); ++k)

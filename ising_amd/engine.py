@@ -336,6 +336,16 @@ class Engine:
         self._check(self._lib.ising_time_sweeps(self._h, int(n_sweeps), C.byref(ms)))
         return ms.value
 
+    def time_pt_sweeps(self, n_sweeps: int, swap_every: int = 10, measure_every: int = 2, winding_every: int = 0,
+                       winding_layer: int | None = None) -> float:
+        """Mean device time (ms) per half sweep of the plain sweep launches of
+        n_sweeps sweeps of a PT schedule, timed inside that schedule."""
+        ms = C.c_double()
+        sch = _capi.ising_schedule(int(n_sweeps), swap_every, measure_every, winding_every, 0,
+                                   0 if winding_layer is None else int(winding_layer) + 1)
+        self._check(self._lib.ising_time_pt_sweeps(self._h, C.byref(sch), int(n_sweeps), C.byref(ms)))
+        return ms.value
+
     def bandwidth_probe(self, nbytes: int = 256 << 20) -> float:
         """Device time (ms) of one plain copy of nbytes (read nbytes + write nbytes)."""
         ms = C.c_double()

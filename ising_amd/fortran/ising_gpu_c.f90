@@ -381,6 +381,14 @@ module ising_gpu_c
                         real(c_float), intent(out) :: d(*)
                         integer(c_int) :: rc
                 end function
+                function ising_time_pt_sweeps(ctx, sched, n_sweeps, ms) bind(C, name="ising_time_pt_sweeps") result(rc)
+                        import :: c_ptr, c_int, c_double, ising_schedule
+                        type(c_ptr), value :: ctx
+                        type(ising_schedule), intent(in) :: sched
+                        integer(c_int), value :: n_sweeps
+                        real(c_double), intent(out) :: ms
+                        integer(c_int) :: rc
+                end function
                 function ising_demag_field(ctx, replica, h, n) bind(C, name="ising_demag_field") result(rc)
                         import :: c_ptr, c_int, c_int64_t, c_double
                         type(c_ptr), value :: ctx

@@ -203,3 +203,13 @@ def test_refusals():
     with engine(Lattice(4, 4, 4, (True, True, False)), Hamiltonian(J=1.0), [1.0]) as e:
         with pytest.raises(IsingError):
             e.set_hamiltonian(0, Hamiltonian(J=1.0, demag=True))
+    # the learned sweep kernel carries no demagnetising field: no mix in one context
+    lads = [Ladder(Lattice(4, 4, 4), Hamiltonian(J=1.0), [1.0]), Ladder(Lattice(4, 4, 4), Hamiltonian(J=1.0), [2.0])]
+    with Engine(lads, seed=3) as e:
+        e.mlp_enable(True, ladder=0)
+        with pytest.raises(IsingError):
+            e.set_hamiltonian(1, Hamiltonian(J=1.0, demag=True))
+        e.mlp_enable(False)
+        e.set_hamiltonian(1, Hamiltonian(J=1.0, demag=True))
+        with pytest.raises(IsingError):
+            e.mlp_enable(True, ladder=0)

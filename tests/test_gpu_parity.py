@@ -370,6 +370,25 @@ def test_swap_sweep_energies_are_exact(lname, hname):
             assert np.allclose(o["magnetisation"][-1], mag, rtol=0, atol=1e-12)
 
 
+def test_time_pt_sweeps_advances_like_pt_run():
+    """The in-schedule kernel timing runs the same chain as ising_pt_run:
+    from one checkpoint both end in the same spins and temperature map."""
+    lat = Lattice(12, 12, 8)
+    betas = 1.0 / geometric_temperatures(0.4, 2.0, 4)
+    with engine(lat, HAMS["bulk_K"], betas) as e:
+        e.init_random()
+        e.pt_run(n_sweeps=20, swap_every=5, measure_every=0)
+        e.save_state()
+        e.pt_run(n_sweeps=40, swap_every=10, measure_every=2, winding_every=4, winding_layer=3)
+        a = [e.get_spins(r) for r in range(4)] + [e.temperature_map()]
+        e.restore_state()
+        ms = e.time_pt_sweeps(40, 10, 2, 4, winding_layer=3)
+        b = [e.get_spins(r) for r in range(4)] + [e.temperature_map()]
+    assert ms > 0
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
 def test_checkpoint_restore_replays_exactly():
     """ising_save_state / ising_restore_state: a restored run repeats the
     same random numbers, swaps and records bit for bit."""
