@@ -232,7 +232,12 @@ def main() -> int:
         "learned": summary(learned),
         "learned_over_fixed": {"samples_per_s_E": learned["samples_per_s_E"] / fixed["samples_per_s_E"],
                                "samples_per_s_Q": learned["samples_per_s_Q"] / fixed["samples_per_s_Q"],
-                               "site_updates_per_s": learned["updates_per_s"] / fixed["updates_per_s"]},
+                               "site_updates_per_s": learned["updates_per_s"] / fixed["updates_per_s"],
+                               # both runs make the same sweeps: the statistical gain alone
+                               "samples_per_sweep_E": (learned["samples_per_s_E"] * learned["wall"])
+                                                      / (fixed["samples_per_s_E"] * fixed["wall"]),
+                               "samples_per_sweep_Q": (learned["samples_per_s_Q"] * learned["wall"])
+                                                      / (fixed["samples_per_s_Q"] * fixed["wall"])},
         "training": {"records": n_records, "steps": TRAIN_STEPS, "batch": TRAIN_BATCH,
                      "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-10:].mean()),
                      "prep_seconds": prep_s},
