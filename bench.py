@@ -21,7 +21,13 @@ half-sweep kernel's HBM roofline fraction from hipEvent pairs around its
 launches inside the same schedule (ising_time_pt_sweeps, 200 sweeps after
 each timed run), and the FP64 CPU oracle's rate on this host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+`--workload c3` measures the batched uneven configuration instead: 256
+replicas = 8 field values Bz = 0, 0.05, .., 0.35 x 32 temperatures (0.2..2.0
+geometric), lattices 24^3, 32^3, 48^3 by field (b mod 3), J = 1, bulk D = 0.5,
+all in one launch per half sweep; both autotuners run (launch shape, then the
+sweeps-per-swap interval), then 2000 sweeps are timed with E/M every 10.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3]
 """
 from __future__ import annotations
 
@@ -103,6 +109,8 @@ def main() -> int:
                     help="with --profile-sweeps: use these launch-tuning flags instead of autotuning")
     ap.add_argument("--eager", action="store_true",
                     help="launch every period eagerly instead of replaying a captured hipGraph (same kernels)")
+    ap.add_argument("--workload", choices=("c5", "c3"), default="c5",
+                    help="c5: 64^3 x 64 PT, learned vs fixed (default); c3: 256 replicas of mixed lattices")
     ap.add_argument("--profile-sweeps", type=int, default=0,
                     help="only run N plain sweeps, N learned sweeps (untrained network) and one 256 MiB "
                          "copy probe (for rocprofv3 counter runs)")
@@ -115,6 +123,9 @@ def main() -> int:
     from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
     from ising_amd.analysis import independent_samples
     from ising_amd.learned import generate_data, train
+
+    if args.workload == "c3":
+        return bench_c3(args)
 
     T = geometric_temperatures(T_MIN, T_MAX, N_REP)
     ham = Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2, Kz=0.0)
@@ -254,6 +265,74 @@ def main() -> int:
                      "tuning_ms_per_half_sweep": {str(f): float(t) for f, t in tune_ms.items()},
                      "learned_kernel_mfma_TFLOPs": mlp_tflops, "mfma_bf16_peak_TFLOPs": 2500.0},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+    return 0
+
+
+C3_SIZES = (24, 32, 48)
+C3_SWEEPS = 2000
+C3_MEASURE_EVERY = 10
+C3_SWAP_CANDIDATES = (1, 2, 5, 10, 20, 50)
+C3_MAX_OVERHEAD = 0.05
+
+
+def bench_c3(args) -> int:
+    """Batched uneven PT: 8 ladders x 32 temperatures of mixed lattice sizes in
+    one launch; launch shape and sweeps-per-swap interval chosen by the
+    autotuners; 2000 timed sweeps."""
+    from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
+    T = geometric_temperatures(T_MIN, T_MAX, 32)
+    ladders = [Ladder(Lattice(C3_SIZES[b % 3], C3_SIZES[b % 3], C3_SIZES[b % 3]),
+                      Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.05 * b), list(1.0 / T), CONE) for b in range(8)]
+    n_sites = sum(len(l.betas) * l.lattice.sites for l in ladders)
+    eng = Engine(ladders, seed=SEED)
+    eng.init_random()
+    tune_flags, tune_ms = eng.autotune(10)
+    progress(f"c3 autotune: flags {tune_flags}")
+    swap_every, swap_ms = eng.autotune_schedule(C3_SWAP_CANDIDATES, n_sweeps=20, max_overhead=C3_MAX_OVERHEAD,
+                                                measure_every=C3_MEASURE_EVERY)
+    progress(f"c3 schedule: swap every {swap_every} ({swap_ms})")
+    eng.pt_run(200, swap_every, 0, 0, use_graph=not args.eager)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    out = eng.pt_run(C3_SWEEPS, swap_every, C3_MEASURE_EVERY, 0, use_graph=not args.eager)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    progress(f"c3 timed: {wall * 1e3:.1f} ms for {C3_SWEEPS} sweeps")
+    ms_launch = eng.time_pt_sweeps(200, swap_every, C3_MEASURE_EVERY, 0)
+    att, acc = eng.swap_stats()
+    bytes_launch = BYTES_PER_UPDATE * n_sites / 2
+    achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
+    line = {
+        "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
+        "value": n_sites * C3_SWEEPS / wall,
+        "unit": "site updates/s",
+        "n_gpus": 1,
+        "steps": C3_SWEEPS,
+        "warmup": 200,
+        "ms_per_step": wall * 1e3 / C3_SWEEPS,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 spins, fp64 reductions",
+        "data": "synthetic (seeded random-sphere start)",
+        "config": {"workload": "batched uneven PT: 256 replicas = 8 Bz (0..0.35) x 32 T (0.2-2.0 geometric), "
+                               "lattices 24^3/32^3/48^3 by field, J=1 D=0.5 (bulk); one launch per half sweep; "
+                               f"2000 sweeps, E/M every {C3_MEASURE_EVERY}, swap interval from the schedule tuner",
+                   "sites": n_sites, "step": "one sweep of every replica"},
+        "schedule": {"swap_every": swap_every, "candidates_ms_per_sweep": {str(k): v for k, v in swap_ms.items()},
+                     "max_overhead": C3_MAX_OVERHEAD},
+        "swap_acceptance": {"mean": float((acc / np.maximum(att, 1))[att > 0].mean()),
+                            "min": float((acc / np.maximum(att, 1))[att > 0].min())},
+        "energy_per_site_first_ladder": (out["energy"][-1][:32:8] / ladders[0].lattice.sites).tolist(),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_half_sweep",
+                     "ms_per_launch": ms_launch, "algorithmic_bytes_per_launch": bytes_launch,
+                     "tuning_flags": tune_flags,
+                     "tuning_ms_per_half_sweep": {str(f): float(t) for f, t in tune_ms.items()}},
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     eng.close()

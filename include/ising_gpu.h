@@ -269,6 +269,19 @@ int  ising_tuning_candidates(const ising_ctx* ctx, uint32_t* flags, int* count);
  * candidate in ising_tuning_candidates order. */
 int  ising_autotune(ising_ctx* ctx, int n_sweeps, uint32_t* best, double* ms);
 
+/* Sweeps-per-swap schedule tuning (no reference counterpart; the reference
+ * fixes 100 sweeps per swap round, src/PT.f90:46-48). Times the schedule
+ * *sched with swap_every = each of cand[n_cand] (at most
+ * ISING_TUNE_MAX_CANDIDATES, every value >= 1) over n_sweeps real sweeps
+ * each (they advance the chains; records are computed, not stored), and sets
+ * sched->swap_every to the smallest candidate whose time per sweep is within
+ * (1 + max_overhead) of the fastest candidate's: deterministic even/odd
+ * exchange mixes temperatures fastest when it runs most often, so the tuner
+ * trades only device time. ms_per_sweep (may be NULL, n_cand entries)
+ * receives each candidate's time per sweep. */
+int  ising_autotune_schedule(ising_ctx* ctx, ising_schedule* sched, const int32_t* cand, int n_cand,
+                             int n_sweeps, double max_overhead, double* ms_per_sweep);
+
 /* ---- learned proposal (no reference counterpart) ---------------------- */
 /* A per-site network sets the proposal radius from the six neighbour spins
  * and beta*h (never the site's own spin, so the proposal stays symmetric):

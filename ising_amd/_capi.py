@@ -114,6 +114,7 @@ SIGNATURES = {
     "ising_get_tuning": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "ising_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint32), _dp]),
     "ising_tuning_candidates": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
+    "ising_autotune_schedule": (C.c_int, [_P, C.POINTER(ising_schedule), _i32p, C.c_int, C.c_int, C.c_double, _dp]),
     "ising_mlp_set_params": (C.c_int, [_P, _fp, C.c_int]),
     "ising_mlp_get_params": (C.c_int, [_P, _fp, C.c_int]),
     "ising_mlp_enable": (C.c_int, [_P, C.c_int, C.c_int]),

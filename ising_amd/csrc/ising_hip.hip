@@ -3101,6 +3101,70 @@ int ising_autotune(ising_ctx* c, int n_sweeps, uint32_t* best, double* ms)
     return ISING_OK;
 }
 
+// Sweeps-per-swap schedule: times each candidate interval over n_sweeps
+// sweeps of the schedule (eager periods, one hipEvent pair around the whole
+// run of a candidate; two passes in alternating order, the faster kept) and
+// picks the most frequent exchange whose time per sweep is within
+// (1 + max_overhead) of the cheapest candidate's. Deterministic even/odd
+// exchange mixes temperatures fastest when it runs most often (Syed,
+// Bouchard-Cote, Deligiannidis, Doucet 2019: round trips of non-reversible
+// PT take O(N) rounds), so the device time of a round is its only cost.
+int ising_autotune_schedule(ising_ctx* c, ising_schedule* sched, const int32_t* cand, int n_cand, int n_sweeps,
+                            double max_overhead, double* ms_per_sweep)
+{
+    if (!c || !sched || !cand || n_cand < 1 || n_cand > ISING_TUNE_MAX_CANDIDATES || n_sweeps < 1 ||
+        !(max_overhead >= 0.0) || !std::isfinite(max_overhead))
+        return ISING_ERR_INVALID_ARG;
+    if (c->any_demag)
+        return set_err(c, ISING_ERR_UNSUPPORTED, "parallel tempering with the demagnetisation term");
+    for (int k = 0; k < n_cand; ++k)
+        if (cand[k] < 1) return set_err(c, ISING_ERR_INVALID_ARG, "swap interval candidate %d", cand[k]);
+    ising_schedule s = *sched;
+    if (s.measure_every < 0 || s.winding_every < 0 || s.winding_layer < 0 || s.winding_layer > c->zmax)
+        return set_err(c, ISING_ERR_INVALID_ARG, "bad schedule");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
+    std::vector<double> t(n_cand, 1e300);
+    SeriesDev none;
+    int rc = ISING_OK;
+    for (int pass = 0; pass < 2 && !rc; ++pass) {
+        for (int j = 0; j < n_cand && !rc; ++j) {
+            const int k = pass == 0 ? j : n_cand - 1 - j;
+            s.swap_every = cand[k];
+            int64_t period = 1;
+            for (int v : { s.swap_every, s.measure_every, s.winding_every })
+                if (v > 0) period = period / gcd64(period, v) * v;
+            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = set_err(c, ISING_ERR_HIP, "hipEventRecord"); break; }
+            for (int done = 0; done < n_sweeps && !rc;) {
+                const int cnt = (int)std::min<int64_t>(period, n_sweeps - done);
+                rc = enqueue_period(c, s, cnt, none);
+                done += cnt;
+            }
+            if (rc) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = set_err(c, ISING_ERR_HIP, "schedule timing");
+                break;
+            }
+            advance_host(c, (uint32_t)n_sweeps);
+            t[k] = std::min(t[k], (double)ms / n_sweeps);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    const double fastest = *std::min_element(t.begin(), t.end());
+    int best = -1;
+    for (int k = 0; k < n_cand; ++k)
+        if (t[k] <= (1.0 + max_overhead) * fastest && (best < 0 || cand[k] < cand[best])) best = k;
+    sched->swap_every = cand[best];
+    if (ms_per_sweep) for (int k = 0; k < n_cand; ++k) ms_per_sweep[k] = t[k];
+    return ISING_OK;
+}
+
 // ---- learned proposal ------------------------------------------------------
 
 static int mlp_pack(ising_ctx* c)

@@ -377,6 +377,21 @@ class Engine:
                                              ms.ctypes.data_as(C.POINTER(C.c_double))))
         return best.value, {f: float(t) for f, t in zip(cand, ms)}
 
+    def autotune_schedule(self, candidates: Sequence[int] = (1, 2, 5, 10, 20, 50), n_sweeps: int = 20,
+                          max_overhead: float = 0.05, measure_every: int = 0, winding_every: int = 0,
+                          winding_layer: int | None = None) -> tuple[int, dict[int, float]]:
+        """Sweeps-per-swap interval: the most frequent exchange whose time per
+        sweep is within (1 + max_overhead) of the fastest candidate's;
+        returns (interval, {candidate: ms per sweep})."""
+        cand = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(cand))
+        sch = _capi.ising_schedule(0, 0, measure_every, winding_every, 0,
+                                   0 if winding_layer is None else int(winding_layer) + 1)
+        self._check(self._lib.ising_autotune_schedule(self._h, C.byref(sch), cand.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                      len(cand), int(n_sweeps), float(max_overhead),
+                                                      ms.ctypes.data_as(C.POINTER(C.c_double))))
+        return sch.swap_every, {int(c): float(t) for c, t in zip(cand, ms)}
+
     # -- learned proposal ---------------------------------------------------
     @staticmethod
     def _f32(a) -> np.ndarray:

@@ -381,6 +381,17 @@ module ising_gpu_c
                         real(c_float), intent(out) :: d(*)
                         integer(c_int) :: rc
                 end function
+                function ising_autotune_schedule(ctx, sched, cand, n_cand, n_sweeps, max_overhead, ms) &
+                                bind(C, name="ising_autotune_schedule") result(rc)
+                        import :: c_ptr, c_int, c_int32_t, c_double, ising_schedule
+                        type(c_ptr), value :: ctx
+                        type(ising_schedule), intent(inout) :: sched
+                        integer(c_int32_t), intent(in) :: cand(*)
+                        integer(c_int), value :: n_cand, n_sweeps
+                        real(c_double), value :: max_overhead
+                        real(c_double), intent(out) :: ms(*)
+                        integer(c_int) :: rc
+                end function
                 function ising_time_pt_sweeps(ctx, sched, n_sweeps, ms) bind(C, name="ising_time_pt_sweeps") result(rc)
                         import :: c_ptr, c_int, c_double, ising_schedule
                         type(c_ptr), value :: ctx

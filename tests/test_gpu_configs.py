@@ -106,3 +106,29 @@ def test_batched_uneven_256_replicas():
         ho = orc.ham_from(ladders[b].ham)
         assert np.array_equal(st, p1[slot])
         assert abs(en0[slot] - orc.total_energy(lo, ho, st)) < 1e-9 * st.shape[1]
+
+
+def test_schedule_autotune_rule():
+    """The sweeps-per-swap tuner returns the most frequent exchange within the
+    overhead budget of the fastest candidate; with no budget the fastest
+    candidate, with an unbounded one the most frequent."""
+    T = geometric_temperatures(0.3, 2.0, 8)
+    ladders = [Ladder(Lattice(L, L, L), Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.1), list(1.0 / T), 0.5)
+               for L in (16, 24)]
+    cand = (1, 2, 5, 10)
+    with Engine(ladders, seed=5) as e:
+        e.init_random()
+        k, ms = e.autotune_schedule(cand, n_sweeps=20, max_overhead=0.05, measure_every=2)
+        assert set(ms) == set(cand) and all(t > 0 for t in ms.values())
+        fastest = min(ms.values())
+        assert k == min(c for c in cand if ms[c] <= 1.05 * fastest)
+        k1, _ = e.autotune_schedule(cand, n_sweeps=10, max_overhead=1e9)
+        assert k1 == 1
+        k0, ms0 = e.autotune_schedule(cand, n_sweeps=10, max_overhead=0.0)
+        assert ms0[k0] == min(ms0.values())
+        # the tuning runs real sweeps: the chains stay valid
+        en = e.energy()
+        s = e.get_spins(9)
+        lo = olat(ladders[1].lattice)
+        assert abs(en[9] - orc.total_energy(lo, orc.ham_from(ladders[1].ham), s)) < 1e-9 * s.shape[1]
+
