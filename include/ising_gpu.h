@@ -1,8 +1,8 @@
 /*
  * ising_gpu.h -- C ABI of libising_hip.so, the MI355X (gfx950) Metropolis +
- * parallel-tempering engine for classical Heisenberg spins on simple-cubic
- * and body-centred-cubic lattices (exchange + DMI + Zeeman + uniaxial
- * anisotropy + the reference's demagnetisation term).
+ * parallel-tempering engine for classical Heisenberg spins on simple-cubic,
+ * body-centred-cubic and hyper-kagome lattices (exchange + DMI + Zeeman +
+ * uniaxial anisotropy + the reference's demagnetisation term).
  *
  * Plain C types only; no exceptions cross this boundary. Every function
  * returns ISING_OK (0) on success or a negative ISING_ERR_* code; the text of
@@ -23,7 +23,11 @@
  *         - zeeman_total_scale*Bz sum_i S_i^z - Kz sum_i (S_i^z)^2
  * with d_a = D r_a for bulk DMI and d_a = D (z x (-r_a)) for the reference's
  * interfacial form (src/energyMin.f90:114,135-137). Single-site energy
- * changes use Bz with coefficient 1 (src/energyMin.f90:161-162).
+ * changes use Bz with coefficient 1 (src/energyMin.f90:161-162). On bcc and
+ * hyper-kagome cells a bond i -> j with unit vector r carries
+ * d_ij = sum_a r_a d_a (D r for bulk DMI, D (z x (-r)) for the interfacial
+ * form), the nearest-neighbour shell being the reference's first atom shell
+ * (src/chainMesh.f90:495-591).
  *
  * With ising_hamiltonian.demag = 1 the reference's demagnetisation term is
  * added: single-site changes -mu0 (S'-S).H_i and totals -mu0/2 sum S_i.H_i
@@ -69,6 +73,11 @@ typedef struct ising_ctx ising_ctx;
 #define ISING_CELL_SC  0  /* one atom per cell, 6 nearest neighbours          */
 #define ISING_CELL_BCC 1  /* atoms at (0,0,0) and (1/2,1/2,1/2), 8 nearest
                              neighbours (src/unit_cells.f90:7-30)             */
+#define ISING_CELL_HYPERKAGOME 2 /* the 12 atoms of hyper_kagome_unit_cell
+                             (src/unit_cells.f90:32-77), 4 nearest neighbours
+                             at sqrt(2)/4 of the cell edge; swept in three
+                             colours; periodic axes need >= 2 cells; no fused
+                             sweep or learned proposal                        */
 
 typedef struct {
     int32_t lx, ly, lz;      /* cells per axis                              */

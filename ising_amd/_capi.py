@@ -24,7 +24,7 @@ ERRORS = {
     -8: "ISING_ERR_STATE",
 }
 DMI_NONE, DMI_BULK, DMI_INTERFACIAL_REF = 0, 1, 2
-CELL_SC, CELL_BCC = 0, 1
+CELL_SC, CELL_BCC, CELL_HYPERKAGOME = 0, 1, 2
 MLP_NPARAM = 1793
 MLP_RECORD = 32
 TUNE_XCD_REMAP = 1

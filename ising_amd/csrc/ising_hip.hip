@@ -85,8 +85,8 @@ __device__ __forceinline__ uint32_t fmodq(uint32_t n, uint32_t quot, const FastD
 // Per replica storage slot.
 struct Rep {
     int32_t lx, ly, lz;
-    int32_t hx;             // row length in one colour's arrays: lx/2 (SC), lx (bcc)
-    int32_t cell;           // ISING_CELL_SC / ISING_CELL_BCC
+    int32_t hx;             // row length in one colour's arrays: lx/2 (SC), lx (bcc, hyper-kagome)
+    int32_t cell;           // ISING_CELL_*
     int32_t per;            // bit 0/1/2: axis x/y/z periodic
     int32_t ladder;
     int32_t first_block;    // first workgroup in the sweep/measure grid
@@ -98,10 +98,11 @@ struct Rep {
     int32_t f_first_block;  // first workgroup in the fused-sweep grid
     int32_t f_nblocks;      // fused-sweep workgroups (z chunks)
     int32_t plane;          // ly*hx: sites of one colour on one z plane
-    uint32_t half;          // N/2
+    int32_t ncol;           // colours of the sweep: 2, or 3 on hyper-kagome
+    uint32_t half;          // sites of one colour: N/2, or N/3 on hyper-kagome
     int64_t off;            // float offset of the x component block
     int64_t n;              // N
-    FastDiv div_hx, div_ly, div_lx;
+    FastDiv div_hx, div_ly, div_lx, div_lz;
 };
 
 // Per ladder Hamiltonian, FP32 copy for sweeps and FP64 copy for energies.
@@ -316,6 +317,111 @@ __device__ __forceinline__ f3 bcc_field(const Lad& L, const BccSums& b)
     return field_from_sums(L, b.sum, b.ax, b.ay, b.az);
 }
 
+// Hyper-kagome (src/unit_cells.f90:32-77): twelve atoms per cubic cell, four
+// nearest neighbours each at sqrt(2)/4 of the edge. The bond graph has
+// triangles, so the sweep takes three colours of four atom types each. A
+// site's storage rank is colour*4 + its type's place in the colour; site
+// (rank r, cell c) sits at r*C + c of the component arrays, C = lx*ly*lz,
+// c = (iz*ly + iy)*lx + ix, so each colour is one block of half = 4C sites
+// and the per-colour planes of the two-colour cells carry over. The host
+// builds the table from the atom positions (hk_build_table) into constant
+// memory.
+constexpr int kHkAtoms = 12;
+constexpr int kHkBonds = 4;
+constexpr int kHkPerColour = 4;
+
+struct HkBond {
+    int32_t rank;            // storage rank of the neighbour's atom type
+    int32_t dx, dy, dz;      // cell step to the neighbour, each in {-1, 0, 1}
+    float r[3];              // unit bond vector, site -> neighbour
+    double rd[3];
+};
+
+struct HkTable {
+    int32_t type_of_rank[kHkAtoms];
+    int32_t rank_of_type[kHkAtoms];
+    HkBond bond[kHkAtoms][kHkBonds];    // by storage rank
+    double frac[kHkAtoms][3];           // atom position in its cell (units of the edge), by type
+};
+
+__constant__ HkTable c_hk;
+
+struct HkSite { uint32_t ix, iy, iz, rank, cell; };
+
+// Cell and storage rank of index q of colour `colour` (q < half).
+__device__ __forceinline__ HkSite hk_site(const Rep& R, uint32_t q, int colour)
+{
+    const uint32_t row = fdiv(q, R.div_hx);
+    const uint32_t ix = q - row * (uint32_t)R.hx;
+    const uint32_t r2 = fdiv(row, R.div_ly);
+    const uint32_t iy = row - r2 * (uint32_t)R.ly;
+    const uint32_t j = fdiv(r2, R.div_lz);
+    const uint32_t iz = r2 - j * (uint32_t)R.lz;
+    const uint32_t C = (uint32_t)R.plane * (uint32_t)R.lz;
+    return HkSite{ ix, iy, iz, (uint32_t)colour * kHkPerColour + j, q - j * C };
+}
+
+// Index rank*C + cell of bond B's neighbour (always in range: the periodic
+// image across an open face) and whether that bond exists.
+__device__ __forceinline__ bool hk_nbr(const Rep& R, const HkSite& t, const HkBond& B, uint32_t& idx)
+{
+    int c[3] = { (int)t.ix + B.dx, (int)t.iy + B.dy, (int)t.iz + B.dz };
+    const int n[3] = { R.lx, R.ly, R.lz };
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] < 0) { c[a] += n[a]; ok = ok && ((R.per >> a) & 1); }
+        else if (c[a] >= n[a]) { c[a] -= n[a]; ok = ok && ((R.per >> a) & 1); }
+    }
+    const uint32_t C = (uint32_t)R.plane * (uint32_t)R.lz;
+    idx = (uint32_t)B.rank * C + ((uint32_t)c[2] * (uint32_t)R.ly + (uint32_t)c[1]) * (uint32_t)R.lx + (uint32_t)c[0];
+    return ok;
+}
+
+// h = J sum_b S_b - sum_a A_a x d_a + Bz z with A_a = sum_b r_a(b) S_b, i.e.
+// d_b = sum_a r_a(b) d_a per bond (field_from_sums, d_a unscaled). `all`:
+// the colour-0 planes, indexed over the whole replica.
+__device__ __forceinline__ f3 hk_field(const Lad& L, const Rep& R, const Planes& all, const HkSite& t)
+{
+    f3 sum{ 0.f, 0.f, 0.f }, ax = sum, ay = sum, az = sum;
+#pragma unroll
+    for (int b = 0; b < kHkBonds; ++b) {
+        const HkBond& B = c_hk.bond[t.rank][b];
+        uint32_t i;
+        const bool ok = hk_nbr(R, t, B, i);
+        f3 v = ld3(all, i);
+        if (!ok) v = f3{ 0.f, 0.f, 0.f };
+        sum.x += v.x; sum.y += v.y; sum.z += v.z;
+        ax.x += B.r[0] * v.x; ax.y += B.r[0] * v.y; ax.z += B.r[0] * v.z;
+        ay.x += B.r[1] * v.x; ay.y += B.r[1] * v.y; ay.z += B.r[1] * v.z;
+        az.x += B.r[2] * v.x; az.y += B.r[2] * v.y; az.z += B.r[2] * v.z;
+    }
+    return field_from_sums(L, sum, ax, ay, az);
+}
+
+// FP64 energy of site t (spin s) over its bonds to higher storage ranks
+// (every bond once: a bond never joins two atoms of one type):
+// -J S.T + d_b.(S x T), d_b = sum_a r_a(b) d_a.
+__device__ __forceinline__ double hk_bond_energy(const Lad& L, const Rep& R, const Planes& all, const HkSite& t,
+                                                 double sx, double sy, double sz)
+{
+    double e = 0.0;
+    for (int b = 0; b < kHkBonds; ++b) {
+        const HkBond& B = c_hk.bond[t.rank][b];
+        if ((uint32_t)B.rank < t.rank) continue;
+        uint32_t i;
+        if (!hk_nbr(R, t, B, i)) continue;
+        const f3 tf = ld3(all, i);
+        const double tx = tf.x, ty = tf.y, tz = tf.z;
+        const double dbx = B.rd[0] * L.dd[0] + B.rd[1] * L.dd[3] + B.rd[2] * L.dd[6];
+        const double dby = B.rd[0] * L.dd[1] + B.rd[1] * L.dd[4] + B.rd[2] * L.dd[7];
+        const double dbz = B.rd[0] * L.dd[2] + B.rd[1] * L.dd[5] + B.rd[2] * L.dd[8];
+        const double cx = sy * tz - sz * ty, cy = sz * tx - sx * tz, cz = sx * ty - sy * tx;
+        e += -L.Jd * (sx * tx + sy * ty + sz * tz) + (dbx * cx + dby * cy + dbz * cz);
+    }
+    return e;
+}
+
 __device__ __forceinline__ f3 local_field(const Rep& R, const Lad& L, const Planes& oth, const Nbr& nb)
 {
     f3 v[6];
@@ -525,6 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
     const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
     const int slot = block_rep[blk];
     const Rep R = reps[slot];
+    if (colour >= R.ncol) return;   // the third colour's launch, on a two-colour replica (uniform)
     const Lad L = lads[R.ladder];
     const float beta = beta_slot[slot];
     const uint32_t sweep = *sweep_ctr + sweep_off;
@@ -549,6 +656,12 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
 #pragma unroll
         for (int k = 0; k < 6; ++k) nbr[k] = f3{ 0.f, 0.f, 0.f };
         site = 2u * site_number(R, h, iy, iz) + (uint32_t)colour;
+    } else if (R.cell == ISING_CELL_HYPERKAGOME) {    // uniform; never learned, never measured here
+        const HkSite t = hk_site(R, q, colour);
+        hf = hk_field(L, R, planes(spins, R, 0), t);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nbr[k] = f3{ 0.f, 0.f, 0.f };
+        site = (uint32_t)kHkAtoms * site_number(R, t.ix, t.iy, t.iz) + (uint32_t)c_hk.type_of_rank[t.rank];
     } else {
         const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
         const uint32_t ix = 2u * h + p;
@@ -987,6 +1100,22 @@ __global__ __launch_bounds__(kBlock) void k_measure(
             e -= 0.5 * L.mu0 * (ux * d[0] + uy * d[R.n] + uz * d[2 * R.n]);
         }
         v[0] = e; v[1] = sx + ux; v[2] = sy + uy; v[3] = sz + uz;
+    } else if (q < R.half && R.cell == ISING_CELL_HYPERKAGOME) {
+        // The sites at index q of the three colours: each one's bonds to
+        // higher storage ranks, its on-site terms and spin.
+        const Planes all = planes(spins, R, 0);
+        for (int colour = 0; colour < R.ncol; ++colour) {
+            const HkSite t = hk_site(R, q, colour);
+            const f3 sf = ld3(planes(spins, R, colour), q);
+            const double sx = sf.x, sy = sf.y, sz = sf.z;
+            double e = hk_bond_energy(L, R, all, t, sx, sy, sz);
+            e -= L.zs * L.Bzd * sz + L.Kzd * sz * sz;
+            if (L.demag) {
+                const double* d = demag + R.off + (uint32_t)colour * R.half + q;
+                e -= 0.5 * L.mu0 * (sx * d[0] + sy * d[R.n] + sz * d[2 * R.n]);
+            }
+            v[0] += e; v[1] += sx; v[2] += sy; v[3] += sz;
+        }
     } else if (q < R.half) {
         const uint32_t row = fdiv(q, R.div_hx);
         const uint32_t h = q - row * (uint32_t)R.hx;
@@ -1068,10 +1197,13 @@ __global__ __launch_bounds__(kBlock) void k_wave_finalize(
     }
 }
 
-// One workgroup per slot: fixed-order sum of its partials (deterministic).
+// One workgroup per slot: fixed-order sum of its partials (deterministic);
+// with rec, the sums are also written to the series at the slot's
+// temperature (measurements of three-colour sweeps).
 __global__ __launch_bounds__(kBlock) void k_measure_finalize(
     const Rep* __restrict__ reps, const double* __restrict__ partial, int slot_base,
-    double* __restrict__ energy, double* __restrict__ mag)
+    double* __restrict__ energy, double* __restrict__ mag, const int32_t* __restrict__ slot_of_temp, int nrep,
+    SeriesRec rec)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
     const int slot = blockIdx.x + slot_base;
@@ -1082,10 +1214,19 @@ __global__ __launch_bounds__(kBlock) void k_measure_finalize(
         v[0] += p[0]; v[1] += p[1]; v[2] += p[2]; v[3] += p[3];
     }
     block_sum4(v, lds);
+    const double inv = 1.0 / (double)R.n;
     if (threadIdx.x == 0) {
         energy[slot] = v[0];
-        const double inv = 1.0 / (double)R.n;
         mag[3 * slot] = v[1] * inv; mag[3 * slot + 1] = v[2] * inv; mag[3 * slot + 2] = v[3] * inv;
+    }
+    if (rec.a) {       // uniform
+        const int t = temp_of_slot(slot_of_temp, nrep, slot);
+        const uint32_t m = *rec.ctr + rec.local;
+        if (threadIdx.x == 0 && m < rec.cap && t >= 0) {
+            const int64_t e = (int64_t)m * nrep + t;
+            rec.a[e] = v[0];
+            rec.b[3 * e] = v[1] * inv; rec.b[3 * e + 1] = v[2] * inv; rec.b[3 * e + 2] = v[3] * inv;
+        }
     }
 }
 
@@ -1115,13 +1256,20 @@ __device__ __forceinline__ double triangle_sigma(const double a[3], const double
     return 2.0 * atan2(im, re);
 }
 
-// Spin of the plaquette corner at cell (ix, iy, iz): the site on SC, atom 0
-// of the cell on bcc (the reference's derivativeList steps to the same atom
-// of the neighbouring cell, src/chainMesh.f90:931-1031).
+// Spin of the plaquette corner at cell (ix, iy, iz): the site on SC; on
+// multi-atom cells the cell's firstAtomInMeshCell, which is the LAST atom of
+// the unit cell (makeChainMesh adds a cell's atoms in order and
+// addAtomToChainCell puts each at the head of the list,
+// src/chainMesh.f90:663-671,873-881): the body-centre atom on bcc, atom 12 on
+// hyper-kagome. derivativeList steps to the same atom of the neighbouring
+// cell (src/chainMesh.f90:931-1031).
 __device__ __forceinline__ f3 spin_at(const float* __restrict__ spins, const Rep& R, uint32_t ix, uint32_t iy, uint32_t iz)
 {
     if (R.cell == ISING_CELL_BCC)
-        return ld3(planes(spins, R, 0), (iz * (uint32_t)R.ly + iy) * (uint32_t)R.hx + ix);
+        return ld3(planes(spins, R, 1), (iz * (uint32_t)R.ly + iy) * (uint32_t)R.hx + ix);
+    if (R.cell == ISING_CELL_HYPERKAGOME)
+        return ld3(planes(spins, R, 0), (uint32_t)c_hk.rank_of_type[kHkAtoms - 1] * ((uint32_t)R.plane * (uint32_t)R.lz)
+                                            + (iz * (uint32_t)R.ly + iy) * (uint32_t)R.lx + ix);
     const uint32_t k = (ix + iy + iz) & 1u;
     const uint32_t idx = (iz * (uint32_t)R.ly + iy) * (uint32_t)R.hx + (ix >> 1);
     return ld3(planes(spins, R, (int)k), idx);
@@ -1310,13 +1458,20 @@ __global__ __launch_bounds__(kBlock) void k_init(
     const uint32_t h = q - row * (uint32_t)R.hx;
     const uint32_t iz = fdiv(row, R.div_ly);
     const uint32_t iy = row - iz * (uint32_t)R.ly;
-    const bool bcc = R.cell == ISING_CELL_BCC;
-    for (int colour = 0; colour < 2; ++colour) {
+    const bool bcc = R.cell == ISING_CELL_BCC, hk = R.cell == ISING_CELL_HYPERKAGOME;
+    for (int colour = 0; colour < R.ncol; ++colour) {
         const Planes own = planes(spins, R, colour);
-        const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
-        const uint32_t ix = bcc ? h : 2u * h + p;
-        const uint32_t cellsite = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
-        const uint32_t site = bcc ? 2u * cellsite + (uint32_t)colour : cellsite;
+        uint32_t site;
+        if (hk) {
+            const HkSite t = hk_site(R, q, colour);
+            site = (uint32_t)kHkAtoms * ((t.ix * (uint32_t)R.ly + t.iy) * (uint32_t)R.lz + t.iz)
+                   + (uint32_t)c_hk.type_of_rank[t.rank];
+        } else {
+            const uint32_t p = (iy + iz + (uint32_t)colour) & 1u;
+            const uint32_t ix = bcc ? h : 2u * h + p;
+            const uint32_t cellsite = (ix * (uint32_t)R.ly + iy) * (uint32_t)R.lz + iz;
+            site = bcc ? 2u * cellsite + (uint32_t)colour : cellsite;
+        }
         const u32x4 r = philox4x32_10(u32x4{ site, 0xFFFFFFFFu, kStreamInit, seed_hi }, seed_lo, (uint32_t)slot);
         float x, y, z;
         if (mode == 0) {
@@ -1345,6 +1500,22 @@ __global__ __launch_bounds__(kBlock) void k_delta_energy(
     if (e >= count) return;
     const Rep R = reps[slot];
     const Lad L = lads[R.ladder];
+    const f3 sn{ props[3 * e], props[3 * e + 1], props[3 * e + 2] };
+    if (R.cell == ISING_CELL_HYPERKAGOME) {
+        const uint32_t site = (uint32_t)sites[e];
+        const uint32_t type = site % (uint32_t)kHkAtoms, cs = site / (uint32_t)kHkAtoms;
+        const uint32_t iz = cs % (uint32_t)R.lz, iy = (cs / (uint32_t)R.lz) % (uint32_t)R.ly;
+        const uint32_t ix = cs / ((uint32_t)R.lz * (uint32_t)R.ly);
+        const uint32_t rank = (uint32_t)c_hk.rank_of_type[type];
+        const uint32_t cell = (iz * (uint32_t)R.ly + iy) * (uint32_t)R.lx + ix;
+        const uint32_t t = rank * ((uint32_t)R.plane * (uint32_t)R.lz) + cell;   // colour*half + q
+        const Planes all = planes(spins, R, 0);
+        const f3 s = ld3(all, t);
+        f3 hf = hk_field(L, R, all, HkSite{ ix, iy, iz, rank, cell });
+        if (L.demag) add_demag(L, R, demag, t, hf);
+        out[e] = (double)delta_e(s, sn, hf, L.Kz);
+        return;
+    }
     const bool bcc = R.cell == ISING_CELL_BCC;
     const uint32_t site = (uint32_t)sites[e];
     const uint32_t cellsite = bcc ? site >> 1 : site;
@@ -1361,7 +1532,6 @@ __global__ __launch_bounds__(kBlock) void k_delta_energy(
     f3 hf = bcc ? bcc_field(L, bcc_sums(R, oth, q, ix, iy, iz, (int)colour))
                 : local_field(R, L, oth, neighbours<false>(R, q, h, iy, iz, ix & 1u));
     if (L.demag) add_demag(L, R, demag, colour * R.half + q, hf);
-    const f3 sn{ props[3 * e], props[3 * e + 1], props[3 * e + 2] };
     out[e] = (double)delta_e(s, sn, hf, L.Kz);
 }
 
@@ -1420,8 +1590,24 @@ __device__ __forceinline__ double cic_axis(int atom_cell, double frac, int centr
 __device__ __forceinline__ uint32_t atom_index(const Rep& R, int i, int j, int k, int a)
 {
     const uint32_t row = (uint32_t)k * (uint32_t)R.ly + (uint32_t)j;
+    if (R.cell == ISING_CELL_HYPERKAGOME)
+        return (uint32_t)c_hk.rank_of_type[a] * ((uint32_t)R.plane * (uint32_t)R.lz) + row * (uint32_t)R.lx + (uint32_t)i;
     if (R.cell == ISING_CELL_BCC) return (uint32_t)a * R.half + row * (uint32_t)R.hx + (uint32_t)i;
     return (uint32_t)((i + j + k) & 1) * R.half + row * (uint32_t)R.hx + ((uint32_t)i >> 1);
+}
+
+// Atoms per cell and the position of atom a in its cell along `axis` (units
+// of the edge): the corner on SC, corner and body centre on bcc, the table's
+// fractions on hyper-kagome (down to -3/4: such an atom still deposits only
+// on the 27 cells around its own, as the reference's loop does).
+__device__ __forceinline__ int atoms_per_cell(const Rep& R)
+{
+    return R.cell == ISING_CELL_HYPERKAGOME ? kHkAtoms : (R.cell == ISING_CELL_BCC ? 2 : 1);
+}
+__device__ __forceinline__ double atom_frac(const Rep& R, int a, int axis)
+{
+    if (R.cell == ISING_CELL_HYPERKAGOME) return c_hk.frac[a][axis];
+    return 0.5 * (double)a;
 }
 
 __global__ __launch_bounds__(kBlock) void k_demag_deposit(
@@ -1435,7 +1621,7 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (cell >= Nc) return;
     const int k = (int)(cell % nz), j = (int)((cell / nz) % ny), i = (int)(cell / ((int64_t)nz * ny));
-    const int na = R.cell == ISING_CELL_BCC ? 2 : 1;
+    const int na = atoms_per_cell(R);
     const float* sp = spins + R.off;
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
     for (int di = -1; di <= 1; ++di) {
@@ -1445,8 +1631,8 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
             for (int dk = -1; dk <= 1; ++dk) {
                 const int nk = wrap_cell(k + dk, nz);
                 for (int a = 0; a < na; ++a) {
-                    const double f = 0.5 * a;
-                    const double w = cic_axis(ni, f, i, nx) * cic_axis(nj, f, j, ny) * cic_axis(nk, f, k, nz);
+                    const double w = cic_axis(ni, atom_frac(R, a, 0), i, nx) * cic_axis(nj, atom_frac(R, a, 1), j, ny)
+                                     * cic_axis(nk, atom_frac(R, a, 2), k, nz);
                     if (w == 0.0) continue;
                     const uint32_t t = atom_index(R, ni, nj, nk, a);
                     m0 += w * (double)sp[t];
@@ -1603,28 +1789,42 @@ __global__ __launch_bounds__(kBlock) void k_demag_gather(
     const int64_t tt = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (tt >= R.n) return;
     const uint32_t t = (uint32_t)tt;
-    const uint32_t colour = t >= R.half ? 1u : 0u;
-    const uint32_t q = t - colour * R.half;
-    const uint32_t row = q / (uint32_t)R.hx, h = q - row * (uint32_t)R.hx;
-    const int iz = (int)(row / (uint32_t)R.ly), iy = (int)(row - (uint32_t)iz * (uint32_t)R.ly);
-    const bool bcc = R.cell == ISING_CELL_BCC;
-    const int ix = bcc ? (int)h : (int)(2u * h + (((uint32_t)(iy + iz) + colour) & 1u));
-    const double f = bcc && colour ? 0.5 : 0.0;
+    int ix, iy, iz;
+    double f[3];
+    if (R.cell == ISING_CELL_HYPERKAGOME) {
+        const uint32_t C = (uint32_t)R.plane * (uint32_t)R.lz;
+        const uint32_t rank = t / C, cell = t - rank * C;
+        const uint32_t row = cell / (uint32_t)R.lx;
+        ix = (int)(cell - row * (uint32_t)R.lx);
+        iz = (int)(row / (uint32_t)R.ly);
+        iy = (int)(row - (uint32_t)iz * (uint32_t)R.ly);
+        const int type = c_hk.type_of_rank[rank];
+        for (int a = 0; a < 3; ++a) f[a] = c_hk.frac[type][a];
+    } else {
+        const uint32_t colour = t >= R.half ? 1u : 0u;
+        const uint32_t q = t - colour * R.half;
+        const uint32_t row = q / (uint32_t)R.hx, h = q - row * (uint32_t)R.hx;
+        iz = (int)(row / (uint32_t)R.ly);
+        iy = (int)(row - (uint32_t)iz * (uint32_t)R.ly);
+        const bool bcc = R.cell == ISING_CELL_BCC;
+        ix = bcc ? (int)h : (int)(2u * h + (((uint32_t)(iy + iz) + colour) & 1u));
+        f[0] = f[1] = f[2] = bcc && colour ? 0.5 : 0.0;
+    }
     const int nx = R.lx, ny = R.ly, nz = R.lz;
     const int64_t Nc = (int64_t)nx * ny * nz;
     const double* m = mesh + dreps[blockIdx.y].moff;
     double h0 = 0.0, h1 = 0.0, h2 = 0.0;
     for (int di = -1; di <= 1; ++di) {
         const int ci = wrap_cell(ix + di, nx);
-        const double wx = cic_axis(ix, f, ci, nx);
+        const double wx = cic_axis(ix, f[0], ci, nx);
         if (wx == 0.0) continue;
         for (int dj = -1; dj <= 1; ++dj) {
             const int cj = wrap_cell(iy + dj, ny);
-            const double wy = cic_axis(iy, f, cj, ny);
+            const double wy = cic_axis(iy, f[1], cj, ny);
             if (wy == 0.0) continue;
             for (int dk = -1; dk <= 1; ++dk) {
                 const int ck = wrap_cell(iz + dk, nz);
-                const double w = wx * wy * cic_axis(iz, f, ck, nz);
+                const double w = wx * wy * cic_axis(iz, f[2], ck, nz);
                 if (w == 0.0) continue;
                 const int64_t cell = ((int64_t)ci * ny + cj) * nz + ck;
                 h0 += w * m[cell];
@@ -1677,6 +1877,7 @@ struct ising_ctx {
     std::vector<int> learned;    // per ladder
     // demagnetising field (allocated on first use)
     bool any_demag = false;      // some ladder has the demagnetisation term on
+    int max_ncol = 2;            // colour launches per sweep: 3 with a hyper-kagome ladder
     int demag_interval = 1;      // sweeps between field recomputations in ising_sweep
     bool demag_ready = false;
     std::vector<DemagRep> dreps;
@@ -1805,24 +2006,118 @@ Lad make_lad(const ising_hamiltonian& h, double cone, int cell)
     return L;
 }
 
+int atoms_of(int cell) { return cell == ISING_CELL_HYPERKAGOME ? kHkAtoms : (cell == ISING_CELL_BCC ? 2 : 1); }
+
+// The hyper-kagome table. Atom positions of hyper_kagome_unit_cell
+// (src/unit_cells.f90:32-77) in units of the edge a = 6; each atom's bonds
+// are the first shell of initAtomShells (src/chainMesh.f90:495-591): the
+// atoms of the cells within two cells sorted by distance, the shell being the
+// nearest one and each next one at most 0.1 a (makeChainMesh's threshold,
+// :783) further than the one before. Colours: the fewest for which the atom
+// types' bond graph has a proper colouring, the lexicographically first one
+// (each type the smallest colour no earlier bonded type holds, backtracking).
+// Storage ranks run colour by colour, types in order within a colour.
+static const double kHkFrac[kHkAtoms][3] = {
+    { -0.5, 0.0, 0.5 },   { -0.25, 0.75, 0.5 }, { -0.5, 0.75, 0.25 }, { -0.25, 0.25, 0.0 },
+    { -0.5, 0.25, 0.75 }, { -0.25, 0.5, 0.75 }, { -0.75, 0.5, 0.25 }, { 0.0, 0.5, 0.5 },
+    { 0.0, 0.25, 0.25 },  { -0.75, 0.75, 0.0 }, { 0.0, 0.0, 0.0 },    { -0.75, 0.0, 0.75 } };
+
+static bool hk_colour_from(int t, int k, const bool (&adj)[kHkAtoms][kHkAtoms], int (&col)[kHkAtoms])
+{
+    if (t == kHkAtoms) return true;
+    for (int cc = 0; cc < k; ++cc) {
+        bool ok = true;
+        for (int u = 0; u < t && ok; ++u) ok = !(adj[t][u] && col[u] == cc);
+        if (!ok) continue;
+        col[t] = cc;
+        if (hk_colour_from(t + 1, k, adj, col)) return true;
+    }
+    return false;
+}
+
+static bool hk_build_table(HkTable& T)
+{
+    struct Cand { double d; int u, s[3]; };
+    const double a = 6.0, thr = 0.1 * a;
+    bool adj[kHkAtoms][kHkAtoms] = {};
+    std::vector<Cand> shell[kHkAtoms];
+    for (int t = 0; t < kHkAtoms; ++t) {
+        std::vector<Cand> v;
+        for (int u = 0; u < kHkAtoms; ++u)
+            for (int sx = -2; sx <= 2; ++sx)
+                for (int sy = -2; sy <= 2; ++sy)
+                    for (int sz = -2; sz <= 2; ++sz) {
+                        if (u == t && sx == 0 && sy == 0 && sz == 0) continue;
+                        const double d[3] = { (kHkFrac[u][0] + sx - kHkFrac[t][0]) * a,
+                                              (kHkFrac[u][1] + sy - kHkFrac[t][1]) * a,
+                                              (kHkFrac[u][2] + sz - kHkFrac[t][2]) * a };
+                        v.push_back(Cand{ std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), u, { sx, sy, sz } });
+                    }
+        std::stable_sort(v.begin(), v.end(), [](const Cand& x, const Cand& y) { return x.d < y.d; });
+        shell[t].push_back(v[0]);
+        for (size_t j = 1; j < v.size() && v[j].d - v[j - 1].d <= thr; ++j) shell[t].push_back(v[j]);
+        if ((int)shell[t].size() != kHkBonds) return false;
+        for (const Cand& b : shell[t]) {
+            if (b.u == t || std::abs(b.s[0]) > 1 || std::abs(b.s[1]) > 1 || std::abs(b.s[2]) > 1) return false;
+            adj[t][b.u] = adj[b.u][t] = true;
+        }
+    }
+    int col[kHkAtoms] = {};
+    int k = 1;
+    while (k <= kHkAtoms && !hk_colour_from(0, k, adj, col)) ++k;
+    if (k * kHkPerColour != kHkAtoms) return false;     // three colours of four types
+    int fill[kHkAtoms] = {};
+    for (int t = 0; t < kHkAtoms; ++t) {
+        if (fill[col[t]] == kHkPerColour) return false;
+        const int r = col[t] * kHkPerColour + fill[col[t]]++;
+        T.rank_of_type[t] = r;
+        T.type_of_rank[r] = t;
+    }
+    for (int t = 0; t < kHkAtoms; ++t) {
+        for (int x = 0; x < 3; ++x) T.frac[t][x] = kHkFrac[t][x];
+        for (int b = 0; b < kHkBonds; ++b) {
+            const Cand& C = shell[t][b];
+            HkBond& B = T.bond[T.rank_of_type[t]][b];
+            B.rank = T.rank_of_type[C.u];
+            B.dx = C.s[0]; B.dy = C.s[1]; B.dz = C.s[2];
+            for (int x = 0; x < 3; ++x) {
+                B.rd[x] = (kHkFrac[C.u][x] + C.s[x] - kHkFrac[t][x]) * a / C.d;
+                B.r[x] = (float)B.rd[x];
+            }
+        }
+    }
+    return true;
+}
+
+// The table, built once per process (nullptr if the construction failed).
+const HkTable* hk_table()
+{
+    static HkTable T;
+    static const bool ok = hk_build_table(T);
+    return ok ? &T : nullptr;
+}
+
 int check_lattice(ising_ctx* c, const ising_lattice& l)
 {
     const int n[3] = { l.lx, l.ly, l.lz };
-    if (l.cell != ISING_CELL_SC && l.cell != ISING_CELL_BCC)
+    if (l.cell != ISING_CELL_SC && l.cell != ISING_CELL_BCC && l.cell != ISING_CELL_HYPERKAGOME)
         return set_err(c, ISING_ERR_BAD_DIMS, "unknown unit cell %d", l.cell);
-    const bool bcc = l.cell == ISING_CELL_BCC;
+    const bool sc = l.cell == ISING_CELL_SC;
     for (int a = 0; a < 3; ++a) {
         if (n[a] < 1) return set_err(c, ISING_ERR_BAD_DIMS, "axis %d has %d cells", a, n[a]);
-        // bcc's sublattices are the two colours for any length; a periodic
-        // axis of one cell would bond an atom to the same neighbour twice
-        if (bcc && l.periodic[a] && n[a] < 2)
-            return set_err(c, ISING_ERR_BAD_DIMS, "periodic bcc axis %d needs >= 2 cells (got %d)", a, n[a]);
-        if (!bcc && l.periodic[a] && (n[a] < 4 || (n[a] & 1)))
+        // bcc and hyper-kagome colour by atom, for any length; a periodic axis
+        // of one cell would bond an atom to the same neighbour twice (bcc) or
+        // bring atoms of other shells to nearest-neighbour distance
+        // (hyper-kagome)
+        if (!sc && l.periodic[a] && n[a] < 2)
+            return set_err(c, ISING_ERR_BAD_DIMS, "periodic %s axis %d needs >= 2 cells (got %d)",
+                           l.cell == ISING_CELL_BCC ? "bcc" : "hyper-kagome", a, n[a]);
+        if (sc && l.periodic[a] && (n[a] < 4 || (n[a] & 1)))
             return set_err(c, ISING_ERR_BAD_DIMS,
                            "periodic axis %d needs an even length >= 4 for the two-colour sweep (got %d)", a, n[a]);
     }
-    if (!bcc && (l.lx & 1)) return set_err(c, ISING_ERR_BAD_DIMS, "lx must be even (got %d)", l.lx);
-    const int64_t N = (int64_t)l.lx * l.ly * l.lz * (bcc ? 2 : 1);
+    if (sc && (l.lx & 1)) return set_err(c, ISING_ERR_BAD_DIMS, "lx must be even (got %d)", l.lx);
+    const int64_t N = (int64_t)l.lx * l.ly * l.lz * atoms_of(l.cell);
     if (N >= (1ll << 31)) return set_err(c, ISING_ERR_BAD_DIMS, "lattice of %lld sites too large", (long long)N);
     return ISING_OK;
 }
@@ -1862,10 +2157,18 @@ int upload_lads(ising_ctx* c)
 }
 
 // Index colour*half + q in a replica's component arrays of the site with
-// reference index `site` (SC: (ix*ly + iy)*lz + iz; bcc: that cell number
-// times 2 plus the atom).
+// reference index `site` (SC: (ix*ly + iy)*lz + iz; bcc and hyper-kagome:
+// that cell number times the atoms per cell plus the atom).
 int64_t dev_index(const Rep& r, int64_t site)
 {
+    if (r.cell == ISING_CELL_HYPERKAGOME) {
+        const int type = (int)(site % kHkAtoms);
+        int64_t t = site / kHkAtoms;
+        const int64_t iz = t % r.lz; t /= r.lz;
+        const int64_t iy = t % r.ly;
+        const int64_t ix = t / r.ly;
+        return (int64_t)hk_table()->rank_of_type[type] * ((int64_t)r.lx * r.ly * r.lz) + (iz * r.ly + iy) * r.lx + ix;
+    }
     const bool bcc = r.cell == ISING_CELL_BCC;
     int64_t t = bcc ? site >> 1 : site;
     const int64_t iz = t % r.lz; t /= r.lz;
@@ -1969,8 +2272,18 @@ struct LaunchTimer {
     }
 };
 
+// Buffer offset of the state after `done` sweeps of a period.
+uint32_t buf_off(const ising_ctx* c, int done) { return fused_active(c) ? (uint32_t)done : 0u; }
+
+// Exact FP64 E and M of every slot for the state after `done` sweeps of the
+// period; with rec, also recorded in the series.
+int launch_measure_all(ising_ctx* c, int done = 0, const SeriesRec& rec = kNoRecord);
+
 // exact (half sweeps only): the colour-1 launch records FP64 per-site
-// energies (pair_energy_exact) instead of FP32 ones.
+// energies (pair_energy_exact) instead of FP32 ones. With a hyper-kagome
+// ladder every sweep is three colour launches (colour 2 leaves two-colour
+// replicas alone) and measurements come from the measure kernel, the
+// colour-1 pair energies needing a bipartite bond graph.
 int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesRec& rec = kNoRecord,
                  bool exact = false, LaunchTimer* timer = nullptr)
 {
@@ -2004,6 +2317,15 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
     if (learned) launch_half_kernel<false, true>(c, off, 0, t0, t1);
     else launch_half_kernel<false, false>(c, off, 0, t0, t1);
     LAUNCH_CHECK(c);
+    if (c->max_ncol > 2) {
+        for (int k = 1; k < c->max_ncol; ++k) {
+            if ((rc = pair())) return rc;
+            if (learned) launch_half_kernel<false, true>(c, off, k, t0, t1);
+            else launch_half_kernel<false, false>(c, off, k, t0, t1);
+            LAUNCH_CHECK(c);
+        }
+        return measure ? launch_measure_all(c, 0, rec) : ISING_OK;
+    }
     if (measure) {
         if (exact) {
             if (learned) launch_half_kernel<true, true, true>(c, off, 1);
@@ -2024,9 +2346,6 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
-
-// Buffer offset of the state after `done` sweeps of a period.
-uint32_t buf_off(const ising_ctx* c, int done) { return fused_active(c) ? (uint32_t)done : 0u; }
 
 // Mesh, twiddle and field buffers of the demagnetising field, allocated on
 // first use (the field starts at zero).
@@ -2108,16 +2427,14 @@ int launch_demag(ising_ctx* c, int only)
     return ISING_OK;
 }
 
-// Exact FP64 E and M of every slot, for the state after `done` sweeps of the
-// period.
-int launch_measure_all(ising_ctx* c, int done = 0)
+int launch_measure_all(ising_ctx* c, int done, const SeriesRec& rec)
 {
     hipLaunchKernelGGL(k_measure, dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
                        spin_buf(c), buf_off(c, done), c->d_reps, c->d_lads, (const Lad*)nullptr, c->d_block_rep, 0, c->d_partial,
                        (const double*)c->d_demag);
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_measure_finalize, dim3(c->R), dim3(kBlock), 0, c->stream,
-                       c->d_reps, c->d_partial, 0, c->d_energy, c->d_mag);
+                       c->d_reps, c->d_partial, 0, c->d_energy, c->d_mag, (const int32_t*)c->d_slot_of_temp, c->R, rec);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
@@ -2187,10 +2504,10 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
         const SeriesRec rm = (do_meas && sd.e) ? SeriesRec{ c->d_ctr + 1, nm, sd.n_meas, sd.e, sd.m, nullptr } : kNoRecord;
         // Measurements ride on the sweep launches. Before a swap, whose
         // exponent takes FP64 per-site energies, the half sweep's colour-1
-        // launch computes them (the fused sweep leaves them to the measure
-        // kernel); the swap kernel then records E and M at the new
-        // temperatures.
-        const bool exact = do_swap && !fused_active(c);
+        // launch computes them (the fused sweep and three-colour sweeps leave
+        // them to the measure kernel); the swap kernel then records E and M
+        // at the new temperatures.
+        const bool exact = do_swap && !fused_active(c) && c->max_ncol == 2;
         if ((rc = launch_sweep(c, (uint32_t)k, (do_meas && !do_swap) || exact, do_swap ? kNoRecord : rm, exact, timer)))
             return rc;
         if (do_swap && !exact)
@@ -2375,11 +2692,13 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
             memset(&r, 0, sizeof r);
             r.lx = L.lat.lx; r.ly = L.lat.ly; r.lz = L.lat.lz;
             r.cell = L.lat.cell;
-            r.hx = r.cell == ISING_CELL_BCC ? r.lx : r.lx / 2;
+            r.hx = r.cell == ISING_CELL_SC ? r.lx / 2 : r.lx;
             r.per = (L.lat.periodic[0] ? 1 : 0) | (L.lat.periodic[1] ? 2 : 0) | (L.lat.periodic[2] ? 4 : 0);
             r.ladder = l;
-            r.n = (int64_t)r.lx * r.ly * r.lz * (r.cell == ISING_CELL_BCC ? 2 : 1);
-            r.half = (uint32_t)(r.n / 2);
+            r.n = (int64_t)r.lx * r.ly * r.lz * atoms_of(r.cell);
+            r.ncol = r.cell == ISING_CELL_HYPERKAGOME ? kHkAtoms / kHkPerColour : 2;
+            r.half = (uint32_t)(r.n / r.ncol);
+            c->max_ncol = std::max(c->max_ncol, r.ncol);
             r.off = off;
             off += 3 * r.n;
             off = (off + 63) & ~(int64_t)63;      // 256-byte aligned replica blocks
@@ -2394,6 +2713,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
             r.div_hx = make_fastdiv((uint32_t)r.hx);
             r.div_ly = make_fastdiv((uint32_t)r.ly);
             r.div_lx = make_fastdiv((uint32_t)r.lx);
+            r.div_lz = make_fastdiv((uint32_t)r.lz);
             zmax = std::max(zmax, r.lz);
             c->reps.push_back(r);
             for (int b = 0; b < r.nblocks; ++b) c->block_rep.push_back((int32_t)c->reps.size() - 1);
@@ -2416,6 +2736,12 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if (hipSetDevice(device) != hipSuccess) return fail(set_err(c, ISING_ERR_NO_DEVICE, "hipSetDevice(%d)", device));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(c, ISING_ERR_HIP, "hipStreamCreate"));
+    if (c->max_ncol > 2) {      // hyper-kagome ladders: the cell table in this device's constant memory
+        const HkTable* T = hk_table();
+        if (!T) return fail(set_err(c, ISING_ERR_STATE, "hyper-kagome table construction failed"));
+        if (hipMemcpyToSymbol(HIP_SYMBOL(c_hk), T, sizeof(HkTable), 0, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(c, ISING_ERR_HIP, "upload of the hyper-kagome table"));
+    }
 
     if ((rc = dalloc(c, &c->d_spins, 2 * (size_t)c->total_floats))) return fail(rc);
     if ((rc = dalloc(c, &c->d_reps, c->reps.size()))) return fail(rc);
@@ -2687,7 +3013,7 @@ int ising_energy_with(ising_ctx* c, int replica, const ising_hamiltonian* h, dou
                        (const double*)c->d_demag);
     LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_measure_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->d_reps, c->d_partial, replica,
-                       c->d_energy, c->d_mag);
+                       c->d_energy, c->d_mag, (const int32_t*)nullptr, 0, kNoRecord);
     LAUNCH_CHECK(c);
     HIP_TRY(c, hipMemcpyAsync(e, c->d_energy + replica, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -2943,7 +3269,7 @@ int ising_time_sweeps(ising_ctx* c, int n_sweeps, double* ms_per_launch)
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     advance_host(c, (uint32_t)n_sweeps);
-    *ms_per_launch = (double)ms / (2.0 * n_sweeps);
+    *ms_per_launch = (double)ms / ((double)c->max_ncol * n_sweeps);
     return ISING_OK;
 }
 

@@ -42,24 +42,28 @@ class Lattice:
     ly: int
     lz: int
     periodic: tuple = (True, True, True)
-    cell: str = "sc"          # "sc" or "bcc" (two atoms per cell)
+    cell: str = "sc"          # "sc", "bcc" (two atoms per cell) or "hyperkagome" (twelve)
 
     @property
     def atoms_per_cell(self) -> int:
-        return 2 if self.cell == "bcc" else 1
+        return _ATOMS[self.cell]
 
     @property
     def sites(self) -> int:
         return self.lx * self.ly * self.lz * self.atoms_per_cell
 
     def _c(self) -> _capi.ising_lattice:
-        if self.cell not in ("sc", "bcc"):
-            raise ValueError("cell must be 'sc' or 'bcc'")
+        if self.cell not in _ATOMS:
+            raise ValueError("cell must be 'sc', 'bcc' or 'hyperkagome'")
         lat = _capi.ising_lattice(self.lx, self.ly, self.lz)
         for a in range(3):
             lat.periodic[a] = 1 if self.periodic[a] else 0
-        lat.cell = _capi.CELL_BCC if self.cell == "bcc" else _capi.CELL_SC
+        lat.cell = _CELLS[self.cell]
         return lat
+
+
+_ATOMS = {"sc": 1, "bcc": 2, "hyperkagome": 12}
+_CELLS = {"sc": _capi.CELL_SC, "bcc": _capi.CELL_BCC, "hyperkagome": _capi.CELL_HYPERKAGOME}
 
 
 @dataclass

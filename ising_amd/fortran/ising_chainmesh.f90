@@ -1,10 +1,11 @@
-! ising_chainmesh.f90 -- drop-in ChainMesh module for simple-cubic and bcc lattices.
+! ising_chainmesh.f90 -- drop-in ChainMesh module for simple-cubic, bcc and
+! hyper-kagome lattices.
 !
 ! Keeps the fields of the reference's ChainMesh_t that the Monte Carlo path
 ! and the PT driver read (src/chainMesh.f90:17-47): numCellsX/Y/Z, numAtoms
 ! and atomSpins(numAtoms,3) real(8) in the reference's site order
-! (IndexFromCoordinatesInteger, src/chainMesh.f90:673-685: z fastest; with a
-! two-atom bcc cell, atom a of cell c is site 2*(c-1) + a, src/chainMesh.f90:878).
+! (IndexFromCoordinatesInteger, src/chainMesh.f90:673-685: z fastest; with n
+! atoms per cell, atom a of cell c is site n*(c-1) + a, src/chainMesh.f90:878).
 ! Each mesh owns one device context (one replica) created on first use;
 ! the spins stay host-owned and are copied in and out per call, exactly as
 ! the reference's routines mutate chainMesh%atomSpins in place.
@@ -17,7 +18,7 @@ module ChainMesh
                 integer :: numCellsX = 0, numCellsY = 0, numCellsZ = 0
                 integer :: numAtoms = 0
                 logical :: periodic(3) = [.true., .true., .true.]
-                integer :: cell = 0           ! ISING_CELL_SC or ISING_CELL_BCC
+                integer :: cell = 0           ! ISING_CELL_SC, ISING_CELL_BCC or ISING_CELL_HYPERKAGOME
                 integer :: atomsPerCell = 1
                 real(kind=8) :: a = 1.0_8     ! lattice constant (positions in output files)
                 real(kind=8), allocatable :: atomSpins(:,:)
@@ -39,9 +40,11 @@ module ChainMesh
 contains
 
         ! Mesh of numCellsX x numCellsY x numCellsZ cubic cells, one atom per
-        ! cell (SC) or two (cell = ISING_CELL_BCC: atoms at (0,0,0) and
-        ! (1/2,1/2,1/2), src/unit_cells.f90:7-30). The reference builds general
-        ! Bravais cells (src/chainMesh.f90:729-911); these two are provided.
+        ! cell (SC), two (cell = ISING_CELL_BCC: atoms at (0,0,0) and
+        ! (1/2,1/2,1/2), src/unit_cells.f90:7-30) or twelve (cell =
+        ! ISING_CELL_HYPERKAGOME, src/unit_cells.f90:32-77). The reference
+        ! builds general Bravais cells (src/chainMesh.f90:729-911); these three
+        ! are provided.
         function makeChainMesh(numCellsX, numCellsY, numCellsZ, periodic, cell) result(mesh)
                 integer, intent(in) :: numCellsX, numCellsY, numCellsZ
                 logical, intent(in), optional :: periodic(3)
@@ -52,7 +55,14 @@ contains
                 mesh%numCellsY = numCellsY
                 mesh%numCellsZ = numCellsZ
                 if (present(cell)) mesh%cell = cell
-                mesh%atomsPerCell = merge(2, 1, mesh%cell == ISING_CELL_BCC)
+                select case (mesh%cell)
+                case (ISING_CELL_BCC)
+                        mesh%atomsPerCell = 2
+                case (ISING_CELL_HYPERKAGOME)
+                        mesh%atomsPerCell = 12
+                case default
+                        mesh%atomsPerCell = 1
+                end select
                 mesh%numAtoms = numCellsX * numCellsY * numCellsZ * mesh%atomsPerCell
                 if (present(periodic)) mesh%periodic = periodic
                 allocate(mesh%atomSpins(mesh%numAtoms, 3), stat=stat)

@@ -35,6 +35,7 @@ module ising_gpu_c
 
         integer(c_int32_t), parameter :: ISING_CELL_SC = 0
         integer(c_int32_t), parameter :: ISING_CELL_BCC = 1
+        integer(c_int32_t), parameter :: ISING_CELL_HYPERKAGOME = 2
 
         type, bind(C) :: ising_lattice
                 integer(c_int32_t) :: lx, ly, lz

@@ -11,7 +11,7 @@ program test_dropin
         use PT_Utils
         use reciprocal_space_processes
         implicit none
-        type(ChainMesh_t) :: mesh, fm, dm
+        type(ChainMesh_t) :: mesh, fm, dm, hk
         integer(kind=OMP_LOCK_KIND), allocatable :: lockArray(:)
         real(kind=8) :: e0, e1, eold, enew, winding_middle, winding_spread, m(3), sp(3), s0(3)
         real(kind=8) :: J, Dz, B, beta, pi
@@ -127,8 +127,23 @@ program test_dropin
         call calculate_demagnetisation_field(dm, hfield)
         print '(A,ES24.16)', "demag_after_sweeps ", maxval(abs(dm%demagnetisation_array - hfield))
         print '(A,ES24.16)', "demag_norm_dev ", maxval(abs(sqrt(sum(dm%atomSpins**2, dim=2)) - 1.0_8))
+        ! Hyper-kagome mesh 3 x 3 x 2 (the lattice of src/ISING.f90:105): a
+        ! ferromagnet along z has 2 bonds per atom, E = (2 J - 2 B) numAtoms in
+        ! the reference's convention; its layer charge is zero; sweeps keep
+        ! unit spins.
+        hk = makeChainMesh(3, 3, 2, cell=ISING_CELL_HYPERKAGOME)
+        hk%atomSpins(:, 1) = 0.0_8
+        hk%atomSpins(:, 2) = 0.0_8
+        hk%atomSpins(:, 3) = 1.0_8
+        call TotalHeisenbergEnergy(hk, J, 0.0_8, Dz, 0.0_8, B, lockArray, e0, .false.)
+        print '(A,2ES24.16)', "hk_fm_energy ", e0, (2.0_8 * J - 2.0_8 * B) * hk%numAtoms
+        print '(A,ES24.16)', "hk_fm_winding ", calculate_winding_number2(hk, 1)
+        call Metropolis_mcs(hk, beta, 5, J, 0.0_8, Dz, 0.0_8, B, 0.5_8, lockArray, demag=.false.)
+        print '(A,ES24.16)', "hk_norm_dev ", maxval(abs(sqrt(sum(hk%atomSpins**2, dim=2)) - 1.0_8))
+        print '(A,I0)', "hk_atoms ", hk%numAtoms
         call deallocateChainMesh(mesh)
         call deallocateChainMesh(fm)
         call deallocateChainMesh(dm)
+        call deallocateChainMesh(hk)
         print '(A)', "done"
 end program test_dropin

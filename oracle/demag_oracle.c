@@ -60,14 +60,18 @@ static inline double min_image(double d, int n)
 }
 
 /* Cloud-in-cell weight of atom `a` of cell (ni, nj, nk) at the mesh point
- * (centre) of cell (ci, cj, ck); atom positions in lattice units, SC atom at
- * the cell corner, bcc atoms at the corner and the body centre. */
+ * (centre) of cell (ci, cj, ck); atom positions in lattice units, the cell
+ * corner plus the atom's fraction of the cell (orc_atom_frac). An atom that
+ * lies outside its own cell (hyper-kagome fractions down to -3/4) still
+ * deposits only on the 27 cells around the cell it belongs to, as the
+ * reference's loop over getNeighboringCells does. */
 static double cic_weight(const orc_lattice* L, int ci, int cj, int ck, int ni, int nj, int nk, int a)
 {
-    const double f = (L->cell == ORC_CELL_BCC && a == 1) ? 0.5 : 0.0;
-    const double dx = min_image(((double)ni + f) - ((double)ci + 0.5), L->lx);
-    const double dy = min_image(((double)nj + f) - ((double)cj + 0.5), L->ly);
-    const double dz = min_image(((double)nk + f) - ((double)ck + 0.5), L->lz);
+    double f[3];
+    orc_atom_frac(L, a, f);
+    const double dx = min_image(((double)ni + f[0]) - ((double)ci + 0.5), L->lx);
+    const double dy = min_image(((double)nj + f[1]) - ((double)cj + 0.5), L->ly);
+    const double dz = min_image(((double)nk + f[2]) - ((double)ck + 0.5), L->lz);
     const double wx = fmax(1.0 - fabs(dx), 0.0);
     const double wy = fmax(1.0 - fabs(dy), 0.0);
     const double wz = fmax(1.0 - fabs(dz), 0.0);
@@ -83,7 +87,7 @@ static inline double complex twiddle(int64_t m, int n, int sign)
 
 void orc_demag_deposit(const orc_lattice* L, const double* s, double* mesh)
 {
-    const int nx = L->lx, ny = L->ly, nz = L->lz, na = L->cell == ORC_CELL_BCC ? 2 : 1;
+    const int nx = L->lx, ny = L->ly, nz = L->lz, na = orc_atoms_per_cell(L);
     const int64_t Nc = (int64_t)nx * ny * nz, N = Nc * na;
     memset(mesh, 0, 3 * (size_t)Nc * sizeof(double));
     for (int i = 0; i < nx; ++i)
@@ -193,7 +197,7 @@ void orc_demag_mesh_field(const orc_lattice* L, const double* mesh, double* hmes
 
 void orc_demag_gather(const orc_lattice* L, const double* hmesh, double* H)
 {
-    const int nx = L->lx, ny = L->ly, nz = L->lz, na = L->cell == ORC_CELL_BCC ? 2 : 1;
+    const int nx = L->lx, ny = L->ly, nz = L->lz, na = orc_atoms_per_cell(L);
     const int64_t Nc = (int64_t)nx * ny * nz, N = Nc * na;
     memset(H, 0, 3 * (size_t)N * sizeof(double));
     for (int i = 0; i < nx; ++i)

@@ -10,6 +10,15 @@
  *   orc_ref_uniform         reference rand.f90 built into oracle/_ref (tests/test_oracle_ref.py)
  *   orc_neighbour           brute-force enumeration on tiny lattices (tests/test_oracle_geometry.py)
  *   orc_bcc_bonds           brute-force distance enumeration of bcc positions (tests/test_oracle_bcc.py)
+ *   orc_hk_bonds            brute-force minimum-image distance enumeration over the whole
+ *                           periodic lattice; four bonds per site, each bond seen from both
+ *                           ends (tests/test_oracle_hyperkagome.py)
+ *   orc_site_colour         no bond joins two sites of one colour (same tests)
+ *   hyper-kagome energies   closed forms (ferromagnet, uniform rotation of the bond sum),
+ *                           dE = E(after) - E(before), dE linear in the proposal
+ *   orc_plaquette           the reference's firstAtomInMeshCell list order and
+ *                           computeAdjacentAtoms restated on atom positions (bcc,
+ *                           hyper-kagome)
  *   bcc energies            closed forms (ferromagnet, Neel, bulk and interfacial helices),
  *                           ref restatement in reference mode (tests/test_oracle_bcc.py)
  *   orc_total_energy        closed forms: ferromagnet, bulk helix (tests/test_oracle_energy.py)
@@ -179,11 +188,148 @@ int orc_bcc_bonds(const orc_lattice* L, int64_t site, int64_t nb[8], int sgn[8][
     return k;
 }
 
-/* Colour of a site in the two-colour sweep: (ix+iy+iz) mod 2 on SC, the atom
- * of the cell on bcc (both nearest-neighbour graphs are bipartite). */
-static inline int site_colour(const orc_lattice* L, int64_t i)
+/* ---- hyper-kagome cell ---------------------------------------------- */
+
+/* hyper_kagome_unit_cell (src/unit_cells.f90:32-77): a = b = c = 6, right
+ * angles, twelve atoms at these fractions of the cell edges (every value is
+ * exact in binary). makeChainMesh places atom i of cell (ix, iy, iz) at
+ * (ix, iy, iz) + frac_i in units of the edge (src/chainMesh.f90:873-880). */
+#define HK_ATOMS 12
+#define HK_MAXB 12
+static const double hk_frac[HK_ATOMS][3] = {
+    { -0.5, 0.0, 0.5 },   { -0.25, 0.75, 0.5 }, { -0.5, 0.75, 0.25 }, { -0.25, 0.25, 0.0 },
+    { -0.5, 0.25, 0.75 }, { -0.25, 0.5, 0.75 }, { -0.75, 0.5, 0.25 }, { 0.0, 0.5, 0.5 },
+    { 0.0, 0.25, 0.25 },  { -0.75, 0.75, 0.0 }, { 0.0, 0.0, 0.0 },    { -0.75, 0.0, 0.75 } };
+
+typedef struct { int u; int shift[3]; double r[3]; } hk_bond;
+static hk_bond hk_bonds[HK_ATOMS][HK_MAXB];
+static int hk_nb[HK_ATOMS];
+static int hk_col[HK_ATOMS], hk_ncol;
+
+/* Proper colouring of the atom types' bond graph with the fewest colours:
+ * types in order, each the smallest colour that no earlier bonded type has,
+ * backtracking when none is left (the lexicographically first colouring). */
+static int hk_colour_from(int t, int k, const int adj[HK_ATOMS][HK_ATOMS])
+{
+    if (t == HK_ATOMS) return 1;
+    for (int c = 0; c < k; ++c) {
+        int ok = 1;
+        for (int u = 0; u < t && ok; ++u)
+            if (adj[t][u] && hk_col[u] == c) ok = 0;
+        if (!ok) continue;
+        hk_col[t] = c;
+        if (hk_colour_from(t + 1, k, adj)) return 1;
+    }
+    return 0;
+}
+
+/* First neighbour shell of every atom, as initAtomShells
+ * (src/chainMesh.f90:495-591) forms it with makeChainMesh's threshold 0.1 a
+ * (:783): every atom of the cells within two cells, its distance, sorted;
+ * the shell is the nearest atom and each next one whose distance exceeds
+ * the previous by at most the threshold. Bond vectors are the displacements
+ * (site -> neighbour) divided by their length. */
+__attribute__((constructor)) static void hk_build(void)
+{
+    const double a = 6.0, thr = 0.1 * a;
+    enum { NC = HK_ATOMS * 125 };
+    static double dist[NC];
+    static int cand[NC][4];
+    int adj[HK_ATOMS][HK_ATOMS];
+    memset(adj, 0, sizeof adj);
+    for (int t = 0; t < HK_ATOMS; ++t) {
+        int n = 0;
+        for (int u = 0; u < HK_ATOMS; ++u)
+            for (int sx = -2; sx <= 2; ++sx)
+                for (int sy = -2; sy <= 2; ++sy)
+                    for (int sz = -2; sz <= 2; ++sz) {
+                        if (u == t && sx == 0 && sy == 0 && sz == 0) continue;
+                        const double d[3] = { (hk_frac[u][0] + sx - hk_frac[t][0]) * a,
+                                              (hk_frac[u][1] + sy - hk_frac[t][1]) * a,
+                                              (hk_frac[u][2] + sz - hk_frac[t][2]) * a };
+                        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                        /* insertion into the sorted list */
+                        int j = n++;
+                        while (j > 0 && dist[j - 1] > r) {
+                            dist[j] = dist[j - 1];
+                            memcpy(cand[j], cand[j - 1], sizeof cand[j]);
+                            --j;
+                        }
+                        dist[j] = r;
+                        cand[j][0] = u; cand[j][1] = sx; cand[j][2] = sy; cand[j][3] = sz;
+                    }
+        int m = 0;
+        for (int j = 0; j < n && m < HK_MAXB; ++j) {
+            if (j > 0 && fabs(dist[j] - dist[j - 1]) > thr) break;
+            hk_bond* b = &hk_bonds[t][m++];
+            b->u = cand[j][0];
+            for (int k = 0; k < 3; ++k) {
+                b->shift[k] = cand[j][1 + k];
+                b->r[k] = (hk_frac[b->u][k] + b->shift[k] - hk_frac[t][k]) * a / dist[j];
+            }
+            adj[t][b->u] = adj[b->u][t] = 1;
+        }
+        hk_nb[t] = m;
+    }
+    for (hk_ncol = 1; hk_ncol <= HK_ATOMS; ++hk_ncol)
+        if (hk_colour_from(0, hk_ncol, adj)) break;
+}
+
+int orc_atoms_per_cell(const orc_lattice* L)
+{
+    return L->cell == ORC_CELL_BCC ? 2 : L->cell == ORC_CELL_HYPERKAGOME ? HK_ATOMS : 1;
+}
+
+/* Position of atom `atom` in its cell, in units of the cell edge. */
+void orc_atom_frac(const orc_lattice* L, int atom, double f[3])
+{
+    for (int k = 0; k < 3; ++k)
+        f[k] = L->cell == ORC_CELL_HYPERKAGOME ? hk_frac[atom][k] : (L->cell == ORC_CELL_BCC && atom == 1 ? 0.5 : 0.0);
+}
+
+/* Site (cell*natoms + atom) of atom `atom` of cell (ix, iy, iz), the
+ * reference order (src/chainMesh.f90:684,878). */
+int64_t orc_cell_site(const orc_lattice* L, int ix, int iy, int iz, int atom)
+{
+    return (((int64_t)ix * L->ly + iy) * L->lz + iz) * orc_atoms_per_cell(L) + atom;
+}
+
+/* Nearest neighbours of a hyper-kagome site and the unit bond vectors
+ * (site -> neighbour); bonds across open faces are left out. */
+int orc_hk_bonds(const orc_lattice* L, int64_t site, int64_t* nb, double (*r)[3])
+{
+    const int t = (int)(site % HK_ATOMS);
+    int64_t cs = site / HK_ATOMS;
+    const int c[3] = { (int)(cs / ((int64_t)L->ly * L->lz)), (int)((cs / L->lz) % L->ly), (int)(cs % L->lz) };
+    const int n[3] = { L->lx, L->ly, L->lz };
+    int k = 0;
+    for (int b = 0; b < hk_nb[t]; ++b) {
+        const hk_bond* B = &hk_bonds[t][b];
+        int cc[3], ok = 1;
+        for (int ax = 0; ax < 3 && ok; ++ax) {
+            cc[ax] = c[ax] + B->shift[ax];
+            if (cc[ax] < 0 || cc[ax] >= n[ax]) {
+                if (!L->periodic[ax]) ok = 0;
+                else cc[ax] = ((cc[ax] % n[ax]) + n[ax]) % n[ax];
+            }
+        }
+        if (!ok) continue;
+        nb[k] = orc_cell_site(L, cc[0], cc[1], cc[2], B->u);
+        for (int ax = 0; ax < 3; ++ax) r[k][ax] = B->r[ax];
+        ++k;
+    }
+    return k;
+}
+
+int orc_num_colours(const orc_lattice* L) { return L->cell == ORC_CELL_HYPERKAGOME ? hk_ncol : 2; }
+
+/* Colour of a site in the sweep: (ix+iy+iz) mod 2 on SC, the atom of the cell
+ * on bcc (both nearest-neighbour graphs are bipartite), the colour of the
+ * atom type on hyper-kagome (its triangles need three). */
+int orc_site_colour(const orc_lattice* L, int64_t i)
 {
     if (L->cell == ORC_CELL_BCC) return (int)(i & 1);
+    if (L->cell == ORC_CELL_HYPERKAGOME) return hk_col[i % HK_ATOMS];
     int64_t iz = i % L->lz, r = i / L->lz;
     int64_t iy = r % L->ly, ix = r / L->ly;
     return (int)((ix + iy + iz) & 1);
@@ -219,7 +365,7 @@ static inline void load(const double* s, int64_t N, int64_t i, double v[3])
 }
 static inline int64_t nsites(const orc_lattice* L)
 {
-    return (int64_t)L->lx * L->ly * L->lz * (L->cell == ORC_CELL_BCC ? 2 : 1);
+    return (int64_t)L->lx * L->ly * L->lz * orc_atoms_per_cell(L);
 }
 int64_t orc_nsites(const orc_lattice* L) { return nsites(L); }
 
@@ -257,9 +403,40 @@ static void bcc_local_field(const orc_lattice* L, const orc_ham* h, const double
     out[2] += h->Bz;
 }
 
+/* DMI vector of a bond with unit vector r: d_b = sum_a r_a d_a (bulk D r,
+ * reference interfacial D (z x r_{j->i})), the form bcc_bond_dmi takes for
+ * r = sgn/sqrt 3. */
+static void bond_dmi(const double d[3][3], const double r[3], double db[3])
+{
+    for (int m = 0; m < 3; ++m) db[m] = r[0] * d[0][m] + r[1] * d[1][m] + r[2] * d[2][m];
+}
+
+/* hyper-kagome: h_i = J sum_b S_b - sum_b S_b x d_b + Bz z over the first
+ * shell (four bonds). */
+static void hk_local_field(const orc_lattice* L, const orc_ham* h, const double* s, int64_t site, double out[3])
+{
+    const int64_t N = nsites(L);
+    double d[3][3];
+    orc_dmi_vectors(h, d);
+    int64_t nb[HK_MAXB];
+    double r[HK_MAXB][3];
+    const int nbonds = orc_hk_bonds(L, site, nb, r);
+    double sum[3] = { 0, 0, 0 }, dm[3] = { 0, 0, 0 };
+    for (int k = 0; k < nbonds; ++k) {
+        double sj[3], db[3], c[3];
+        load(s, N, nb[k], sj);
+        bond_dmi(d, r[k], db);
+        cross(sj, db, c);
+        for (int m = 0; m < 3; ++m) { sum[m] += sj[m]; dm[m] += c[m]; }
+    }
+    for (int m = 0; m < 3; ++m) out[m] = h->J * sum[m] - dm[m];
+    out[2] += h->Bz;
+}
+
 void orc_local_field(const orc_lattice* L, const orc_ham* h, const double* s, int64_t site, double out[3])
 {
     if (L->cell == ORC_CELL_BCC) { bcc_local_field(L, h, s, site, out); return; }
+    if (L->cell == ORC_CELL_HYPERKAGOME) { hk_local_field(L, h, s, site, out); return; }
     const int64_t N = nsites(L);
     double d[3][3];
     orc_dmi_vectors(h, d);
@@ -318,9 +495,39 @@ static double bcc_total_energy(const orc_lattice* L, const orc_ham* h, const dou
     return e;
 }
 
+/* hyper-kagome: every bond once, from the end whose atom comes first in the
+ * cell (bonds never join two atoms of one type), plus the on-site terms. */
+static double hk_total_energy(const orc_lattice* L, const orc_ham* h, const double* s)
+{
+    const int64_t N = nsites(L);
+    double d[3][3];
+    orc_dmi_vectors(h, d);
+    double e = 0.0;
+    #pragma omp parallel for reduction(+:e) schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        double si[3], sj[3], c[3], db[3];
+        load(s, N, i, si);
+        double ei = 0.0;
+        int64_t nb[HK_MAXB];
+        double r[HK_MAXB][3];
+        const int nbonds = orc_hk_bonds(L, i, nb, r);
+        for (int k = 0; k < nbonds; ++k) {
+            if (nb[k] % HK_ATOMS < i % HK_ATOMS) continue;
+            load(s, N, nb[k], sj);
+            cross(si, sj, c);
+            bond_dmi(d, r[k], db);
+            ei += -h->J * dot(si, sj) + dot(db, c);
+        }
+        ei -= h->zeeman_total_scale * h->Bz * si[2] + h->Kz * si[2] * si[2];
+        e += ei;
+    }
+    return e;
+}
+
 double orc_total_energy(const orc_lattice* L, const orc_ham* h, const double* s)
 {
     if (L->cell == ORC_CELL_BCC) return bcc_total_energy(L, h, s);
+    if (L->cell == ORC_CELL_HYPERKAGOME) return hk_total_energy(L, h, s);
     const int64_t N = nsites(L);
     double d[3][3];
     orc_dmi_vectors(h, d);
@@ -352,24 +559,26 @@ void orc_magnetisation(const orc_lattice* L, const double* s, double m[3])
 }
 
 /* Corners (i,j), (i+1,j), (i,j+1), (i+1,j+1) of the plaquette of layer z:
- * the atom-0 sites of those cells on bcc (the reference's derivativeList
- * steps to the same atom of the next cell, src/chainMesh.f90:931-1031);
- * returns 0 when a corner lies across an open face. ids = {s1, s2, s3, s4}
+ * one atom of each of those cells, the cell's firstAtomInMeshCell. makeChainMesh
+ * adds the atoms of a cell in order and addAtomToChainCell puts each new one
+ * at the head of the cell's list (src/chainMesh.f90:663-671,873-881), so that
+ * atom is the LAST of the unit cell: the body-centre atom on bcc, atom 12 on
+ * hyper-kagome. derivativeList then steps to the same atom of the next cell
+ * (src/chainMesh.f90:931-1031; no other atom lies on its x or y line).
+ * Returns 0 when a corner lies across an open face. ids = {s1, s2, s3, s4}
  * with s2 = +x, s3 = +y, s4 = +x+y. */
 static int plaquette(const orc_lattice* L, int i, int j, int z, int64_t ids[4])
 {
     int i2 = i + 1, j2 = j + 1;
     if (i2 == L->lx) { if (!L->periodic[0]) return 0; i2 = 0; }
     if (j2 == L->ly) { if (!L->periodic[1]) return 0; j2 = 0; }
-    if (L->cell == ORC_CELL_BCC) {
-        ids[0] = orc_bcc_site(L, i, j, z, 0); ids[1] = orc_bcc_site(L, i2, j, z, 0);
-        ids[2] = orc_bcc_site(L, i, j2, z, 0); ids[3] = orc_bcc_site(L, i2, j2, z, 0);
-    } else {
-        ids[0] = orc_site(L, i, j, z); ids[1] = orc_site(L, i2, j, z);
-        ids[2] = orc_site(L, i, j2, z); ids[3] = orc_site(L, i2, j2, z);
-    }
+    const int a = orc_atoms_per_cell(L) - 1;
+    ids[0] = orc_cell_site(L, i, j, z, a); ids[1] = orc_cell_site(L, i2, j, z, a);
+    ids[2] = orc_cell_site(L, i, j2, z, a); ids[3] = orc_cell_site(L, i2, j2, z, a);
     return 1;
 }
+
+int orc_plaquette(const orc_lattice* L, int i, int j, int z, int64_t ids[4]) { return plaquette(L, i, j, z, ids); }
 
 /* ---- reference restatements ----------------------------------------- */
 
@@ -385,24 +594,30 @@ static void ref_min_image(const orc_lattice* L, const double p1[3], const double
     }
 }
 
-/* Atom position in cell units (the bcc body atom at +1/2 on every axis,
- * src/unit_cells.f90:27-28). */
+/* Atom position in cell units: the cell corner plus the atom's fraction
+ * (src/unit_cells.f90; makeChainMesh, src/chainMesh.f90:873-880). */
 static void ref_position(const orc_lattice* L, int64_t site, double p[3])
 {
-    int c[3], a = 0;
-    if (L->cell == ORC_CELL_BCC) bcc_decode(L, site, c, &a);
-    else decode(L, site, c);
-    for (int k = 0; k < 3; ++k) p[k] = c[k] + 0.5 * a;
+    const int na = orc_atoms_per_cell(L);
+    int c[3];
+    double f[3];
+    decode(L, site / na, c);
+    orc_atom_frac(L, (int)(site % na), f);
+    for (int k = 0; k < 3; ++k) p[k] = c[k] + f[k];
 }
 
 /* Nearest-neighbour list of a site of a periodic lattice, as the reference's
  * atomShells(i,1) holds it (src/chainMesh.f90:495-591): 6 on SC (-x, +x,
- * -y, +y, -z, +z), 8 on bcc. */
-static int ref_nn(const orc_lattice* L, int64_t site, int64_t nb[8])
+ * -y, +y, -z, +z), 8 on bcc, 4 on hyper-kagome. */
+static int ref_nn(const orc_lattice* L, int64_t site, int64_t nb[HK_MAXB])
 {
     if (L->cell == ORC_CELL_BCC) {
         int sg[8][3];
         return orc_bcc_bonds(L, site, nb, sg);
+    }
+    if (L->cell == ORC_CELL_HYPERKAGOME) {
+        double r[HK_MAXB][3];
+        return orc_hk_bonds(L, site, nb, r);
     }
     int k = 0;
     for (int a = 0; a < 3; ++a)
@@ -410,8 +625,8 @@ static int ref_nn(const orc_lattice* L, int64_t site, int64_t nb[8])
     return k;
 }
 
-/* calculateHeisenbergEnergy (src/energyMin.f90:86-171) for a periodic SC or
- * bcc lattice with J' = 0 and demag off: old/new local energy of site i for
+/* calculateHeisenbergEnergy (src/energyMin.f90:86-171) for a periodic SC,
+ * bcc or hyper-kagome lattice with J' = 0 and demag off: old/new local energy of site i for
  * proposal sp. For each nearest neighbour j: r = -(x_j - x_i)_min/|.|,
  * D = Dz (c x r), energy += J S.S_j + D.(S x S_j); then -B S_z. */
 void orc_ref_local_energy(const orc_lattice* L, double J_r, double Dz, double B, const double* s,
@@ -424,7 +639,7 @@ void orc_ref_local_energy(const orc_lattice* L, double J_r, double Dz, double B,
     ref_position(L, site, p1);
     double tv[3] = { Dz * chat[0], Dz * chat[1], Dz * chat[2] };
     double oe = 0.0, ne = 0.0;
-    int64_t nb[8];
+    int64_t nb[HK_MAXB];
     const int nn = ref_nn(L, site, nb);
     for (int k = 0; k < nn; ++k) {
         const int64_t j = nb[k];
@@ -457,7 +672,7 @@ double orc_ref_total_energy(const orc_lattice* L, double J_r, double Dz, double 
         const double tv[3] = { 0.0, 0.0, Dz };
         load(s, N, i, S);
         ref_position(L, i, p1);
-        int64_t nb[8];
+        int64_t nb[HK_MAXB];
         const int nn = ref_nn(L, i, nb);
         for (int k = 0; k < nn; ++k) {
             const int64_t j = nb[k];
@@ -657,12 +872,11 @@ int orc_propose(const double so[3], double radius, const uint32_t rnd[4], double
     return 1;
 }
 
-/* One colour of the exact checkerboard sweep: every site with
- * (ix+iy+iz) % 2 == colour gets one proposal, accepted iff u < exp(-beta dE).
- * Same-colour sites share no bond, so the order inside a colour is free. */
-/* One colour of one sweep; H (may be NULL) is an external per-site field
- * entering dE as -mu0 (S' - S).H_i, the reference's demagnetisation term
- * (src/energyMin.f90:113-116,163-166). */
+/* One colour of the exact sweep: every site of that colour (orc_site_colour)
+ * gets one proposal, accepted iff u < exp(-beta dE). Same-colour sites share
+ * no bond, so the order inside a colour is free. H (may be NULL) is an
+ * external per-site field entering dE as -mu0 (S' - S).H_i, the reference's
+ * demagnetisation term (src/energyMin.f90:113-116,163-166). */
 static int64_t half_sweep_field(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
                                 uint64_t seed, uint32_t slot, uint32_t sweep, int colour, const double* H, double mu0)
 {
@@ -670,7 +884,7 @@ static int64_t half_sweep_field(const orc_lattice* L, const orc_ham* h, double* 
     int64_t acc = 0;
     #pragma omp parallel for reduction(+:acc) schedule(static)
     for (int64_t i = 0; i < N; ++i) {
-        if (site_colour(L, i) != colour) continue;
+        if (orc_site_colour(L, i) != colour) continue;
         uint32_t rnd[4];
         orc_metropolis_random(seed, slot, (uint32_t)i, sweep, rnd);
         double so[3], sn[3], u;
@@ -695,10 +909,9 @@ int64_t orc_sweeps(const orc_lattice* L, const orc_ham* h, double* s, double bet
                    uint64_t seed, uint32_t slot, uint32_t sweep0, int n_sweeps)
 {
     int64_t acc = 0;
-    for (int k = 0; k < n_sweeps; ++k) {
-        acc += orc_half_sweep(L, h, s, beta, radius, seed, slot, sweep0 + (uint32_t)k, 0);
-        acc += orc_half_sweep(L, h, s, beta, radius, seed, slot, sweep0 + (uint32_t)k, 1);
-    }
+    for (int k = 0; k < n_sweeps; ++k)
+        for (int c = 0; c < orc_num_colours(L); ++c)
+            acc += orc_half_sweep(L, h, s, beta, radius, seed, slot, sweep0 + (uint32_t)k, c);
     return acc;
 }
 
@@ -724,8 +937,8 @@ int64_t orc_sweeps_demag(const orc_lattice* L, const orc_ham* h, double* s, doub
     if (orc_ref_demag_field(L, s, H)) return -1;
     int64_t acc = 0;
     for (int k = 0; k < n_sweeps; ++k) {
-        acc += half_sweep_field(L, h, s, beta, radius, seed, slot, sweep0 + (uint32_t)k, 0, H, mu0);
-        acc += half_sweep_field(L, h, s, beta, radius, seed, slot, sweep0 + (uint32_t)k, 1, H, mu0);
+        for (int c = 0; c < orc_num_colours(L); ++c)
+            acc += half_sweep_field(L, h, s, beta, radius, seed, slot, sweep0 + (uint32_t)k, c, H, mu0);
         if (interval > 0 && (k + 1) % interval == 0) orc_ref_demag_field(L, s, H);
     }
     return acc;

@@ -23,12 +23,14 @@
 #include <stdint.h>
 
 /* cell: ORC_CELL_SC, one atom per cell; ORC_CELL_BCC, atoms at (0,0,0) and
- * (1/2,1/2,1/2) of each cell (src/unit_cells.f90:7-30), site index
- * ((ix*Ly + iy)*Lz + iz)*2 + atom, the reference order for two-atom cells
- * (src/chainMesh.f90:684,878). */
+ * (1/2,1/2,1/2) of each cell (src/unit_cells.f90:7-30); ORC_CELL_HYPERKAGOME,
+ * the twelve atoms of src/unit_cells.f90:32-77. Site index
+ * ((ix*Ly + iy)*Lz + iz)*natoms + atom, the reference order for multi-atom
+ * cells (src/chainMesh.f90:684,878). */
 typedef struct { int32_t lx, ly, lz; int32_t periodic[3]; int32_t cell; } orc_lattice;
 #define ORC_CELL_SC 0
 #define ORC_CELL_BCC 1
+#define ORC_CELL_HYPERKAGOME 2
 typedef struct {
     double J, D; int32_t dmi_type; int32_t reserved;
     double Bz, Kz, zeeman_total_scale;
@@ -55,6 +57,13 @@ int64_t orc_site(const orc_lattice* L, int ix, int iy, int iz);
 int64_t orc_neighbour(const orc_lattice* L, int64_t site, int axis, int dir);
 int64_t orc_bcc_site(const orc_lattice* L, int ix, int iy, int iz, int atom);
 int     orc_bcc_bonds(const orc_lattice* L, int64_t site, int64_t nb[8], int sgn[8][3]);
+int     orc_atoms_per_cell(const orc_lattice* L);
+void    orc_atom_frac(const orc_lattice* L, int atom, double f[3]);
+int64_t orc_cell_site(const orc_lattice* L, int ix, int iy, int iz, int atom);
+int     orc_hk_bonds(const orc_lattice* L, int64_t site, int64_t* nb, double (*r)[3]);
+int     orc_num_colours(const orc_lattice* L);
+int     orc_site_colour(const orc_lattice* L, int64_t site);
+int     orc_plaquette(const orc_lattice* L, int i, int j, int z, int64_t ids[4]);
 int64_t orc_nsites(const orc_lattice* L);
 void    orc_dmi_vectors(const orc_ham* h, double d[3][3]);
 

@@ -22,7 +22,8 @@ class orc_lattice(C.Structure):
                 ("cell", C.c_int32)]
 
 
-CELL_SC, CELL_BCC = 0, 1
+CELL_SC, CELL_BCC, CELL_HYPERKAGOME = 0, 1, 2
+CELLS = {"sc": CELL_SC, "bcc": CELL_BCC, "hyperkagome": CELL_HYPERKAGOME}
 
 
 class orc_ham(C.Structure):
@@ -63,6 +64,13 @@ def lib() -> C.CDLL:
             "orc_winding_layer": (C.c_double, [lp, dp, C.c_int, C.POINTER(C.c_int64)]),
             "orc_winding_density": (None, [lp, dp, C.c_int, dp]),
             "orc_bcc_bonds": (C.c_int, [lp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+            "orc_atoms_per_cell": (C.c_int, [lp]),
+            "orc_atom_frac": (None, [lp, C.c_int, dp]),
+            "orc_cell_site": (C.c_int64, [lp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "orc_hk_bonds": (C.c_int, [lp, C.c_int64, C.POINTER(C.c_int64), dp]),
+            "orc_num_colours": (C.c_int, [lp]),
+            "orc_site_colour": (C.c_int, [lp, C.c_int64]),
+            "orc_plaquette": (C.c_int, [lp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)]),
             "orc_ref_winding_density": (None, [lp, dp, C.c_int, dp]),
             "orc_propose": (C.c_int, [dp, C.c_double, u32p, dp, dp]),
             "orc_half_sweep": (C.c_int64, [lp, hp, dp, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
@@ -113,12 +121,49 @@ def lattice(lx, ly, lz, periodic=(True, True, True), cell: str = "sc") -> orc_la
     L = orc_lattice(lx, ly, lz)
     for a in range(3):
         L.periodic[a] = 1 if periodic[a] else 0
-    L.cell = {"sc": CELL_SC, "bcc": CELL_BCC}[cell]
+    L.cell = CELLS[cell]
     return L
 
 
+def atoms_per_cell(L) -> int:
+    return lib().orc_atoms_per_cell(C.byref(L))
+
+
 def nsites(L) -> int:
-    return L.lx * L.ly * L.lz * (2 if L.cell == CELL_BCC else 1)
+    return L.lx * L.ly * L.lz * atoms_per_cell(L)
+
+
+def cell_site(L, ix, iy, iz, atom) -> int:
+    return ((ix * L.ly + iy) * L.lz + iz) * atoms_per_cell(L) + atom
+
+
+def atom_frac(L, atom) -> np.ndarray:
+    f = np.zeros(3)
+    lib().orc_atom_frac(C.byref(L), atom, _dp(f))
+    return f
+
+
+def hk_bonds(L, site) -> tuple[np.ndarray, np.ndarray]:
+    """Nearest neighbours of a hyper-kagome site and the unit bond vectors."""
+    nb = (C.c_int64 * 12)()
+    r = np.zeros((12, 3))
+    n = lib().orc_hk_bonds(C.byref(L), site, nb, _dp(r))
+    return np.array(nb[:n]), r[:n].copy()
+
+
+def num_colours(L) -> int:
+    return lib().orc_num_colours(C.byref(L))
+
+
+def site_colour(L, site) -> int:
+    return lib().orc_site_colour(C.byref(L), site)
+
+
+def plaquette(L, i, j, z):
+    """Sites (s1, s2 = +x, s3 = +y, s4 = +x+y) of plaquette (i, j) of layer z
+    (0-based), or None across an open face."""
+    ids = (C.c_int64 * 4)()
+    return list(ids) if lib().orc_plaquette(C.byref(L), i, j, z, ids) else None
 
 
 def bcc_site(L, ix, iy, iz, atom) -> int:

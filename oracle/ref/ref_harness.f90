@@ -1,5 +1,5 @@
 ! ref_harness.f90 -- C-callable probes of the reference's leaf modules
-! (constants, Rand, vecNd), compiled from the reference sources as they lie
+! (constants, Rand, vecNd, Atom, unit_cells), compiled from the reference sources as they lie
 ! under /root/reference/src by oracle/ref/Makefile into oracle/_ref/.
 ! Test infrastructure only: tests/test_oracle_ref.py compares the oracle's
 ! restatements with these.
@@ -8,6 +8,8 @@ module ref_harness
         use constants, only: pi, mu_0
         use Rand, only: random, makeRandom, algor_uniform_random
         use vecNd
+        use Atom, only: Atom_t
+        use unit_cells, only: bcc_unit_cell, hyper_kagome_unit_cell
         implicit none
 contains
         ! n successive draws of algor_uniform_random after makeRandom(seed)
@@ -59,4 +61,29 @@ contains
                 z = 1.0_8 + cmplx(0.0_8, x)
                 y = aimag(z)
         end function ref_cmplx_default_kind_imag
+
+        ! The unit cells of src/unit_cells.f90 (kind 0: bcc_unit_cell, 1:
+        ! hyper_kagome_unit_cell): edge lengths, angles in degrees, and the
+        ! atoms' fractional positions as stored (default real).
+        subroutine ref_unit_cell(kind, abc, angles, n, xyz) bind(C, name="ref_unit_cell")
+                integer(c_int), value :: kind
+                real(c_double), intent(out) :: abc(3), angles(3)
+                integer(c_int), intent(out) :: n
+                real(c_double), intent(out) :: xyz(3, 16)
+                type(Atom_t), allocatable :: atoms(:)
+                real(kind=8) :: a, b, c, al, be, ga
+                integer :: i
+                if (kind == 0) then
+                        call bcc_unit_cell(a, b, c, al, be, ga, atoms)
+                else
+                        call hyper_kagome_unit_cell(a, b, c, al, be, ga, atoms)
+                end if
+                abc = [a, b, c]
+                angles = [al, be, ga]
+                n = size(atoms)
+                xyz = 0.0_c_double
+                do i = 1, n
+                        xyz(:, i) = [real(atoms(i)%x, c_double), real(atoms(i)%y, c_double), real(atoms(i)%z, c_double)]
+                end do
+        end subroutine ref_unit_cell
 end module ref_harness

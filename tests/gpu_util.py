@@ -29,6 +29,8 @@ def de_tolerance(lat_o, ham, s, sites, props) -> np.ndarray:
         acc = abs(ham.Bz)
         if lat_o.cell == orc.CELL_BCC:
             nbrs = orc.bcc_bonds(lat_o, int(i))[0].tolist()
+        elif lat_o.cell == orc.CELL_HYPERKAGOME:
+            nbrs = orc.hk_bonds(lat_o, int(i))[0].tolist()
         else:
             nbrs = [orc.neighbour(lat_o, int(i), a, d) for a in range(3) for d in (-1, 1)]
         for j in nbrs:

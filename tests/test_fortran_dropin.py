@@ -88,6 +88,10 @@ def test_dropin_entry_points_on_gpu():
     assert abs(v["sk_density_sum"][0] - v["sk_winding_middle"][0]) < 1e-12
     assert 0 < v["sk_density_max"][0] < 1
     assert all(abs(c - 7.5) <= 1.5 for c in v["sk_density_peak"])
+    # hyper-kagome mesh through the same entry points
+    assert v["hk_atoms"][0] == 12 * 3 * 3 * 2
+    assert abs(v["hk_fm_energy"][0] - v["hk_fm_energy"][1]) < 1e-12 * abs(v["hk_fm_energy"][1])
+    assert v["hk_fm_winding"][0] == 0.0 and v["hk_norm_dev"][0] < 1e-6
 
 
 @pytest.mark.gpu

@@ -124,8 +124,9 @@ def test_bcc_sweep_decision_replay():
 
 
 def test_bcc_winding_exact():
-    """Q per layer from the atom-0 square lattices (bit-exact integer, FP64
-    value and density against the oracle) for a skyrmion on atom 0."""
+    """Q per layer from the square lattices of body-centre atoms (each cell's
+    firstAtomInMeshCell): bit-exact integer, FP64 value and density against
+    the oracle for a skyrmion on atom 1."""
     lx = ly = 16
     lat = Lattice(lx, ly, 2, (True, True, False), cell="bcc")
     lo = olat(lat)
@@ -137,7 +138,7 @@ def test_bcc_winding_exact():
             th = np.pi * np.exp(-(np.hypot(dx, dy) / 2.8) ** 2)
             ph = np.arctan2(dy, dx)
             for iz in range(2):
-                st[:, orc.bcc_site(lo, ix, iy, iz, 0)] = [np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]
+                st[:, orc.bcc_site(lo, ix, iy, iz, 1)] = [np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]
     with engine(lat, HAMS["bulk_K"], [1.0]) as e:
         e.set_spins(0, st)
         s = e.get_spins(0)

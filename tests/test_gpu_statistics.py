@@ -13,8 +13,10 @@ deterministic.
 Workloads: the thin film of the skyrmion configuration at reduced size
 (12x12x4, periodic in-plane, open z, J = 1, bulk D = 0.6, Bz = 0.3) at
 T = 1.0 and 1.4, where the layer charge fluctuates (below T ~ 0.8 a chain
-stays in one topological sector for longer than either run), and a 6^3 bcc
-lattice with anisotropy at T = 2.4."""
+stays in one topological sector for longer than either run), a 6^3 bcc
+lattice with anisotropy at T = 2.4, and a 4x4x2 hyper-kagome lattice
+(three-colour sweeps) with the reference's interfacial DMI at T = 0.8 and
+1.6."""
 import numpy as np
 import pytest
 
@@ -27,6 +29,8 @@ pytestmark = pytest.mark.gpu
 
 FILM = (Lattice(12, 12, 4, (True, True, False)), Hamiltonian(J=1.0, D=0.6, dmi="bulk", Bz=0.3), (1.0, 1.4))
 BCC = (Lattice(6, 6, 6, cell="bcc"), Hamiltonian(J=1.0, D=0.3, dmi="bulk", Bz=0.1, Kz=0.05), (2.4,))
+HK = (Lattice(4, 4, 2, cell="hyperkagome"), Hamiltonian(J=1.0, D=0.4, dmi="interfacial_ref", Bz=0.2, Kz=0.05),
+      (0.8, 1.6))
 EVERY = 2
 
 
@@ -61,9 +65,9 @@ def oracle_series(lat, ham, temp, n_meas, seed, slot):
     return E, M, Q
 
 
-@pytest.mark.parametrize("case", ["film", "bcc"])
+@pytest.mark.parametrize("case", ["film", "bcc", "hyperkagome"])
 def test_observables_match_oracle_chains(case):
-    lat, ham, temps = {"film": FILM, "bcc": BCC}[case]
+    lat, ham, temps = {"film": FILM, "bcc": BCC, "hyperkagome": HK}[case]
     orc.set_threads(1)
     gpu = gpu_series(lat, ham, temps, 40000)
     n = lat.sites

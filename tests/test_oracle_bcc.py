@@ -136,10 +136,11 @@ def test_bcc_one_bond_dimer(orc):
     assert abs(m - exact) < 4 * err, (m, exact, err)
 
 
-def test_bcc_winding_uses_atom0_planes(orc):
-    """Q of a bcc layer comes from the square lattice of atom-0 sites: a
-    skyrmion texture on atom 0 with the body atoms anywhere has |Q| = 1, and
-    the density sums to it."""
+def test_bcc_winding_uses_body_atom_planes(orc):
+    """Q of a bcc layer comes from the square lattice of the body-centre atoms
+    (each cell's firstAtomInMeshCell, src/chainMesh.f90:663-671): a skyrmion
+    texture on atom 1 with the corner atoms anywhere has |Q| = 1, and the
+    density sums to it."""
     lx = ly = 20
     L = orc.lattice(lx, ly, 2, cell="bcc")
     rng = np.random.default_rng(3)
@@ -148,7 +149,7 @@ def test_bcc_winding_uses_atom0_planes(orc):
         dx, dy = ix - 9.7, iy - 10.2
         th = np.pi * np.exp(-(np.hypot(dx, dy) / 3.2) ** 2)
         ph = np.arctan2(dy, dx)
-        s[:, orc.bcc_site(L, ix, iy, iz, 0)] = [np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]
+        s[:, orc.bcc_site(L, ix, iy, iz, 1)] = [np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]
     for z in range(2):
         q, raw = orc.winding_layer(L, s, z)
         assert abs(q) == 1 and abs(raw - q) < 1e-9

@@ -12,8 +12,9 @@ of the energies (src/energyMin.f90:113-116,163-166,223-226).
 * transform and projection against numpy.fft (rfft/fft/ifft/irfft), an
   independent implementation, on random meshes with Nyquist planes;
 * the deposit on axes of one and two cells, where the reference's 27-cell
-  list repeats cells, against a direct enumeration; the gather is the
-  transpose of the deposit;
+  list repeats cells, against a direct enumeration, also on hyper-kagome
+  cells, whose atoms at -3/4 of an edge keep only the weight of the 27 cells
+  around their own cell; the gather is the transpose of the deposit;
 * mu_0 from the reference's own constants.f90 (tests/golden/ref_leaf.json).
 """
 import json
@@ -140,8 +141,11 @@ def wrap(i, n):
 
 
 def brute_deposit(orc, L, s):
-    """interpolate_to_fft_array as written: 27 listed cells (repeats kept)."""
-    na = 2 if L.cell == orc.CELL_BCC else 1
+    """interpolate_to_fft_array as written: 27 listed cells (repeats kept);
+    atom positions are the cell corner plus the atom's fraction (pinned
+    against the reference's unit cells in test_oracle_hyperkagome.py)."""
+    na = orc.atoms_per_cell(L)
+    fr = [orc.atom_frac(L, a) for a in range(na)]
     dims = (L.lx, L.ly, L.lz)
     mesh = np.zeros((3,) + dims)
     for i in range(L.lx):
@@ -152,10 +156,9 @@ def brute_deposit(orc, L, s):
                         for dk in (-1, 0, 1):
                             nc = (wrap(i + di, L.lx), wrap(j + dj, L.ly), wrap(k + dk, L.lz))
                             for a in range(na):
-                                f = 0.5 * a
                                 w = 1.0
                                 for ax in range(3):
-                                    d = (nc[ax] + f) - ((i, j, k)[ax] + 0.5)
+                                    d = (nc[ax] + fr[a][ax]) - ((i, j, k)[ax] + 0.5)
                                     n = dims[ax]
                                     q = d / n
                                     d -= n * (np.floor(q + 0.5) if q >= 0 else -np.floor(-q + 0.5))
@@ -166,7 +169,8 @@ def brute_deposit(orc, L, s):
 
 
 @pytest.mark.parametrize("cell,dims", [("sc", (2, 3, 1)), ("sc", (1, 2, 4)), ("bcc", (2, 2, 3)),
-                                       ("bcc", (1, 3, 2)), ("sc", (4, 3, 5))])
+                                       ("bcc", (1, 3, 2)), ("sc", (4, 3, 5)), ("hyperkagome", (3, 2, 4)),
+                                       ("hyperkagome", (1, 3, 2))])
 def test_deposit_small_axes_brute_force(orc, cell, dims):
     L = orc.lattice(*dims, cell=cell)
     rng = np.random.default_rng(3)
@@ -176,7 +180,8 @@ def test_deposit_small_axes_brute_force(orc, cell, dims):
     assert np.max(np.abs(got - want)) < 1e-13
 
 
-@pytest.mark.parametrize("cell,dims", [("sc", (2, 3, 1)), ("bcc", (2, 2, 3)), ("sc", (5, 4, 3)), ("bcc", (3, 4, 4))])
+@pytest.mark.parametrize("cell,dims", [("sc", (2, 3, 1)), ("bcc", (2, 2, 3)), ("sc", (5, 4, 3)), ("bcc", (3, 4, 4)),
+                                       ("hyperkagome", (3, 4, 2))])
 def test_gather_is_transpose_of_deposit(orc, cell, dims):
     L = orc.lattice(*dims, cell=cell)
     rng = np.random.default_rng(11)
