@@ -615,11 +615,14 @@ __device__ __forceinline__ void bcc_sums64(const Rep& R, const Planes& oth, uint
 // replica, one thread per site, in place in the current buffer (accepted
 // moves are stored).
 //
+// kThree: the variant launched in contexts with a three-colour (hyper-kagome)
+// replica: it carries that cell's branch and skips colours a replica does
+// not have.
 // kMeasure (colour-1 launches only): per-site pair_energy and spins in FP32,
 // summed in FP64 per wavefront into wave_part[wave][4] = (E, Mx, My, Mz);
 // with kExact the per-site energies are FP64 (pair_energy_exact), the exact
 // energies the replica exchange takes.
-template <bool kMeasure, bool kLearned, bool kExact = false>
+template <bool kMeasure, bool kLearned, bool kExact = false, bool kThree = false>
 __global__ __launch_bounds__(kBlock) void k_half_sweep(
     SpinBuf sb, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ block_rep,
@@ -631,7 +634,8 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
     const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
     const int slot = block_rep[blk];
     const Rep R = reps[slot];
-    if (colour >= R.ncol) return;   // the third colour's launch, on a two-colour replica (uniform)
+    if constexpr (kThree)
+        if (colour >= R.ncol) return;   // the third colour's launch, on a two-colour replica (uniform)
     const Lad L = lads[R.ladder];
     const float beta = beta_slot[slot];
     const uint32_t sweep = *sweep_ctr + sweep_off;
@@ -656,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
 #pragma unroll
         for (int k = 0; k < 6; ++k) nbr[k] = f3{ 0.f, 0.f, 0.f };
         site = 2u * site_number(R, h, iy, iz) + (uint32_t)colour;
-    } else if (R.cell == ISING_CELL_HYPERKAGOME) {    // uniform; never learned, never measured here
+    } else if (kThree && R.cell == ISING_CELL_HYPERKAGOME) {    // uniform; never learned, never measured here
         const HkSite t = hk_site(R, q, colour);
         hf = hk_field(L, R, planes(spins, R, 0), t);
 #pragma unroll
@@ -2204,11 +2208,11 @@ bool fused_active(const ising_ctx* c) { return fused_lc(c) > 0 && !c->any_demag;
 
 // t0/t1: start/stop events in the dispatch packet (timed launches only;
 // graph-captured launches take the plain path).
-template <bool kMeasure, bool kLearned, bool kExact = false>
+template <bool kMeasure, bool kLearned, bool kExact = false, bool kThree = false>
 void launch_half_kernel(ising_ctx* c, uint32_t off, int colour, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr)
 {
     if (t0) {
-        hipExtLaunchKernelGGL((k_half_sweep<kMeasure, kLearned, kExact>), dim3(c->n_blocks), dim3(kBlock), 0,
+        hipExtLaunchKernelGGL((k_half_sweep<kMeasure, kLearned, kExact, kThree>), dim3(c->n_blocks), dim3(kBlock), 0,
                               c->stream, t0, t1, 0u, spin_buf(c), c->d_reps, c->d_lads, (const float*)c->d_beta_slot,
                               (const int32_t*)c->d_block_rep, (const uint32_t*)c->d_ctr, off, colour,
                               (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
@@ -2216,7 +2220,7 @@ void launch_half_kernel(ising_ctx* c, uint32_t off, int colour, hipEvent_t t0 = 
                               (const double*)c->d_demag);
         return;
     }
-    hipLaunchKernelGGL((k_half_sweep<kMeasure, kLearned, kExact>), dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
+    hipLaunchKernelGGL((k_half_sweep<kMeasure, kLearned, kExact, kThree>), dim3(c->n_blocks), dim3(kBlock), 0, c->stream,
                        spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_block_rep,
                        (const uint32_t*)c->d_ctr, off, colour, (uint32_t)c->seed, (uint32_t)(c->seed >> 32), c->d_acc_wave,
                        kMeasure ? c->d_wave_part : (double*)nullptr, c->tune & 0xFFu, (const MlpDev*)c->d_mlp,
@@ -2313,19 +2317,21 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
         }
         return ISING_OK;
     }
-    if ((rc = pair())) return rc;
-    if (learned) launch_half_kernel<false, true>(c, off, 0, t0, t1);
-    else launch_half_kernel<false, false>(c, off, 0, t0, t1);
-    LAUNCH_CHECK(c);
     if (c->max_ncol > 2) {
-        for (int k = 1; k < c->max_ncol; ++k) {
+        // the kernel variant that carries the hyper-kagome branch (kept out
+        // of the two-colour contexts' kernels, where it costs ~5% of a sweep)
+        for (int k = 0; k < c->max_ncol; ++k) {
             if ((rc = pair())) return rc;
-            if (learned) launch_half_kernel<false, true>(c, off, k, t0, t1);
-            else launch_half_kernel<false, false>(c, off, k, t0, t1);
+            if (learned) launch_half_kernel<false, true, false, true>(c, off, k, t0, t1);
+            else launch_half_kernel<false, false, false, true>(c, off, k, t0, t1);
             LAUNCH_CHECK(c);
         }
         return measure ? launch_measure_all(c, 0, rec) : ISING_OK;
     }
+    if ((rc = pair())) return rc;
+    if (learned) launch_half_kernel<false, true>(c, off, 0, t0, t1);
+    else launch_half_kernel<false, false>(c, off, 0, t0, t1);
+    LAUNCH_CHECK(c);
     if (measure) {
         if (exact) {
             if (learned) launch_half_kernel<true, true, true>(c, off, 1);
