@@ -26,6 +26,7 @@ LATS = {
     "sc_4x4x6": Lattice(4, 4, 6),
     "bcc_4x3x3": Lattice(4, 3, 3, cell="bcc"),
     "bcc_5x2x3": Lattice(5, 2, 3, cell="bcc"),     # an axis of two cells: repeated cells in the deposit
+    "hk_3x2x2": Lattice(3, 2, 2, cell="hyperkagome"),
 }
 
 
@@ -86,7 +87,7 @@ def test_uneven_ladders_and_selection():
             e.demag_field(3)          # open z
 
 
-@pytest.mark.parametrize("lname", ["sc_8x6x4", "bcc_4x3x3"])
+@pytest.mark.parametrize("lname", ["sc_8x6x4", "bcc_4x3x3", "hk_3x2x2"])
 def test_delta_energy_with_field(lname):
     lat = LATS[lname]
     ham = Hamiltonian(J=1.0, D=0.4, Bz=0.1, Kz=0.05, demag=True)

@@ -214,9 +214,21 @@ def test_mixed_with_sc_and_bcc_ladders():
             e.set_tuning(tune_fused(4))
         with pytest.raises(IsingError):
             e.mlp_enable(True, ladder=2)
+        # the learned proposal on the SC ladder: the hyper-kagome replicas
+        # keep their fixed cone in the same (three-colour) launches
+        e.mlp_enable(True, ladder=0)
+        e.sweep(2)
+        s2 = [e.get_spins(r) for r in range(5)]
+        e.mlp_enable(False)
         o = e.pt_run(n_sweeps=20, swap_every=5, measure_every=5, winding_every=10)
         assert np.all(np.isfinite(o["energy"]))
     ho = orc.ham_from(ham)
+    for r, beta in ((3, 2.0), (4, 0.6)):
+        lo = olat(lads[2].lattice)
+        s = s1[r].copy()          # the oracle sweeps in place
+        orc.sweeps(lo, ho, s, beta, 0.5, 77, r, 3, 2)
+        assert np.sum(np.max(np.abs(s - s2[r]), axis=0) > 1e-3) <= 2, r
+    assert np.allclose(np.linalg.norm(s2[0], axis=0), 1.0, atol=1e-5)
     slots = [(0, 0, 1.0, 0.5), (1, 0, 0.5, 0.5), (2, 1, 1.5, 0.4), (3, 2, 2.0, 0.5), (4, 2, 0.6, 0.5)]
     for r, li, beta, cone in slots:
         lo = olat(lads[li].lattice)
@@ -244,6 +256,29 @@ def test_demagnetisation_field_and_energy():
     assert np.max(np.abs(want)) > 1e-3
     e_ref = orc.total_energy(lo, orc.ham(1.0, 0.3, "bulk", Bz=0.2), s) + orc.demag_energy(lo, s, want, orc.MU0_REF)
     assert abs(en - e_ref) < 1e-12 * abs(e_ref)
+
+
+@pytest.mark.parametrize("interval", [1, 2])
+def test_demagnetisation_sweeps_replay(interval):
+    """Sweeps with the term (three colour launches adding mu0 H of each site)
+    against the oracle's Metropolis_mcs restatement with the same Philox
+    counters and field schedule."""
+    lat = Lattice(3, 2, 2, cell="hyperkagome")
+    ham = Hamiltonian(J=1.0, D=0.5, Bz=0.2, demag=True)
+    n = 4
+    with engine(lat, ham, [1.3], seed=31) as e:
+        e.set_demag_interval(interval)
+        e.init_random()
+        s = e.get_spins(0)
+        e.sweep(n)
+        got = e.get_spins(0)
+        stored = e.stored_demag_field(0)
+    lo = olat(lat)
+    ref = s.copy()
+    _, H = orc.sweeps_demag(lo, orc.ham(1.0, 0.5, "bulk", Bz=0.2), ref, 1.3, 0.5, 31, 0, 0, n, orc.MU0_REF, interval)
+    bad = int(np.sum(np.max(np.abs(ref - got), axis=0) > 1e-3))
+    assert bad <= 2, bad
+    assert np.max(np.abs(stored - orc.demag_field(lo, got))) < 1e-12     # n is a multiple of the interval
 
 
 def test_lattice_checks():
