@@ -27,7 +27,17 @@ geometric), lattices 24^3, 32^3, 48^3 by field (b mod 3), J = 1, bulk D = 0.5,
 all in one launch per half sweep; both autotuners run (launch shape, then the
 sweeps-per-swap interval), then 2000 sweeps are timed with E/M every 10.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3]
+`--workload bcc` and `--workload hk` measure the reference's two other
+lattices under the same PT schedule shape (64 replicas, T 0.2..2.0
+geometric, J = 1, bulk D = 0.5, Bz = 0.2, swap every 10 sweeps, E/M every
+10): body-centred cubic 50^3 cells (250,000 sites per replica, the cell of
+the reference's PT driver; two colour launches per sweep) and hyper-kagome
+25^3 cells (187,500 sites per replica, the cell of its ISING driver; three
+colour launches per sweep, table-driven neighbours); 2000 timed sweeps,
+in-schedule kernel timing, and the FP64 CPU oracle on one replica of the
+same cell. With `--profile-sweeps N` they run only N plain sweeps.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|bcc|hk]
 """
 from __future__ import annotations
 
@@ -109,8 +119,9 @@ def main() -> int:
                     help="with --profile-sweeps: use these launch-tuning flags instead of autotuning")
     ap.add_argument("--eager", action="store_true",
                     help="launch every period eagerly instead of replaying a captured hipGraph (same kernels)")
-    ap.add_argument("--workload", choices=("c5", "c3"), default="c5",
-                    help="c5: 64^3 x 64 PT, learned vs fixed (default); c3: 256 replicas of mixed lattices")
+    ap.add_argument("--workload", choices=("c5", "c3", "bcc", "hk"), default="c5",
+                    help="c5: 64^3 x 64 PT, learned vs fixed (default); c3: 256 replicas of mixed lattices; "
+                         "bcc / hk: 64-replica PT on body-centred-cubic / hyper-kagome cells")
     ap.add_argument("--profile-sweeps", type=int, default=0,
                     help="only run N plain sweeps, N learned sweeps (untrained network) and one 256 MiB "
                          "copy probe (for rocprofv3 counter runs)")
@@ -126,6 +137,8 @@ def main() -> int:
 
     if args.workload == "c3":
         return bench_c3(args)
+    if args.workload in ("bcc", "hk"):
+        return bench_cell(args, args.workload)
 
     T = geometric_temperatures(T_MIN, T_MAX, N_REP)
     ham = Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2, Kz=0.0)
@@ -336,6 +349,100 @@ def bench_c3(args) -> int:
     }
     print(json.dumps(line), flush=True)
     eng.close()
+    return 0
+
+
+CELL_WORKLOADS = {
+    # cell name, cells per axis, colour launches per sweep, description
+    "bcc": ("bcc", 50, 2, "body-centred cubic 50^3 cells (2 atoms each; the reference's PT lattice)"),
+    "hk": ("hyperkagome", 25, 3, "hyper-kagome 25^3 cells (12 atoms each; the reference's ISING lattice)"),
+}
+CELL_SWEEPS = 2000
+CELL_MEASURE_EVERY = 10
+
+
+def cpu_baseline_cell(cell: str, n_cells: int, seconds: float = 10.0) -> dict:
+    """FP64 OpenMP oracle sweeps on one replica of the cell at T = 0.5 J."""
+    from oracle import pyoracle as orc
+    cores = min(16, os.cpu_count() or 1)
+    orc.set_threads(cores)
+    lat = orc.lattice(n_cells, n_cells, n_cells, cell=cell)
+    n = orc.nsites(lat)
+    h = orc.ham(1.0, 0.5, "bulk", 0.2, 0.0)
+    s = orc.init_random(lat, SEED, 0)
+    t0 = time.perf_counter()
+    orc.sweeps(lat, h, s, 2.0, CONE, SEED, 0, 0, 1)
+    one = time.perf_counter() - t0
+    k = max(1, int(seconds / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    orc.sweeps(lat, h, s, 2.0, CONE, SEED, 0, 1, k)
+    dt = time.perf_counter() - t0
+    return {"value": n * k / dt, "unit": "site updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 C/OpenMP oracle, 1 replica of {n_cells}^3 {cell} cells ({n} sites), T=0.5, r=0.5, "
+                      f"{k} sweeps in {dt:.1f} s"}
+
+
+def bench_cell(args, name: str) -> int:
+    """64-replica PT on a multi-atom cell: autotuned launch shape, 2000 timed
+    sweeps (swap every 10, E/M every 10), the colour launches timed inside
+    the schedule."""
+    from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
+    cell, nc, launches, desc = CELL_WORKLOADS[name]
+    T = geometric_temperatures(T_MIN, T_MAX, N_REP)
+    lat = Lattice(nc, nc, nc, cell=cell)
+    eng = Engine([Ladder(lat, Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2), list(1.0 / T), CONE)], seed=SEED)
+    eng.init_random()
+    n_sites = lat.sites * N_REP
+    if args.profile_sweeps:
+        if args.tuning is not None:
+            eng.set_tuning(args.tuning)
+        eng.sweep(args.profile_sweeps)
+        eng.synchronize()
+        progress(f"{name}: {args.profile_sweeps} plain sweeps done")
+        eng.close()
+        return 0
+    tune_flags, tune_ms = eng.autotune(10)
+    progress(f"{name} autotune: flags {tune_flags}")
+    eng.pt_run(200, SWAP_EVERY, 0, 0, use_graph=not args.eager)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    out = eng.pt_run(CELL_SWEEPS, SWAP_EVERY, CELL_MEASURE_EVERY, 0, use_graph=not args.eager)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    progress(f"{name} timed: {wall * 1e3:.1f} ms for {CELL_SWEEPS} sweeps")
+    ms_launch = eng.time_pt_sweeps(200, SWAP_EVERY, CELL_MEASURE_EVERY, 0)
+    att, acc = eng.swap_stats()
+    eng.close()
+    bytes_launch = BYTES_PER_UPDATE * n_sites / launches
+    achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline_cell(cell, {"bcc": 24, "hk": 12}[name])
+    line = {
+        "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
+        "value": n_sites * CELL_SWEEPS / wall,
+        "unit": "site updates/s",
+        "n_gpus": 1,
+        "steps": CELL_SWEEPS,
+        "warmup": 200,
+        "ms_per_step": wall * 1e3 / CELL_SWEEPS,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 spins, fp64 reductions",
+        "data": "synthetic (seeded random-sphere start)",
+        "config": {"workload": f"PT on {desc}: 64 replicas, T 0.2-2.0 geometric, J=1 D=0.5 (bulk) Bz=0.2, "
+                               f"swap every {SWAP_EVERY}, E/M every {CELL_MEASURE_EVERY}; {launches} colour launches "
+                               "per sweep", "sites": n_sites, "step": "one sweep of every replica"},
+        "swap_acceptance": {"mean": float((acc / np.maximum(att, 1))[att > 0].mean()),
+                            "min": float((acc / np.maximum(att, 1))[att > 0].min())},
+        "energy_per_site": (out["energy"][-1][::16] / lat.sites).tolist(),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_half_sweep",
+                     "ms_per_launch": ms_launch, "algorithmic_bytes_per_launch": bytes_launch,
+                     "tuning_flags": tune_flags,
+                     "tuning_ms_per_launch": {str(f): float(t) for f, t in tune_ms.items()}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
     return 0
 
 
