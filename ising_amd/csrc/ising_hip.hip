@@ -378,18 +378,46 @@ __device__ __forceinline__ bool hk_nbr(const Rep& R, const HkSite& t, const HkBo
     return ok;
 }
 
-// h = J sum_b S_b - sum_a A_a x d_a + Bz z with A_a = sum_b r_a(b) S_b, i.e.
-// d_b = sum_a r_a(b) d_a per bond (field_from_sums, d_a unscaled). `all`:
-// the colour-0 planes, indexed over the whole replica.
-__device__ __forceinline__ f3 hk_field(const Lad& L, const Rep& R, const Planes& all, const HkSite& t)
+// Cell-index steps of one site to the cell below and above it on each axis
+// (the periodic image at a face) and whether those cells exist.
+struct HkSteps { int32_t dn[3], up[3]; bool okdn[3], okup[3]; };
+
+__device__ __forceinline__ HkSteps hk_steps(const Rep& R, const HkSite& t)
 {
+    const uint32_t c[3] = { t.ix, t.iy, t.iz };
+    const int32_t n[3] = { R.lx, R.ly, R.lz };
+    const int32_t stride[3] = { 1, R.lx, R.lx * R.ly };
+    HkSteps s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool per = (R.per >> a) & 1;
+        s.dn[a] = c[a] == 0u ? (n[a] - 1) * stride[a] : -stride[a];
+        s.up[a] = c[a] + 1u == (uint32_t)n[a] ? -(n[a] - 1) * stride[a] : stride[a];
+        s.okdn[a] = c[a] > 0u || per;
+        s.okup[a] = c[a] + 1u < (uint32_t)n[a] || per;
+    }
+    return s;
+}
+
+// The field sum over the four bonds of storage rank `rank` (wave-uniform in
+// the common case, so the table reads are scalar).
+__device__ __forceinline__ f3 hk_field_rank(const Lad& L, const Rep& R, const Planes& all, const HkSite& t,
+                                            const HkSteps& st, uint32_t rank)
+{
+    const uint32_t C = (uint32_t)R.plane * (uint32_t)R.lz;
     f3 sum{ 0.f, 0.f, 0.f }, ax = sum, ay = sum, az = sum;
 #pragma unroll
     for (int b = 0; b < kHkBonds; ++b) {
-        const HkBond& B = c_hk.bond[t.rank][b];
-        uint32_t i;
-        const bool ok = hk_nbr(R, t, B, i);
-        f3 v = ld3(all, i);
+        const HkBond& B = c_hk.bond[rank][b];
+        const int32_t d[3] = { B.dx, B.dy, B.dz };
+        int32_t step = 0;
+        bool ok = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (d[a] < 0) { step += st.dn[a]; ok = ok && st.okdn[a]; }
+            else if (d[a] > 0) { step += st.up[a]; ok = ok && st.okup[a]; }
+        }
+        f3 v = ld3(all, (uint32_t)B.rank * C + (uint32_t)((int32_t)t.cell + step));
         if (!ok) v = f3{ 0.f, 0.f, 0.f };
         sum.x += v.x; sum.y += v.y; sum.z += v.z;
         ax.x += B.r[0] * v.x; ax.y += B.r[0] * v.y; ax.z += B.r[0] * v.z;
@@ -397,6 +425,19 @@ __device__ __forceinline__ f3 hk_field(const Lad& L, const Rep& R, const Planes&
         az.x += B.r[2] * v.x; az.y += B.r[2] * v.y; az.z += B.r[2] * v.z;
     }
     return field_from_sums(L, sum, ax, ay, az);
+}
+
+// h = J sum_b S_b - sum_a A_a x d_a + Bz z with A_a = sum_b r_a(b) S_b, i.e.
+// d_b = sum_a r_a(b) d_a per bond (field_from_sums, d_a unscaled). `all`:
+// the colour-0 planes, indexed over the whole replica. A wavefront lies in
+// one atom type's block except where it straddles two (one in C/64 at most);
+// then each lane reads its own rank's bonds.
+__device__ __forceinline__ f3 hk_field(const Lad& L, const Rep& R, const Planes& all, const HkSite& t)
+{
+    const HkSteps st = hk_steps(R, t);
+    const uint32_t r0 = __builtin_amdgcn_readfirstlane(t.rank);
+    if (__ballot(t.rank != r0) == 0ull) return hk_field_rank(L, R, all, t, st, r0);
+    return hk_field_rank(L, R, all, t, st, t.rank);
 }
 
 // FP64 energy of site t (spin s) over its bonds to higher storage ranks
