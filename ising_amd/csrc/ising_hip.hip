@@ -485,6 +485,32 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Wavefront sums of four values at once: the first level hands two of the
+// four values to the partner lane (l ^ 32) and keeps the other two, the
+// second hands one of those (l ^ 16), and the last four levels reduce one
+// value over 16 lanes: 7 exchanges and 7 FP64 adds instead of 4 x 6. Sum k
+// ends in lanes 16k..16k+15; write_wave_sum4 stores it from lane 16k.
+__device__ __forceinline__ double wave_sum4(const double (&v)[4])
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const bool h32 = lane & 32u, h16 = lane & 16u;
+    const double k0 = h32 ? v[2] : v[0], k1 = h32 ? v[3] : v[1];
+    const double g0 = h32 ? v[0] : v[2], g1 = h32 ? v[1] : v[3];
+    const double s0 = k0 + __shfl_xor(g0, 32, kWave);
+    const double s1 = k1 + __shfl_xor(g1, 32, kWave);
+    double t = (h16 ? s1 : s0) + __shfl_xor(h16 ? s0 : s1, 16, kWave);
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+    return t;
+}
+
+// o[k] = sum k of wave_sum4 (its result t in every lane).
+__device__ __forceinline__ void write_wave_sum4(double* __restrict__ o, double t)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    if ((lane & 15u) == 0u) o[lane >> 4] = t;
+}
+
 
 // ---------------------------------------------------------------------------
 // Two spin buffers of equal layout. Half-sweep launches update the current
@@ -743,13 +769,8 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
                 mx = fs.x + c0.x; my = fs.y + c0.y; mz = fs.z + c0.z;
             }
         }
-        double v[4] = { me, mx, my, mz };
-#pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
-        if ((threadIdx.x & (kWave - 1)) == 0) {
-            double* o = wave_part + 4 * ((size_t)blk * (kBlock / kWave) + threadIdx.x / kWave);
-            o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
-        }
+        const double v[4] = { me, mx, my, mz };
+        write_wave_sum4(wave_part + 4 * ((size_t)blk * (kBlock / kWave) + threadIdx.x / kWave), wave_sum4(v));
     }
     // Acceptance count: each wavefront owns one counter (stream-ordered
     // launches never overlap, so a plain read-modify-write is race free).
@@ -951,11 +972,8 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
 
     const uint32_t wpb = T / kWave, wave = tid / kWave;
     if constexpr (kMeasure) {
-        me = wave_sum(me); mx = wave_sum(mx); my = wave_sum(my); mz = wave_sum(mz);
-        if ((tid & (kWave - 1)) == 0) {
-            double* o = wave_part + 4 * ((size_t)blk * wpb + wave);
-            o[0] = me; o[1] = mx; o[2] = my; o[3] = mz;
-        }
+        const double v[4] = { me, mx, my, mz };
+        write_wave_sum4(wave_part + 4 * ((size_t)blk * wpb + wave), wave_sum4(v));
     }
     if ((tid & (kWave - 1)) == 0 && n_acc) acc_count[(size_t)blk * wpb + wave] += n_acc;
 }
