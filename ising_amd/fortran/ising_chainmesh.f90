@@ -44,11 +44,14 @@ contains
         ! (1/2,1/2,1/2), src/unit_cells.f90:7-30) or twelve (cell =
         ! ISING_CELL_HYPERKAGOME, src/unit_cells.f90:32-77). The reference
         ! builds general Bravais cells (src/chainMesh.f90:729-911); these three
-        ! are provided.
-        function makeChainMesh(numCellsX, numCellsY, numCellsZ, periodic, cell) result(mesh)
+        ! are provided. `a` is the cell edge used for positions in output files;
+        ! by default the edge its unit-cell routine sets (bcc 2, hyper-kagome 6,
+        ! src/unit_cells.f90:21,47), 1 for simple cubic.
+        function makeChainMesh(numCellsX, numCellsY, numCellsZ, periodic, cell, a) result(mesh)
                 integer, intent(in) :: numCellsX, numCellsY, numCellsZ
                 logical, intent(in), optional :: periodic(3)
                 integer, intent(in), optional :: cell
+                real(kind=8), intent(in), optional :: a
                 type(ChainMesh_t) :: mesh
                 integer :: stat
                 mesh%numCellsX = numCellsX
@@ -58,11 +61,15 @@ contains
                 select case (mesh%cell)
                 case (ISING_CELL_BCC)
                         mesh%atomsPerCell = 2
+                        mesh%a = 2.0_8
                 case (ISING_CELL_HYPERKAGOME)
                         mesh%atomsPerCell = 12
+                        mesh%a = 6.0_8
                 case default
                         mesh%atomsPerCell = 1
+                        mesh%a = 1.0_8
                 end select
+                if (present(a)) mesh%a = a
                 mesh%numAtoms = numCellsX * numCellsY * numCellsZ * mesh%atomsPerCell
                 if (present(periodic)) mesh%periodic = periodic
                 allocate(mesh%atomSpins(mesh%numAtoms, 3), stat=stat)
