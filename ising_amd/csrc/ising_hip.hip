@@ -1228,13 +1228,11 @@ __global__ __launch_bounds__(kBlock) void k_measure(
 // measurement partials of the fused colour-1 half-sweep (deterministic).
 // wpb_f = 0: partials of the half-sweep grid; else of the fused grid with
 // wpb_f wavefronts per workgroup.
-__global__ __launch_bounds__(kBlock) void k_wave_finalize(
-    const Rep* __restrict__ reps, const double* __restrict__ wave_part, int wpb_f,
+__device__ __forceinline__ void wave_finalize_slot(
+    int slot, double* lds, const Rep* __restrict__ reps, const double* __restrict__ wave_part, int wpb_f,
     double* __restrict__ energy, double* __restrict__ mag, const int32_t* __restrict__ slot_of_temp, int nrep,
-    SeriesRec rec)
+    const SeriesRec& rec)
 {
-    __shared__ double lds[4 * (kBlock / kWave)];
-    const int slot = blockIdx.x;
     const Rep R = reps[slot];
     double v[4] = { 0.0, 0.0, 0.0, 0.0 };
     const int nw = wpb_f ? R.f_nblocks * wpb_f : R.nblocks * (kBlock / kWave);
@@ -1258,6 +1256,15 @@ __global__ __launch_bounds__(kBlock) void k_wave_finalize(
             rec.b[3 * e] = v[1] * inv; rec.b[3 * e + 1] = v[2] * inv; rec.b[3 * e + 2] = v[3] * inv;
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_wave_finalize(
+    const Rep* __restrict__ reps, const double* __restrict__ wave_part, int wpb_f,
+    double* __restrict__ energy, double* __restrict__ mag, const int32_t* __restrict__ slot_of_temp, int nrep,
+    SeriesRec rec)
+{
+    __shared__ double lds[4 * (kBlock / kWave)];
+    wave_finalize_slot(blockIdx.x, lds, reps, wave_part, wpb_f, energy, mag, slot_of_temp, nrep, rec);
 }
 
 // One workgroup per slot: fixed-order sum of its partials (deterministic);
@@ -1409,14 +1416,13 @@ __global__ __launch_bounds__(kBlock) void k_winding_density(
 
 // One workgroup per (slot, layer): fixed-order sum, Q = sum / (4 pi). In
 // one-layer mode (layer >= 0) the other layers read 0.
-__global__ __launch_bounds__(kBlock) void k_winding_finalize(
-    const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax, int layer,
+__device__ __forceinline__ void winding_finalize_block(
+    int b, double* lds, const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax, int layer,
     double* __restrict__ qraw, long long* __restrict__ qint, const int32_t* __restrict__ slot_of_temp, int nrep,
-    SeriesRec rec)
+    const SeriesRec& rec)
 {
-    __shared__ double lds[4 * (kBlock / kWave)];
-    const int slot = blockIdx.x / zmax;
-    const int z = blockIdx.x - slot * zmax;
+    const int slot = b / zmax;
+    const int z = b - slot * zmax;
     const Rep R = reps[slot];
     double q = 0.0;
     if (z < R.lz && (layer < 0 || z == layer)) {      // uniform
@@ -1427,8 +1433,8 @@ __global__ __launch_bounds__(kBlock) void k_winding_finalize(
         q = v[0] / (4.0 * 3.14159265358979323846);
     }
     if (threadIdx.x == 0) {
-        qraw[blockIdx.x] = q;
-        qint[blockIdx.x] = llrint(q);
+        qraw[b] = q;
+        qint[b] = llrint(q);
     }
     if (rec.a) {       // uniform
         const int t = temp_of_slot(slot_of_temp, nrep, slot);
@@ -1439,6 +1445,31 @@ __global__ __launch_bounds__(kBlock) void k_winding_finalize(
             rec.qi[e] = llrint(q);
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_winding_finalize(
+    const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax, int layer,
+    double* __restrict__ qraw, long long* __restrict__ qint, const int32_t* __restrict__ slot_of_temp, int nrep,
+    SeriesRec rec)
+{
+    __shared__ double lds[4 * (kBlock / kWave)];
+    winding_finalize_block(blockIdx.x, lds, reps, qpartial, zmax, layer, qraw, qint, slot_of_temp, nrep, rec);
+}
+
+// Both finalizes of a sweep that records E/M and the charge in one launch:
+// blocks [0, nrep*zmax) finish the charges, the next nrep blocks E and M.
+__global__ __launch_bounds__(kBlock) void k_finalize_q_and_e(
+    const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax, int layer,
+    double* __restrict__ qraw, long long* __restrict__ qint, SeriesRec rq,
+    const double* __restrict__ wave_part, int wpb_f, double* __restrict__ energy, double* __restrict__ mag,
+    SeriesRec rm, const int32_t* __restrict__ slot_of_temp, int nrep)
+{
+    __shared__ double lds[4 * (kBlock / kWave)];
+    const int nq = nrep * zmax;
+    if ((int)blockIdx.x < nq)
+        winding_finalize_block(blockIdx.x, lds, reps, qpartial, zmax, layer, qraw, qint, slot_of_temp, nrep, rq);
+    else
+        wave_finalize_slot((int)blockIdx.x - nq, lds, reps, wave_part, wpb_f, energy, mag, slot_of_temp, nrep, rm);
 }
 
 // ---------------------------------------------------------------------------
@@ -2343,12 +2374,13 @@ uint32_t buf_off(const ising_ctx* c, int done) { return fused_active(c) ? (uint3
 int launch_measure_all(ising_ctx* c, int done = 0, const SeriesRec& rec = kNoRecord);
 
 // exact (half sweeps only): the colour-1 launch records FP64 per-site
-// energies (pair_energy_exact) instead of FP32 ones. With a hyper-kagome
+// energies (pair_energy_exact) instead of FP32 ones. finalize = false leaves
+// the sums of a measuring sweep to launch_winding_all's combined finalize. With a hyper-kagome
 // ladder every sweep is three colour launches (colour 2 leaves two-colour
 // replicas alone) and measurements come from the measure kernel, the
 // colour-1 pair energies needing a bipartite bond graph.
 int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesRec& rec = kNoRecord,
-                 bool exact = false, LaunchTimer* timer = nullptr)
+                 bool exact = false, LaunchTimer* timer = nullptr, bool finalize = true)
 {
     const bool learned = c->any_learned;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -2368,7 +2400,7 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
             else launch_fused_kernel<false, false>(c, off, t0, t1);
         }
         LAUNCH_CHECK(c);
-        if (measure) {
+        if (measure && finalize) {
             hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
                                (const double*)c->d_wave_part_f, c->fused_wpb, c->d_energy, c->d_mag,
                                (const int32_t*)c->d_slot_of_temp, c->R, rec);
@@ -2400,9 +2432,10 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
             else launch_half_kernel<true, false>(c, off, 1);
         }
         LAUNCH_CHECK(c);
-        hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
-                           (const double*)c->d_wave_part, 0, c->d_energy, c->d_mag,
-                           (const int32_t*)c->d_slot_of_temp, c->R, rec);
+        if (finalize)
+            hipLaunchKernelGGL(k_wave_finalize, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
+                               (const double*)c->d_wave_part, 0, c->d_energy, c->d_mag,
+                               (const int32_t*)c->d_slot_of_temp, c->R, rec);
     } else {
         if ((rc = pair())) return rc;
         if (learned) launch_half_kernel<false, true>(c, off, 1, t0, t1);
@@ -2504,7 +2537,10 @@ int launch_measure_all(ising_ctx* c, int done, const SeriesRec& rec)
     return ISING_OK;
 }
 
-int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0, const SeriesRec& rec = kNoRecord)
+// with_e: also finish the E/M sums of the measuring sweep just launched (its
+// launch_sweep ran with finalize = false), in the same finalize launch.
+int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0, const SeriesRec& rec = kNoRecord,
+                       bool with_e = false, const SeriesRec& rec_e = kNoRecord)
 {
     const uint32_t off = buf_off(c, done);
     if (layer < 0) {
@@ -2515,9 +2551,17 @@ int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0, const SeriesR
                            spin_buf(c), off, c->d_reps, (const int32_t*)c->d_lblock_rep, layer, c->d_qpartial);
     }
     LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_winding_finalize, dim3(c->R * c->zmax), dim3(kBlock), 0, c->stream,
-                       c->d_reps, c->d_qpartial, c->zmax, layer, c->d_qraw, c->d_qint,
-                       (const int32_t*)c->d_slot_of_temp, c->R, rec);
+    if (with_e) {
+        const bool fz = fused_active(c);
+        hipLaunchKernelGGL(k_finalize_q_and_e, dim3(c->R * c->zmax + c->R), dim3(kBlock), 0, c->stream,
+                           c->d_reps, (const double*)c->d_qpartial, c->zmax, layer, c->d_qraw, c->d_qint, rec,
+                           (const double*)(fz ? c->d_wave_part_f : c->d_wave_part), fz ? c->fused_wpb : 0,
+                           c->d_energy, c->d_mag, rec_e, (const int32_t*)c->d_slot_of_temp, c->R);
+    } else {
+        hipLaunchKernelGGL(k_winding_finalize, dim3(c->R * c->zmax), dim3(kBlock), 0, c->stream,
+                           c->d_reps, c->d_qpartial, c->zmax, layer, c->d_qraw, c->d_qint,
+                           (const int32_t*)c->d_slot_of_temp, c->R, rec);
+    }
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
@@ -2573,7 +2617,11 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
         // them to the measure kernel); the swap kernel then records E and M
         // at the new temperatures.
         const bool exact = do_swap && !fused_active(c) && c->max_ncol == 2;
-        if ((rc = launch_sweep(c, (uint32_t)k, (do_meas && !do_swap) || exact, do_swap ? kNoRecord : rm, exact, timer)))
+        // a sweep that records E/M on its launches and also the charge finishes
+        // both sums in one launch (before a swap E must be final first)
+        const bool merge = do_meas && !do_swap && do_q && c->max_ncol == 2;
+        if ((rc = launch_sweep(c, (uint32_t)k, (do_meas && !do_swap) || exact, do_swap ? kNoRecord : rm, exact, timer,
+                               !merge)))
             return rc;
         if (do_swap && !exact)
             if ((rc = launch_measure_all(c, done))) return rc;
@@ -2581,7 +2629,7 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
             if ((rc = launch_swap(c, rm))) return rc;
         if (do_q) {
             const SeriesRec rq = sd.qr ? SeriesRec{ c->d_ctr + 2, nq, sd.n_q, sd.qr, nullptr, sd.qi } : kNoRecord;
-            if ((rc = launch_winding_all(c, s.winding_layer - 1, done, rq))) return rc;
+            if ((rc = launch_winding_all(c, s.winding_layer - 1, done, rq, merge, rm))) return rc;
         }
         nm += do_meas ? 1u : 0u;
         nq += do_q ? 1u : 0u;

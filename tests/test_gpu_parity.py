@@ -370,6 +370,32 @@ def test_swap_sweep_energies_are_exact(lname, hname):
             assert np.allclose(o["magnetisation"][-1], mag, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("tuning", [0, 1, tune_fused(4)])
+def test_combined_finalize_records(tuning):
+    """Sweeps that record E/M on their launches and the charge finish both
+    sums in one finalize launch: the records equal separate measurements of
+    the final state (E and M to the FP32-per-site bar, Q exactly), with
+    graph replay and eager launches giving identical series."""
+    lat = Lattice(16, 16, 8)
+    betas = 1.0 / geometric_temperatures(0.5, 2.0, 3)
+    outs = []
+    for g in (False, True):
+        with engine(lat, HAMS["bulk_K"], betas, seed=13) as e:
+            e.set_tuning(tuning)
+            e.init_random()
+            o = e.pt_run(n_sweeps=12, swap_every=0, measure_every=2, winding_every=2, use_graph=g)
+            en, m = e.energy(), e.magnetisation()
+            q, qr = e.winding_all()
+            sot = e.temperature_map()
+        assert np.allclose(o["energy"][-1], en[sot], rtol=1e-6, atol=0)
+        assert np.allclose(o["magnetisation"][-1], m[sot], rtol=0, atol=1e-6)
+        assert np.array_equal(o["winding"][-1], q[sot])
+        assert np.allclose(o["winding_raw"][-1], qr[sot], rtol=0, atol=1e-12)
+        outs.append(o)
+    for k in ("energy", "magnetisation", "winding", "winding_raw"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
 def test_time_pt_sweeps_advances_like_pt_run():
     """The in-schedule kernel timing runs the same chain as ising_pt_run:
     from one checkpoint both end in the same spins and temperature map."""
