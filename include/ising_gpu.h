@@ -262,8 +262,9 @@ int  ising_bandwidth_probe(ising_ctx* ctx, int64_t bytes, double* ms);
 #define ISING_TUNE_NT_OWN    2u  /* non-temporal access to the updated colour (half sweeps) */
 /* One fused launch per sweep instead of two half-sweep launches: each
  * workgroup stages whole z planes in LDS and updates both colours of lc
- * consecutive planes (1..255); ISING_ERR_UNSUPPORTED when a plane of the
- * largest replica does not fit in LDS. */
+ * consecutive planes (1..255); simple-cubic contexts only;
+ * ISING_ERR_UNSUPPORTED when a plane of the largest replica does not fit in
+ * LDS or a replica is not simple cubic. */
 #define ISING_TUNE_FUSED_SHIFT 8
 #define ISING_TUNE_FUSED(lc) ((uint32_t)(lc) << ISING_TUNE_FUSED_SHIFT)
 #define ISING_TUNE_MAX_CANDIDATES 16

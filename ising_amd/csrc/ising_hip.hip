@@ -1,17 +1,20 @@
 // ising_hip.hip -- CDNA4 (gfx950) kernels and C ABI of libising_hip.so.
 //
-// Hot path: exact two-colour checkerboard Metropolis on simple-cubic lattices
-// for classical unit spins (exchange + DMI + Zeeman + uniaxial anisotropy),
-// many replicas per launch, FP32 spins, Philox4x32-10 keyed by
-// (seed, replica slot) with counter (site, sweep). Energies, magnetisation and
-// topological charge are reduced in FP64; replica exchange runs on device.
+// Hot path: exact colour-by-colour Metropolis (two-colour checkerboard on
+// simple-cubic and bcc lattices, three colours on hyper-kagome) for
+// classical unit spins (exchange + DMI + Zeeman + uniaxial anisotropy, and the
+// reference's demagnetising field), many replicas per launch, FP32 spins,
+// Philox4x32-10 keyed by (seed, replica slot) with counter (site, sweep).
+// Energies, magnetisation and topological charge are reduced in FP64; replica
+// exchange runs on device.
 //
-// HBM layout of one replica with N = lx*ly*lz sites, hx = lx/2:
+// HBM layout of one simple-cubic replica with N = lx*ly*lz sites, hx = lx/2:
 //   spins[off + c*N + k*N/2 + (iz*ly + iy)*hx + (ix >> 1)]
 // component c in {x,y,z}, colour k = (ix+iy+iz) & 1. One half-sweep of colour
 // k streams the k arrays once (coalesced: consecutive lanes take consecutive
 // ix) and reads the other colour's rows iy+-1, iz+-1 and its own row as
-// neighbours. Replicas are packed back to back; a block table maps each
+// neighbours. bcc and hyper-kagome replicas store each colour as one block
+// of the same arrays (see bcc_steps and HkTable). Replicas are packed back to back; a block table maps each
 // workgroup to (replica, chunk), so replicas of different sizes share one
 // launch.
 #include <hip/hip_runtime.h>
@@ -1113,8 +1116,10 @@ __device__ __forceinline__ void block_sum4(double v[4], double* lds /* [4][kBloc
     }
 }
 
-// Per-site energy of the +x, +y, +z bonds plus on-site terms, and the spin,
-// for both colours at half index q; per-workgroup partial sums.
+// Per-site energy of the sites at half index q of every colour (each bond
+// once: the +x, +y, +z bonds on SC, the bonds from atom 0 on bcc, the bonds
+// to higher storage ranks on hyper-kagome) plus on-site terms, and the
+// spins; per-workgroup partial sums.
 __global__ __launch_bounds__(kBlock) void k_measure(
     SpinBuf sb, uint32_t sweep_off, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const Lad* __restrict__ lad_override, const int32_t* __restrict__ block_rep, int block_base,
@@ -1538,7 +1543,7 @@ __global__ void k_set_u32(uint32_t* __restrict__ p, int n, uint32_t v)
 }
 
 // ---------------------------------------------------------------------------
-// Initial states (both colours at half index q).
+// Initial states (the sites at half index q of every colour).
 __global__ __launch_bounds__(kBlock) void k_init(
     float* __restrict__ spins, const Rep* __restrict__ reps, const int32_t* __restrict__ block_rep,
     int block_base, int mode, double frac, uint32_t seed_lo, uint32_t seed_hi)

@@ -15,8 +15,9 @@
 ! it needs a fully periodic mesh. With demag = .false. the field is not
 ! computed (the reference computes it on every call and discards it).
 !
-! Sampling: Metropolis_mcs runs exact two-colour checkerboard sweeps (every
-! site one proposal per sweep, accept iff u < exp(-beta dE)) with the
+! Sampling: Metropolis_mcs runs exact colour-by-colour sweeps (two colours
+! on simple-cubic and bcc meshes, three on hyper-kagome; every site one
+! proposal per sweep, accept iff u < exp(-beta dE)) with the
 ! reference's ball proposal of radius r, instead of the reference's
 ! snapshot scheme with repeated picks (src/energyMin.f90:287-360).
 module EnergyMin

@@ -28,7 +28,9 @@
  *
  * Pinning (tests/test_oracle_demag.py): closed forms of a single
  * magnetisation mode m0 cos(q.r + phi) on simple-cubic and bcc cells
- * (longitudinal, transverse, uniform); an independent numpy.fft
+ * (longitudinal, transverse, uniform); a brute-force enumeration of the
+ * deposit on hyper-kagome cells (atoms outside their own cell); an
+ * independent numpy.fft
  * (rfftn/irfftn) route for the transform and projection, including the
  * Nyquist planes; a brute-force enumeration of the 27-cell deposit with its
  * repeated cells on axes of one and two cells; the reference's mu_0 from its
