@@ -29,6 +29,7 @@
 #include <vector>
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 
 #include "ising_gpu.h"
 #include "philox.hpp"
@@ -1639,26 +1640,36 @@ __global__ __launch_bounds__(kBlock) void k_delta_energy(
 // (src/reciprocalSpaceProcesses.f90:202-281), FP64 throughout:
 //   k_demag_deposit    cloud-in-cell deposit onto one mesh point per cell
 //                      centre (interpolate_to_fft_array, :76-129);
-//   k_dft_r2c_x        forward real-to-half-complex transform along x;
-//   k_dft_c2c<axis>    complex transforms along y and z (both directions);
+//   k_dft_lines<kR2cX> forward real-to-half-complex transform along x;
+//   k_dft_lines<kC2cY>, <kC2cZ>  complex transforms along y and z (both
+//                      directions);
 //   k_demag_project    H(k) = -(k.M) k / k^2, k = 0 removed (:237-270);
-//   k_dft_c2r_x        half-complex-to-real inverse along x, reading only
+//   k_dft_lines<kC2rX> half-complex-to-real inverse along x, reading only
 //                      the real parts of the DC and Nyquist terms (FFTW's
 //                      c2r convention), scaled by 1/(lx ly lz) (:271-275);
 //   k_demag_gather     the same weights back to the atoms
 //                      (interpolate_fft_to_atoms, :131-178).
-// Mesh arrays per replica: real [3][lx][ly][lz] (cell (i*ly + j)*lz + k, the
-// reference's cell order), complex [3][lx/2+1][ly][lz]. Transforms are direct
-// sums over each axis with a twiddle table e^{-2 pi i m/n} per length, one
-// thread per output: any length, exact FP64 twiddles. Every kernel runs on
-// grid (x blocks, replica slot); a slot is skipped unless it is fully
-// periodic and selected (demag_slot).
+// Mesh arrays per replica: real [3][lz][ly][lx] (cell (k*ly + j)*lx + i, x
+// fastest as in the spin layout, so the deposit and the gather read
+// neighbouring cells and atoms with neighbouring lanes), complex
+// [3][lz][ly][lx/2+1]; the half axis is x, as in the reference. The transforms
+// (k_dft_lines) take every line of one axis through LDS: one Cooley-Tukey
+// step L = P Q per line, with exact FP64 twiddles e^{-2 pi i m/L} from a
+// table per length; any length up to kDftMaxLen. Every kernel runs on grid
+// (x blocks, replica slot); a slot is skipped unless it is fully periodic
+// and selected (demag_slot).
 struct DemagRep {
     int64_t moff;        // doubles: real mesh of this slot
     int64_t koff;        // complex elements: half-complex mesh of this slot
     int32_t tw[3];       // first entry of the twiddle table per axis length
     int32_t hx;          // lx/2 + 1
+    int32_t fp[3];       // P of L = P Q per axis
+    int32_t tl[3];       // lines per workgroup per axis
 };
+
+constexpr int kDftMaxLen = 1024;     // longest axis of a lattice with the field
+constexpr int kDftMaxLines = 64;     // lines per workgroup
+constexpr int kDftLdsTarget = 2560;  // double2 of LDS per workgroup, lengths up to 512
 
 // only >= 0: that slot; kDemagLadders: slots whose ladder has the term on;
 // kDemagAll: every fully periodic slot.
@@ -1667,7 +1678,7 @@ constexpr int kDemagAll = -2;
 
 __device__ __forceinline__ bool demag_slot(const Rep& R, const Lad* __restrict__ lads, int only)
 {
-    if (R.per != 7) return false;
+    if (R.per != 7 || R.lx > kDftMaxLen || R.ly > kDftMaxLen || R.lz > kDftMaxLen) return false;
     if (only >= 0) return (int)blockIdx.y == only;
     return only == kDemagAll || lads[R.ladder].demag;
 }
@@ -1719,21 +1730,28 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
     const int64_t Nc = (int64_t)nx * ny * nz;
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (cell >= Nc) return;
-    const int k = (int)(cell % nz), j = (int)((cell / nz) % ny), i = (int)(cell / ((int64_t)nz * ny));
+    const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((int64_t)nx * ny));
     const int na = atoms_per_cell(R);
     const float* sp = spins + R.off;
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    for (int di = -1; di <= 1; ++di) {
-        const int ni = wrap_cell(i + di, nx);
-        for (int dj = -1; dj <= 1; ++dj) {
-            const int nj = wrap_cell(j + dj, ny);
-            for (int dk = -1; dk <= 1; ++dk) {
-                const int nk = wrap_cell(k + dk, nz);
-                for (int a = 0; a < na; ++a) {
-                    const double w = cic_axis(ni, atom_frac(R, a, 0), i, nx) * cic_axis(nj, atom_frac(R, a, 1), j, ny)
-                                     * cic_axis(nk, atom_frac(R, a, 2), k, nz);
+    for (int a = 0; a < na; ++a) {
+        // the weight is a product of one factor per axis: 9 factors per atom
+        double fx[3], fy[3], fz[3];
+        for (int d = 0; d < 3; ++d) {
+            fx[d] = cic_axis(wrap_cell(i + d - 1, nx), atom_frac(R, a, 0), i, nx);
+            fy[d] = cic_axis(wrap_cell(j + d - 1, ny), atom_frac(R, a, 1), j, ny);
+            fz[d] = cic_axis(wrap_cell(k + d - 1, nz), atom_frac(R, a, 2), k, nz);
+        }
+        for (int di = 0; di < 3; ++di) {
+            if (fx[di] == 0.0) continue;
+            const int ni = wrap_cell(i + di - 1, nx);
+            for (int dj = 0; dj < 3; ++dj) {
+                if (fy[dj] == 0.0) continue;
+                const int nj = wrap_cell(j + dj - 1, ny);
+                for (int dk = 0; dk < 3; ++dk) {
+                    const double w = fx[di] * fy[dj] * fz[dk];
                     if (w == 0.0) continue;
-                    const uint32_t t = atom_index(R, ni, nj, nk, a);
+                    const uint32_t t = atom_index(R, ni, nj, wrap_cell(k + dk - 1, nz), a);
                     m0 += w * (double)sp[t];
                     m1 += w * (double)sp[R.n + t];
                     m2 += w * (double)sp[2 * R.n + t];
@@ -1747,70 +1765,137 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
     o[2 * Nc + cell] = m2;
 }
 
-// X[c][kx][j][k] = sum_n x[c][n][j][k] e^{-2 pi i kx n / lx}, kx <= lx/2.
-__global__ __launch_bounds__(kBlock) void k_dft_r2c_x(
-    const double* __restrict__ mesh, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
-    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw, double2* __restrict__ out)
+// Line transforms of the field mesh, one kernel per direction:
+//   kR2cX  X[c][kx][j][k] = sum_n x[c][n][j][k] e^{-2 pi i kx n/lx}, kx <= lx/2;
+//   kC2cY, kC2cZ  complex transforms along y and z of [3][hx][ly][lz], sign -1
+//          forward (e^{-2 pi i k n/len}), +1 inverse (unnormalised);
+//   kC2rX  x[c][i][j][k] = Re sum_kx Y_kx e^{2 pi i kx i/lx} / (lx ly lz), Y the
+//          Hermitian completion of the lx/2+1 stored values with the DC and
+//          Nyquist terms taken real (FFTW's c2r convention).
+// A workgroup holds T lines of one slot in LDS ([n][T+1]) with the slot's
+// twiddle table W^j = e^{-2 pi i j/L}, and transforms them by one
+// Cooley-Tukey step L = P Q (n = Q n1 + n2, k = k1 + P k2):
+//   Y[n2][k1] = W^{n2 k1} sum_n1 x[Q n1 + n2] W^{Q (n1 k1 mod P)},
+//   X[k1 + P k2] = sum_n2 Y[n2][k1] W^{P (n2 k2 mod Q)},
+// P + Q + 1 complex products per output (a direct sum takes L). The inverse
+// reads the table conjugated. Lines: x (c, row), contiguous; y (c, kz, kx),
+// element stride hx; z (c, ky, kx), stride ly hx. Loads and stores run with
+// consecutive lanes on consecutive addresses (elements for x, lines for y
+// and z).
+enum DftMode { kR2cX = 0, kC2cY = 1, kC2cZ = 2, kC2rX = 3 };
+
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_dft_lines(
+    const std::conditional_t<kMode == kR2cX, double, double2>* __restrict__ in, const Rep* __restrict__ reps,
+    const Lad* __restrict__ lads, const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw,
+    int sign, std::conditional_t<kMode == kC2rX, double, double2>* __restrict__ out)
 {
+    constexpr int kAxis = kMode == kC2cY ? 1 : (kMode == kC2cZ ? 2 : 0);
+    constexpr bool kContig = kAxis == 0;
     const Rep R = reps[blockIdx.y];
     if (!demag_slot(R, lads, only)) return;
     const DemagRep D = dreps[blockIdx.y];
-    const int nx = R.lx;
-    const int64_t plane = (int64_t)R.ly * R.lz, Nc = plane * nx, Nk = plane * D.hx;
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 3 * Nk) return;
-    const int64_t c = e / Nk, r = e - c * Nk;
-    const int kx = (int)(r / plane);
-    const int64_t jk = r - (int64_t)kx * plane;
-    const double* x = mesh + D.moff + c * Nc + jk;
-    const double2* t = tw + D.tw[0];
-    double re = 0.0, im = 0.0;
-    int m = 0;
-    for (int n = 0; n < nx; ++n) {
-        const double v = x[(int64_t)n * plane];
-        re += v * t[m].x;
-        im += v * t[m].y;
-        m += kx;
-        if (m >= nx) m -= nx;
-    }
-    out[D.koff + e] = double2{ re, im };
-}
+    const int L = kAxis == 0 ? R.lx : (kAxis == 1 ? R.ly : R.lz);
+    const int P = D.fp[kAxis], Q = L / P, T = D.tl[kAxis], TP = T + 1;
+    const int64_t rows = (int64_t)R.ly * R.lz;                 // x lines per component
+    const int64_t krow = (int64_t)R.ly * D.hx;                 // modes per kz plane
+    const int64_t lines = kAxis == 0 ? 3 * rows : 3 * (int64_t)D.hx * (kAxis == 1 ? R.lz : R.ly);
+    const int64_t first = (int64_t)blockIdx.x * T;
+    if (first >= lines) return;    // uniform per workgroup
+    const int nt = (int)min((int64_t)T, lines - first);
+    const int64_t stride = kAxis == 0 ? 1 : (kAxis == 1 ? (int64_t)D.hx : krow);
 
-// Complex transform along axis 1 (y) or 2 (z) of [3][hx][ly][lz]; sign -1
-// forward (e^{-2 pi i k n / len}), +1 inverse (unnormalised).
-template <int kAxis>
-__global__ __launch_bounds__(kBlock) void k_dft_c2c(
-    const double2* __restrict__ in, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
-    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw, int sign, double2* __restrict__ out)
-{
-    const Rep R = reps[blockIdx.y];
-    if (!demag_slot(R, lads, only)) return;
-    const DemagRep D = dreps[blockIdx.y];
-    const int ny = R.ly, nz = R.lz;
-    const int64_t Nk = (int64_t)D.hx * ny * nz;
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 3 * Nk) return;
-    const int kz = (int)(e % nz);
-    const int ky = (int)((e / nz) % ny);
-    const int len = kAxis == 1 ? ny : nz;
-    const int kk = kAxis == 1 ? ky : kz;
-    const int64_t stride = kAxis == 1 ? nz : 1;
-    const double2* x = in + D.koff + e - (int64_t)kk * stride;
-    const double2* t = tw + D.tw[kAxis];
-    const double sg = sign < 0 ? 1.0 : -1.0;   // the table holds e^{-...}; conjugate for +1
-    double re = 0.0, im = 0.0;
-    int m = 0;
-    for (int n = 0; n < len; ++n) {
-        const double2 v = x[(int64_t)n * stride];
-        const double tr = t[m].x, ti = sg * t[m].y;
-        re += v.x * tr - v.y * ti;
-        im += v.x * ti + v.y * tr;
-        m += kk;
-        if (m >= len) m -= len;
+    extern __shared__ double2 dft_lds[];
+    double2* a = dft_lds;             // [L][TP] input lines
+    double2* b = a + L * TP;          // [L][TP] after the P-point step
+    double2* w = b + L * TP;          // [L] twiddles
+    __shared__ int64_t s_in[kDftMaxLines], s_out[kDftMaxLines];
+    for (int i = threadIdx.x; i < L; i += kBlock) w[i] = tw[D.tw[kAxis] + i];
+    if ((int)threadIdx.x < nt) {
+        const int64_t l = first + threadIdx.x;
+        if (kAxis == 0) {         // l = c rows + row
+            const int64_t kbase = D.koff + l * D.hx, mbase = D.moff + l * R.lx;
+            s_in[threadIdx.x] = kMode == kR2cX ? mbase : kbase;
+            s_out[threadIdx.x] = kMode == kR2cX ? kbase : mbase;
+        } else {                  // y: l = (c lz + kz) hx + kx; z: l = c krow + (ky hx + kx)
+            const int64_t inner = kAxis == 1 ? D.hx : krow;
+            const int64_t outer = l / inner, r = l - outer * inner;
+            const int64_t kbase = D.koff + outer * (kAxis == 1 ? krow : krow * R.lz) + r;
+            s_in[threadIdx.x] = kbase;
+            s_out[threadIdx.x] = kbase;
+        }
     }
-    out[D.koff + e] = double2{ re, im };
-}
+    __syncthreads();
 
+    for (int i = threadIdx.x; i < L * T; i += kBlock) {
+        int t, n;
+        if (kContig) { t = i / L; n = i - t * L; }
+        else { n = i / T; t = i - n * T; }
+        double2 v = double2{ 0.0, 0.0 };
+        if (t < nt) {
+            const int64_t base = s_in[t];
+            if constexpr (kMode == kR2cX) {
+                v = double2{ in[base + n * stride], 0.0 };
+            } else if constexpr (kMode == kC2rX) {
+                if (n < D.hx) {
+                    v = in[base + n * stride];
+                    if (n == 0 || 2 * n == L) v.y = 0.0;
+                } else {
+                    const double2 u = in[base + (int64_t)(L - n) * stride];
+                    v = double2{ u.x, -u.y };
+                }
+            } else {
+                v = in[base + n * stride];
+            }
+        }
+        a[n * TP + t] = v;
+    }
+    __syncthreads();
+
+    const double sg = (kMode == kC2rX || sign > 0) ? -1.0 : 1.0;   // the table holds e^{-...}
+    for (int i = threadIdx.x; i < L * T; i += kBlock) {
+        const int r = i / T, t = i - r * T;
+        const int n2 = r / P, k1 = r - n2 * P;
+        double re = 0.0, im = 0.0;
+        int m = 0;
+        for (int n1 = 0; n1 < P; ++n1) {
+            const double2 v = a[(Q * n1 + n2) * TP + t];
+            const double2 u = w[m * Q];
+            const double ui = sg * u.y;
+            re += v.x * u.x - v.y * ui;
+            im += v.x * ui + v.y * u.x;
+            m += k1;
+            if (m >= P) m -= P;
+        }
+        const double2 u = w[n2 * k1];
+        const double ui = sg * u.y;
+        b[(n2 * P + k1) * TP + t] = double2{ re * u.x - im * ui, re * ui + im * u.x };
+    }
+    __syncthreads();
+
+    const int kout = kMode == kR2cX ? D.hx : L;
+    const double inv_cells = 1.0 / (double)(rows * R.lx);
+    for (int i = threadIdx.x; i < kout * T; i += kBlock) {
+        int t, k;
+        if (kContig) { t = i / kout; k = i - t * kout; }
+        else { k = i / T; t = i - k * T; }
+        if (t >= nt) continue;
+        const int k2 = k / P, k1 = k - k2 * P;
+        double re = 0.0, im = 0.0;
+        int m = 0;
+        for (int n2 = 0; n2 < Q; ++n2) {
+            const double2 v = b[(n2 * P + k1) * TP + t];
+            const double2 u = w[m * P];
+            const double ui = sg * u.y;
+            re += v.x * u.x - v.y * ui;
+            im += v.x * ui + v.y * u.x;
+            m += k2;
+            if (m >= Q) m -= Q;
+        }
+        if constexpr (kMode == kC2rX) out[s_out[t] + k * stride] = re * inv_cells;
+        else out[s_out[t] + k * stride] = double2{ re, im };
+    }
+}
 // H(k) = -(k.M(k)) k / k^2 on the half-complex mesh, in place; wave indices
 // wx = kx, wy and wz folded to (-n/2, n/2]; k = 2 pi w / n per axis (cubic
 // cells in lattice units, so the reference's cutoff |k_a| > pi/a never
@@ -1826,7 +1911,7 @@ __global__ __launch_bounds__(kBlock) void k_demag_project(
     const int64_t Nk = (int64_t)D.hx * ny * nz;
     const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (e >= Nk) return;
-    const int kz = (int)(e % nz), ky = (int)((e / nz) % ny), kx = (int)(e / ((int64_t)nz * ny));
+    const int kx = (int)(e % D.hx), ky = (int)((e / D.hx) % ny), kz = (int)(e / ((int64_t)D.hx * ny));
     double2* f = F + D.koff + e;
     if (kx == 0 && ky == 0 && kz == 0) {
         f[0] = double2{ 0.0, 0.0 }; f[Nk] = double2{ 0.0, 0.0 }; f[2 * Nk] = double2{ 0.0, 0.0 };
@@ -1844,37 +1929,6 @@ __global__ __launch_bounds__(kBlock) void k_demag_project(
     f[0] = double2{ -dr * k0, -di * k0 };
     f[Nk] = double2{ -dr * k1, -di * k1 };
     f[2 * Nk] = double2{ -dr * k2, -di * k2 };
-}
-
-// x[c][i][j][k] = (Re X_0 + sum_{0<kx<lx/2} 2 Re(X_kx e^{2 pi i kx i/lx})
-//                 [+ (-1)^i Re X_{lx/2}]) / (lx ly lz).
-__global__ __launch_bounds__(kBlock) void k_dft_c2r_x(
-    const double2* __restrict__ in, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
-    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw, double* __restrict__ mesh)
-{
-    const Rep R = reps[blockIdx.y];
-    if (!demag_slot(R, lads, only)) return;
-    const DemagRep D = dreps[blockIdx.y];
-    const int nx = R.lx;
-    const int64_t plane = (int64_t)R.ly * R.lz, Nc = plane * nx, Nk = plane * D.hx;
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= 3 * Nc) return;
-    const int64_t c = e / Nc, cell = e - c * Nc;
-    const int i = (int)(cell / plane);
-    const int64_t jk = cell - (int64_t)i * plane;
-    const double2* X = in + D.koff + c * Nk + jk;
-    const double2* t = tw + D.tw[0];
-    const int nyq = (nx & 1) ? -1 : nx / 2;
-    double acc = X[0].x;
-    int m = 0;
-    for (int kx = 1; kx < D.hx; ++kx) {
-        m += i;
-        if (m >= nx) m -= nx;
-        const double2 v = X[(int64_t)kx * plane];
-        if (kx == nyq) acc += (i & 1) ? -v.x : v.x;
-        else acc += 2.0 * (v.x * t[m].x + v.y * t[m].y);   // Re(v conj(t)), t = e^{-...}
-    }
-    mesh[D.moff + e] = acc / (double)Nc;
 }
 
 // H of every atom: the mesh field of the 27 cells around the atom's cell with
@@ -1925,7 +1979,7 @@ __global__ __launch_bounds__(kBlock) void k_demag_gather(
                 const int ck = wrap_cell(iz + dk, nz);
                 const double w = wx * wy * cic_axis(iz, f[2], ck, nz);
                 if (w == 0.0) continue;
-                const int64_t cell = ((int64_t)ci * ny + cj) * nz + ck;
+                const int64_t cell = ((int64_t)ck * ny + cj) * nx + ci;
                 h0 += w * m[cell];
                 h1 += w * m[Nc + cell];
                 h2 += w * m[2 * Nc + cell];
@@ -1981,6 +2035,8 @@ struct ising_ctx {
     bool demag_ready = false;
     std::vector<DemagRep> dreps;
     int64_t dmax_cells = 0, dmax_modes = 0, dmax_sites = 0;
+    int64_t dft_tiles[3] = { 0, 0, 0 };   // workgroups per slot of the line transforms, per axis
+    size_t dft_lds[3] = { 0, 0, 0 };      // their dynamic LDS bytes
     double* d_demag = nullptr;   // FP64 field per site, spin layout (offsets of d_spins)
     double* d_dmesh = nullptr;   // real mesh [3][cells] per slot
     double2* d_dk1 = nullptr;    // half-complex meshes [3][modes] per slot (ping-pong)
@@ -2238,6 +2294,9 @@ int check_demag_lattice(ising_ctx* c, const ising_lattice& l)
 {
     if (!l.periodic[0] || !l.periodic[1] || !l.periodic[2])
         return set_err(c, ISING_ERR_UNSUPPORTED, "the demagnetising field needs a fully periodic lattice");
+    if (l.lx > kDftMaxLen || l.ly > kDftMaxLen || l.lz > kDftMaxLen)
+        return set_err(c, ISING_ERR_UNSUPPORTED, "the demagnetising field takes at most %d cells per axis",
+                       kDftMaxLen);
     return ISING_OK;
 }
 
@@ -2471,12 +2530,26 @@ int demag_setup(ising_ctx* c)
     c->dreps.clear();
     int64_t moff = 0, koff = 0;
     for (const Rep& r : c->reps) {
-        DemagRep d;
+        DemagRep d{};
         const int64_t cells = (int64_t)r.lx * r.ly * r.lz;
         d.hx = r.lx / 2 + 1;
         const int64_t modes = (int64_t)d.hx * r.ly * r.lz;
         d.moff = moff; d.koff = koff;
         d.tw[0] = table(r.lx); d.tw[1] = table(r.ly); d.tw[2] = table(r.lz);
+        const int len[3] = { r.lx, r.ly, r.lz };
+        const int64_t lines[3] = { 3 * (int64_t)r.ly * r.lz, 3 * modes / r.ly, 3 * modes / r.lz };
+        // slots the kernels skip (demag_slot) take no part in the launch shapes
+        const bool used = r.per == 7 && std::max(r.lx, std::max(r.ly, r.lz)) <= kDftMaxLen;
+        for (int a = 0; a < 3 && used; ++a) {
+            const int L = len[a];
+            int p = 1;   // largest divisor of L not above sqrt(L)
+            for (int f = 1; f * f <= L; ++f)
+                if (L % f == 0) p = f;
+            d.fp[a] = p;
+            d.tl[a] = std::max(1, std::min(kDftMaxLines, (kDftLdsTarget - L) / (2 * L) - 1));
+            c->dft_tiles[a] = std::max(c->dft_tiles[a], (lines[a] + d.tl[a] - 1) / d.tl[a]);
+            c->dft_lds[a] = std::max(c->dft_lds[a], (size_t)(2 * L * (d.tl[a] + 1) + L) * sizeof(double2));
+        }
         moff += 3 * cells; koff += 3 * modes;
         c->dmax_cells = std::max(c->dmax_cells, cells);
         c->dmax_modes = std::max(c->dmax_modes, modes);
@@ -2488,6 +2561,13 @@ int demag_setup(ising_ctx* c)
         (rc = dalloc(c, &c->d_dk1, (size_t)koff)) || (rc = dalloc(c, &c->d_dk2, (size_t)koff)) ||
         (rc = dalloc(c, &c->d_tw, tw.size())) || (rc = dalloc(c, &c->d_dreps, c->dreps.size())))
         return rc;
+    // LDS beyond the default 64 KB per workgroup (axes longer than 512)
+    const void* fns[4] = { (const void*)k_dft_lines<kR2cX>, (const void*)k_dft_lines<kC2cY>,
+                           (const void*)k_dft_lines<kC2cZ>, (const void*)k_dft_lines<kC2rX> };
+    const int fn_axis[4] = { 0, 1, 2, 0 };
+    for (int f = 0; f < 4; ++f)
+        HIP_TRY(c, hipFuncSetAttribute(fns[f], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)c->dft_lds[fn_axis[f]]));
     HIP_TRY(c, hipMemcpyAsync(c->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_dreps, c->dreps.data(), c->dreps.size() * sizeof(DemagRep), hipMemcpyHostToDevice,
                               c->stream));
@@ -2510,20 +2590,21 @@ int launch_demag(ising_ctx* c, int only)
     const double2* tw = c->d_tw;
     hipLaunchKernelGGL(k_demag_deposit, grid(c->dmax_cells), dim3(kBlock), 0, s, (const float*)cur_spins(c),
                        c->d_reps, c->d_lads, dr, only, c->d_dmesh);
-    hipLaunchKernelGGL(k_dft_r2c_x, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double*)c->d_dmesh,
-                       c->d_reps, c->d_lads, dr, only, tw, c->d_dk1);
-    hipLaunchKernelGGL(k_dft_c2c<1>, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double2*)c->d_dk1,
+    auto lines = [&](int a) { return dim3((unsigned)std::max<int64_t>(1, c->dft_tiles[a]), R); };
+    hipLaunchKernelGGL(k_dft_lines<kR2cX>, lines(0), dim3(kBlock), c->dft_lds[0], s, (const double*)c->d_dmesh,
+                       c->d_reps, c->d_lads, dr, only, tw, -1, c->d_dk1);
+    hipLaunchKernelGGL(k_dft_lines<kC2cY>, lines(1), dim3(kBlock), c->dft_lds[1], s, (const double2*)c->d_dk1,
                        c->d_reps, c->d_lads, dr, only, tw, -1, c->d_dk2);
-    hipLaunchKernelGGL(k_dft_c2c<2>, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double2*)c->d_dk2,
+    hipLaunchKernelGGL(k_dft_lines<kC2cZ>, lines(2), dim3(kBlock), c->dft_lds[2], s, (const double2*)c->d_dk2,
                        c->d_reps, c->d_lads, dr, only, tw, -1, c->d_dk1);
     hipLaunchKernelGGL(k_demag_project, grid(c->dmax_modes), dim3(kBlock), 0, s, c->d_dk1, c->d_reps, c->d_lads,
                        dr, only);
-    hipLaunchKernelGGL(k_dft_c2c<2>, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double2*)c->d_dk1,
+    hipLaunchKernelGGL(k_dft_lines<kC2cZ>, lines(2), dim3(kBlock), c->dft_lds[2], s, (const double2*)c->d_dk1,
                        c->d_reps, c->d_lads, dr, only, tw, +1, c->d_dk2);
-    hipLaunchKernelGGL(k_dft_c2c<1>, grid(3 * c->dmax_modes), dim3(kBlock), 0, s, (const double2*)c->d_dk2,
+    hipLaunchKernelGGL(k_dft_lines<kC2cY>, lines(1), dim3(kBlock), c->dft_lds[1], s, (const double2*)c->d_dk2,
                        c->d_reps, c->d_lads, dr, only, tw, +1, c->d_dk1);
-    hipLaunchKernelGGL(k_dft_c2r_x, grid(3 * c->dmax_cells), dim3(kBlock), 0, s, (const double2*)c->d_dk1,
-                       c->d_reps, c->d_lads, dr, only, tw, c->d_dmesh);
+    hipLaunchKernelGGL(k_dft_lines<kC2rX>, lines(0), dim3(kBlock), c->dft_lds[0], s, (const double2*)c->d_dk1,
+                       c->d_reps, c->d_lads, dr, only, tw, +1, c->d_dmesh);
     hipLaunchKernelGGL(k_demag_gather, grid(c->dmax_sites), dim3(kBlock), 0, s, (const double*)c->d_dmesh,
                        c->d_reps, c->d_lads, dr, only, c->d_demag);
     LAUNCH_CHECK(c);

@@ -27,6 +27,13 @@ LATS = {
     "bcc_4x3x3": Lattice(4, 3, 3, cell="bcc"),
     "bcc_5x2x3": Lattice(5, 2, 3, cell="bcc"),     # an axis of two cells: repeated cells in the deposit
     "hk_3x2x2": Lattice(3, 2, 2, cell="hyperkagome"),
+    # line transforms L = P Q: P = 3, 3, 4 and several workgroups per axis
+    "sc_12x18x16": Lattice(12, 18, 16),
+    "bcc_7x13x5": Lattice(7, 13, 5, cell="bcc"),    # prime lengths: P = 1
+    # one line per workgroup; 1024 takes 80 KB of LDS
+    "bcc_600x2x2": Lattice(600, 2, 2, cell="bcc"),
+    "bcc_2x2x1024": Lattice(2, 2, 1024, cell="bcc"),
+    "bcc_1024x2x2": Lattice(1024, 2, 2, cell="bcc"),
 }
 
 
@@ -190,6 +197,9 @@ def test_fused_flag_then_demag_sweeps_in_place():
 def test_refusals():
     with pytest.raises(IsingError) as ex:
         engine(Lattice(4, 4, 4, (True, True, False)), Hamiltonian(demag=True), [1.0])
+    assert ex.value.code == -6
+    with pytest.raises(IsingError) as ex:     # at most 1024 cells per axis
+        engine(Lattice(2, 1026, 2, cell="bcc"), Hamiltonian(demag=True), [1.0])
     assert ex.value.code == -6
     with engine(Lattice(4, 4, 4), Hamiltonian(J=1.0, demag=True), [1.0, 2.0]) as e:
         with pytest.raises(IsingError):
