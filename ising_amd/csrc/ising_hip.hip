@@ -1688,11 +1688,13 @@ __device__ __forceinline__ int wrap_cell(int i, int n) { return i == n ? 0 : (i 
 
 // Weight factor of one axis: d = atom - centre in lattice units, minimum
 // image d - n nint(d/n) (nint half away from zero; distance_points_vec_bravais,
-// src/chainMesh.f90:268-270), then max(1 - |d|, 0).
+// src/chainMesh.f90:268-270), then max(1 - |d|, 0). nint(d/n) is 0 unless
+// |d| >= n/2 (d is a multiple of 1/4 on every cell here, so d/n rounds to 1/2
+// only when it is 1/2): the division runs only on wrapped bonds.
 __device__ __forceinline__ double cic_axis(int atom_cell, double frac, int centre_cell, int n)
 {
     double d = ((double)atom_cell + frac) - ((double)centre_cell + 0.5);
-    d -= (double)n * round(d / (double)n);
+    if (fabs(d) >= 0.5 * (double)n) d -= (double)n * round(d / (double)n);
     return fmax(1.0 - fabs(d), 0.0);
 }
 
@@ -1730,10 +1732,12 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
     const int64_t Nc = (int64_t)nx * ny * nz;
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (cell >= Nc) return;
-    const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((int64_t)nx * ny));
+    const uint32_t row = (uint32_t)cell / (uint32_t)nx;     // cells < 2^31
+    const int i = (int)((uint32_t)cell - row * (uint32_t)nx), j = (int)(row % (uint32_t)ny), k = (int)(row / (uint32_t)ny);
     const int na = atoms_per_cell(R);
     const float* sp = spins + R.off;
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+#pragma unroll 1
     for (int a = 0; a < na; ++a) {
         // the weight is a product of one factor per axis: 9 factors per atom
         double fx[3], fy[3], fz[3];
@@ -1742,6 +1746,7 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
             fy[d] = cic_axis(wrap_cell(j + d - 1, ny), atom_frac(R, a, 1), j, ny);
             fz[d] = cic_axis(wrap_cell(k + d - 1, nz), atom_frac(R, a, 2), k, nz);
         }
+#pragma unroll 1
         for (int di = 0; di < 3; ++di) {
             if (fx[di] == 0.0) continue;
             const int ni = wrap_cell(i + di - 1, nx);
@@ -1766,10 +1771,10 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
 }
 
 // Line transforms of the field mesh, one kernel per direction:
-//   kR2cX  X[c][kx][j][k] = sum_n x[c][n][j][k] e^{-2 pi i kx n/lx}, kx <= lx/2;
-//   kC2cY, kC2cZ  complex transforms along y and z of [3][hx][ly][lz], sign -1
+//   kR2cX  X[c][k][j][kx] = sum_n x[c][k][j][n] e^{-2 pi i kx n/lx}, kx <= lx/2;
+//   kC2cY, kC2cZ  complex transforms along y and z of [3][lz][ly][hx], sign -1
 //          forward (e^{-2 pi i k n/len}), +1 inverse (unnormalised);
-//   kC2rX  x[c][i][j][k] = Re sum_kx Y_kx e^{2 pi i kx i/lx} / (lx ly lz), Y the
+//   kC2rX  x[c][k][j][i] = Re sum_kx Y_kx e^{2 pi i kx i/lx} / (lx ly lz), Y the
 //          Hermitian completion of the lx/2+1 stored values with the DC and
 //          Nyquist terms taken real (FFTW's c2r convention).
 // A workgroup holds T lines of one slot in LDS ([n][T+1]) with the slot's
@@ -1782,36 +1787,50 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
 // element stride hx; z (c, ky, kx), stride ly hx. Loads and stores run with
 // consecutive lanes on consecutive addresses (elements for x, lines for y
 // and z).
+//
+// Two kernels share this: k_dft_lines_r (P, Q <= kDftRadix) gives each thread
+// one (line, n2) of the first step and one (line, k1) of the second, with its
+// P or Q inputs in registers: one LDS read per input, twiddles read as
+// wavefront-uniform broadcasts, the result written back in place. k_dft_lines
+// (any P, Q) loops over outputs with a second LDS buffer.
 enum DftMode { kR2cX = 0, kC2cY = 1, kC2cZ = 2, kC2rX = 3 };
+constexpr int kDftRadix = 16;
+
+template <int kMode> using DftIn = std::conditional_t<kMode == kR2cX, double, double2>;
+template <int kMode> constexpr int kDftAxis = kMode == kC2cY ? 1 : (kMode == kC2cZ ? 2 : 0);
+template <int kMode> using DftOut = std::conditional_t<kMode == kC2rX, double, double2>;
+
+// Lines of this workgroup: sets L, P, Q, T, nt and the element stride,
+// fills the twiddles w[L] and the first element of every line (s_in in the
+// source mesh, s_out in the destination), then loads the lines into
+// a[n * (T + 1) + t]. False (for the whole workgroup) when the slot is
+// skipped or has no line here.
+struct DftLines {
+    int L, P, Q, T, nt;
+    int64_t stride;
+};
 
 template <int kMode>
-__global__ __launch_bounds__(kBlock) void k_dft_lines(
-    const std::conditional_t<kMode == kR2cX, double, double2>* __restrict__ in, const Rep* __restrict__ reps,
-    const Lad* __restrict__ lads, const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw,
-    int sign, std::conditional_t<kMode == kC2rX, double, double2>* __restrict__ out)
+__device__ __forceinline__ bool dft_load(const Rep& R, const DemagRep& D, const DftIn<kMode>* __restrict__ in,
+                                         const double2* __restrict__ tw, double2* a, double2* w, int64_t* s_in,
+                                         int64_t* s_out, DftLines& g)
 {
-    constexpr int kAxis = kMode == kC2cY ? 1 : (kMode == kC2cZ ? 2 : 0);
+    constexpr int kAxis = kDftAxis<kMode>;
     constexpr bool kContig = kAxis == 0;
-    const Rep R = reps[blockIdx.y];
-    if (!demag_slot(R, lads, only)) return;
-    const DemagRep D = dreps[blockIdx.y];
-    const int L = kAxis == 0 ? R.lx : (kAxis == 1 ? R.ly : R.lz);
-    const int P = D.fp[kAxis], Q = L / P, T = D.tl[kAxis], TP = T + 1;
+    g.L = kAxis == 0 ? R.lx : (kAxis == 1 ? R.ly : R.lz);
+    g.P = D.fp[kAxis];
+    g.Q = g.L / g.P;
+    g.T = D.tl[kAxis];
+    const int L = g.L, T = g.T, TP = T + 1;
     const int64_t rows = (int64_t)R.ly * R.lz;                 // x lines per component
     const int64_t krow = (int64_t)R.ly * D.hx;                 // modes per kz plane
     const int64_t lines = kAxis == 0 ? 3 * rows : 3 * (int64_t)D.hx * (kAxis == 1 ? R.lz : R.ly);
     const int64_t first = (int64_t)blockIdx.x * T;
-    if (first >= lines) return;    // uniform per workgroup
-    const int nt = (int)min((int64_t)T, lines - first);
-    const int64_t stride = kAxis == 0 ? 1 : (kAxis == 1 ? (int64_t)D.hx : krow);
-
-    extern __shared__ double2 dft_lds[];
-    double2* a = dft_lds;             // [L][TP] input lines
-    double2* b = a + L * TP;          // [L][TP] after the P-point step
-    double2* w = b + L * TP;          // [L] twiddles
-    __shared__ int64_t s_in[kDftMaxLines], s_out[kDftMaxLines];
+    if (first >= lines) return false;
+    g.nt = (int)min((int64_t)T, lines - first);
+    g.stride = kAxis == 0 ? 1 : (kAxis == 1 ? (int64_t)D.hx : krow);
     for (int i = threadIdx.x; i < L; i += kBlock) w[i] = tw[D.tw[kAxis] + i];
-    if ((int)threadIdx.x < nt) {
+    if ((int)threadIdx.x < g.nt) {
         const int64_t l = first + threadIdx.x;
         if (kAxis == 0) {         // l = c rows + row
             const int64_t kbase = D.koff + l * D.hx, mbase = D.moff + l * R.lx;
@@ -1826,44 +1845,79 @@ __global__ __launch_bounds__(kBlock) void k_dft_lines(
         }
     }
     __syncthreads();
-
     for (int i = threadIdx.x; i < L * T; i += kBlock) {
         int t, n;
         if (kContig) { t = i / L; n = i - t * L; }
         else { n = i / T; t = i - n * T; }
         double2 v = double2{ 0.0, 0.0 };
-        if (t < nt) {
+        if (t < g.nt) {
             const int64_t base = s_in[t];
             if constexpr (kMode == kR2cX) {
-                v = double2{ in[base + n * stride], 0.0 };
+                v = double2{ in[base + n * g.stride], 0.0 };
             } else if constexpr (kMode == kC2rX) {
                 if (n < D.hx) {
-                    v = in[base + n * stride];
+                    v = in[base + n * g.stride];
                     if (n == 0 || 2 * n == L) v.y = 0.0;
                 } else {
-                    const double2 u = in[base + (int64_t)(L - n) * stride];
+                    const double2 u = in[base + (int64_t)(L - n) * g.stride];
                     v = double2{ u.x, -u.y };
                 }
             } else {
-                v = in[base + n * stride];
+                v = in[base + n * g.stride];
             }
         }
         a[n * TP + t] = v;
     }
     __syncthreads();
+    return true;
+}
 
-    const double sg = (kMode == kC2rX || sign > 0) ? -1.0 : 1.0;   // the table holds e^{-...}
+// v * conj^s(u): the table holds e^{-...}; sg = -1 reads it conjugated.
+__device__ __forceinline__ void cmac(double& re, double& im, const double2& v, const double2& u, double sg)
+{
+    const double ui = sg * u.y;
+    re += v.x * u.x - v.y * ui;
+    im += v.x * ui + v.y * u.x;
+}
+
+template <int kMode>
+__device__ __forceinline__ void dft_store(DftOut<kMode>* __restrict__ out, int64_t at, double re, double im,
+                                          double inv_cells)
+{
+    if constexpr (kMode == kC2rX) out[at] = re * inv_cells;
+    else out[at] = double2{ re, im };
+}
+
+template <int kMode>
+__device__ __forceinline__ double dft_sign(int sign) { return (kMode == kC2rX || sign > 0) ? -1.0 : 1.0; }
+
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_dft_lines(
+    const DftIn<kMode>* __restrict__ in, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw, int sign,
+    DftOut<kMode>* __restrict__ out)
+{
+    constexpr bool kContig = kMode == kR2cX || kMode == kC2rX;
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const DemagRep D = dreps[blockIdx.y];
+    extern __shared__ double2 dft_lds[];
+    __shared__ int64_t s_in[kDftMaxLines], s_out[kDftMaxLines];
+    const int len = kDftAxis<kMode> == 0 ? R.lx : (kDftAxis<kMode> == 1 ? R.ly : R.lz);
+    double2* a = dft_lds;                                       // [L][TP] input lines
+    double2* b = a + len * (D.tl[kDftAxis<kMode>] + 1);          // [L][TP] after the first step
+    double2* w = b + len * (D.tl[kDftAxis<kMode>] + 1);          // [L] twiddles
+    DftLines g;
+    if (!dft_load<kMode>(R, D, in, tw, a, w, s_in, s_out, g)) return;
+    const int L = g.L, P = g.P, Q = g.Q, T = g.T, TP = T + 1;
+    const double sg = dft_sign<kMode>(sign);
     for (int i = threadIdx.x; i < L * T; i += kBlock) {
         const int r = i / T, t = i - r * T;
         const int n2 = r / P, k1 = r - n2 * P;
         double re = 0.0, im = 0.0;
         int m = 0;
         for (int n1 = 0; n1 < P; ++n1) {
-            const double2 v = a[(Q * n1 + n2) * TP + t];
-            const double2 u = w[m * Q];
-            const double ui = sg * u.y;
-            re += v.x * u.x - v.y * ui;
-            im += v.x * ui + v.y * u.x;
+            cmac(re, im, a[(Q * n1 + n2) * TP + t], w[m * Q], sg);
             m += k1;
             if (m >= P) m -= P;
         }
@@ -1872,30 +1926,178 @@ __global__ __launch_bounds__(kBlock) void k_dft_lines(
         b[(n2 * P + k1) * TP + t] = double2{ re * u.x - im * ui, re * ui + im * u.x };
     }
     __syncthreads();
-
     const int kout = kMode == kR2cX ? D.hx : L;
-    const double inv_cells = 1.0 / (double)(rows * R.lx);
+    const double inv_cells = 1.0 / ((double)R.lx * R.ly * R.lz);
     for (int i = threadIdx.x; i < kout * T; i += kBlock) {
         int t, k;
         if (kContig) { t = i / kout; k = i - t * kout; }
         else { k = i / T; t = i - k * T; }
-        if (t >= nt) continue;
+        if (t >= g.nt) continue;
         const int k2 = k / P, k1 = k - k2 * P;
         double re = 0.0, im = 0.0;
         int m = 0;
         for (int n2 = 0; n2 < Q; ++n2) {
-            const double2 v = b[(n2 * P + k1) * TP + t];
-            const double2 u = w[m * P];
-            const double ui = sg * u.y;
-            re += v.x * u.x - v.y * ui;
-            im += v.x * ui + v.y * u.x;
+            cmac(re, im, b[(n2 * P + k1) * TP + t], w[m * P], sg);
             m += k2;
             if (m >= Q) m -= Q;
         }
-        if constexpr (kMode == kC2rX) out[s_out[t] + k * stride] = re * inv_cells;
-        else out[s_out[t] + k * stride] = double2{ re, im };
+        dft_store<kMode>(out, s_out[t] + k * g.stride, re, im, inv_cells);
     }
 }
+
+// Bit reversal of n < N = 2^m.
+template <int N>
+__host__ __device__ constexpr int bitrev(int n)
+{
+    int r = 0;
+    for (int b = 1; b < N; b <<= 1) {
+        r = (r << 1) | (n & 1);
+        n >>= 1;
+    }
+    return r;
+}
+
+// In-register DFT of N = 2^m points, natural order in and out:
+// x[k] <- sum_n x[n] W^{(n k mod N) s}, W^j = w[j] (conjugated for sg = -1),
+// by radix-2 decimation in time.
+template <int N>
+__device__ __forceinline__ void fft_pow2(double2 (&x)[kDftRadix], const double2* w, int s, double sg)
+{
+    double2 y[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) y[bitrev<N>(n)] = x[n];
+#pragma unroll
+    for (int len = 2; len <= N; len <<= 1) {
+#pragma unroll
+        for (int j = 0; j < len / 2; ++j) {
+            double2 u = double2{ 1.0, 0.0 };
+            if (j > 0) {
+                u = w[j * (N / len) * s];
+                u.y *= sg;
+            }
+#pragma unroll
+            for (int b = 0; b < N; b += len) {
+                const double2 p = y[b + j], q = y[b + j + len / 2];
+                const double2 t = j > 0 ? double2{ q.x * u.x - q.y * u.y, q.x * u.y + q.y * u.x } : q;
+                y[b + j] = double2{ p.x + t.x, p.y + t.y };
+                y[b + j + len / 2] = double2{ p.x - t.x, p.y - t.y };
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < N; ++n) x[n] = y[n];
+}
+
+// x[0..n) <- its n-point DFT (twiddle stride s) when n is 1, 2, 4 or 8; false
+// for other n (n is uniform per workgroup). Sixteen points in registers would
+// double the registers of the kernel.
+__device__ __forceinline__ bool fft_reg(double2 (&x)[kDftRadix], int n, const double2* w, int s, double sg)
+{
+    switch (n) {
+    case 1: return true;
+    case 2: fft_pow2<2>(x, w, s, sg); return true;
+    case 4: fft_pow2<4>(x, w, s, sg); return true;
+    case 8: fft_pow2<8>(x, w, s, sg); return true;
+    default: return false;
+    }
+}
+
+// P, Q <= kDftRadix, Q T <= kBlock and P T <= kBlock: one item per thread
+// in each step, in place in one LDS buffer; power-of-two factors run as
+// in-register radix-2 FFTs, other factors as direct sums.
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_dft_lines_r(
+    const DftIn<kMode>* __restrict__ in, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw, int sign,
+    DftOut<kMode>* __restrict__ out)
+{
+    constexpr bool kContig = kMode == kR2cX || kMode == kC2rX;
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const DemagRep D = dreps[blockIdx.y];
+    extern __shared__ double2 dft_lds[];
+    __shared__ int64_t s_in[kDftMaxLines], s_out[kDftMaxLines];
+    const int len = kDftAxis<kMode> == 0 ? R.lx : (kDftAxis<kMode> == 1 ? R.ly : R.lz);
+    double2* a = dft_lds;                                       // [L][TP] lines
+    double2* w = a + len * (D.tl[kDftAxis<kMode>] + 1);          // [L] twiddles
+    DftLines g;
+    if (!dft_load<kMode>(R, D, in, tw, a, w, s_in, s_out, g)) return;
+    const int P = g.P, Q = g.Q, T = g.T, TP = T + 1;
+    const double sg = dft_sign<kMode>(sign);
+
+    // first step: thread (n2, t) takes x[Q n1 + n2], n1 < P
+    const int t1 = threadIdx.x % T, n2 = threadIdx.x / T;
+    const bool on1 = n2 < Q;
+    double2 x[kDftRadix];
+#pragma unroll
+    for (int n1 = 0; n1 < kDftRadix; ++n1)
+        if (on1 && n1 < P) x[n1] = a[(Q * n1 + n2) * TP + t1];
+    __syncthreads();
+    if (on1 && fft_reg(x, P, w, Q, sg)) {
+#pragma unroll
+        for (int k1 = 0; k1 < kDftRadix; ++k1) {
+            if (k1 < P) {
+                const double2 u = w[n2 * k1];
+                const double ui = sg * u.y;
+                a[(n2 * P + k1) * TP + t1] = double2{ x[k1].x * u.x - x[k1].y * ui, x[k1].x * ui + x[k1].y * u.x };
+            }
+        }
+    } else if (on1) {
+        for (int k1 = 0; k1 < P; ++k1) {
+            double re = 0.0, im = 0.0;
+            int m = 0;
+#pragma unroll
+            for (int n1 = 0; n1 < kDftRadix; ++n1) {
+                if (n1 < P) {
+                    cmac(re, im, x[n1], w[m * Q], sg);
+                    m += k1;
+                    if (m >= P) m -= P;
+                }
+            }
+            const double2 u = w[n2 * k1];
+            const double ui = sg * u.y;
+            a[(n2 * P + k1) * TP + t1] = double2{ re * u.x - im * ui, re * ui + im * u.x };
+        }
+    }
+    __syncthreads();
+
+    // second step: thread (k1, t) takes Y[n2][k1], n2 < Q, and writes
+    // X[k1 + P k2]; k1 fastest across lanes on contiguous (x) lines
+    int t2, k1;
+    if (kContig) { k1 = threadIdx.x % P; t2 = threadIdx.x / P; }
+    else { t2 = threadIdx.x % T; k1 = threadIdx.x / T; }
+    if (k1 >= P || t2 >= g.nt) return;
+#pragma unroll
+    for (int j = 0; j < kDftRadix; ++j)
+        if (j < Q) x[j] = a[(j * P + k1) * TP + t2];
+    const int kout = kMode == kR2cX ? D.hx : g.L;
+    const double inv_cells = 1.0 / ((double)R.lx * R.ly * R.lz);
+    const int64_t base = s_out[t2];
+    if (fft_reg(x, Q, w, P, sg)) {
+#pragma unroll
+        for (int k2 = 0; k2 < kDftRadix; ++k2) {
+            const int k = k1 + P * k2;
+            if (k2 < Q && k < kout) dft_store<kMode>(out, base + k * g.stride, x[k2].x, x[k2].y, inv_cells);
+        }
+        return;
+    }
+    for (int k2 = 0; k2 < Q; ++k2) {
+        const int k = k1 + P * k2;
+        if (k >= kout) break;
+        double re = 0.0, im = 0.0;
+        int m = 0;
+#pragma unroll
+        for (int j = 0; j < kDftRadix; ++j) {
+            if (j < Q) {
+                cmac(re, im, x[j], w[m * P], sg);
+                m += k2;
+                if (m >= Q) m -= Q;
+            }
+        }
+        dft_store<kMode>(out, base + k * g.stride, re, im, inv_cells);
+    }
+}
+
 // H(k) = -(k.M(k)) k / k^2 on the half-complex mesh, in place; wave indices
 // wx = kx, wy and wz folded to (-n/2, n/2]; k = 2 pi w / n per axis (cubic
 // cells in lattice units, so the reference's cutoff |k_a| > pi/a never
@@ -2036,6 +2238,7 @@ struct ising_ctx {
     std::vector<DemagRep> dreps;
     int64_t dmax_cells = 0, dmax_modes = 0, dmax_sites = 0;
     int64_t dft_tiles[3] = { 0, 0, 0 };   // workgroups per slot of the line transforms, per axis
+    bool dft_reg[3] = { true, true, true };   // k_dft_lines_r for that axis (every P, Q <= kDftRadix)
     size_t dft_lds[3] = { 0, 0, 0 };      // their dynamic LDS bytes
     double* d_demag = nullptr;   // FP64 field per site, spin layout (offsets of d_spins)
     double* d_dmesh = nullptr;   // real mesh [3][cells] per slot
@@ -2536,19 +2739,15 @@ int demag_setup(ising_ctx* c)
         const int64_t modes = (int64_t)d.hx * r.ly * r.lz;
         d.moff = moff; d.koff = koff;
         d.tw[0] = table(r.lx); d.tw[1] = table(r.ly); d.tw[2] = table(r.lz);
-        const int len[3] = { r.lx, r.ly, r.lz };
-        const int64_t lines[3] = { 3 * (int64_t)r.ly * r.lz, 3 * modes / r.ly, 3 * modes / r.lz };
         // slots the kernels skip (demag_slot) take no part in the launch shapes
         const bool used = r.per == 7 && std::max(r.lx, std::max(r.ly, r.lz)) <= kDftMaxLen;
         for (int a = 0; a < 3 && used; ++a) {
-            const int L = len[a];
+            const int L = a == 0 ? r.lx : (a == 1 ? r.ly : r.lz);
             int p = 1;   // largest divisor of L not above sqrt(L)
             for (int f = 1; f * f <= L; ++f)
                 if (L % f == 0) p = f;
             d.fp[a] = p;
-            d.tl[a] = std::max(1, std::min(kDftMaxLines, (kDftLdsTarget - L) / (2 * L) - 1));
-            c->dft_tiles[a] = std::max(c->dft_tiles[a], (lines[a] + d.tl[a] - 1) / d.tl[a]);
-            c->dft_lds[a] = std::max(c->dft_lds[a], (size_t)(2 * L * (d.tl[a] + 1) + L) * sizeof(double2));
+            c->dft_reg[a] = c->dft_reg[a] && L / p <= kDftRadix;
         }
         moff += 3 * cells; koff += 3 * modes;
         c->dmax_cells = std::max(c->dmax_cells, cells);
@@ -2556,18 +2755,42 @@ int demag_setup(ising_ctx* c)
         c->dmax_sites = std::max(c->dmax_sites, r.n);
         c->dreps.push_back(d);
     }
+    // lines per workgroup, once the kernel of each axis is known: one item per
+    // thread (k_dft_lines_r) or two buffers within kDftLdsTarget (k_dft_lines)
+    for (size_t k = 0; k < c->reps.size(); ++k) {
+        const Rep& r = c->reps[k];
+        DemagRep& d = c->dreps[k];
+        if (r.per != 7 || std::max(r.lx, std::max(r.ly, r.lz)) > kDftMaxLen) continue;
+        const int64_t modes = (int64_t)d.hx * r.ly * r.lz;
+        const int64_t lines[3] = { 3 * (int64_t)r.ly * r.lz, 3 * modes / r.ly, 3 * modes / r.lz };
+        for (int a = 0; a < 3; ++a) {
+            const int L = a == 0 ? r.lx : (a == 1 ? r.ly : r.lz);
+            size_t lds;
+            if (c->dft_reg[a]) {
+                d.tl[a] = std::min(kDftMaxLines, kBlock / (L / d.fp[a]));
+                lds = (size_t)(L * (d.tl[a] + 1) + L) * sizeof(double2);
+            } else {
+                d.tl[a] = std::max(1, std::min(kDftMaxLines, (kDftLdsTarget - L) / (2 * L) - 1));
+                lds = (size_t)(2 * L * (d.tl[a] + 1) + L) * sizeof(double2);
+            }
+            c->dft_tiles[a] = std::max(c->dft_tiles[a], (lines[a] + d.tl[a] - 1) / d.tl[a]);
+            c->dft_lds[a] = std::max(c->dft_lds[a], lds);
+        }
+    }
     int rc;
     if ((rc = dalloc(c, &c->d_demag, (size_t)c->total_floats)) || (rc = dalloc(c, &c->d_dmesh, (size_t)moff)) ||
         (rc = dalloc(c, &c->d_dk1, (size_t)koff)) || (rc = dalloc(c, &c->d_dk2, (size_t)koff)) ||
         (rc = dalloc(c, &c->d_tw, tw.size())) || (rc = dalloc(c, &c->d_dreps, c->dreps.size())))
         return rc;
     // LDS beyond the default 64 KB per workgroup (axes longer than 512)
-    const void* fns[4] = { (const void*)k_dft_lines<kR2cX>, (const void*)k_dft_lines<kC2cY>,
-                           (const void*)k_dft_lines<kC2cZ>, (const void*)k_dft_lines<kC2rX> };
+    const void* fns[2][4] = { { (const void*)k_dft_lines<kR2cX>, (const void*)k_dft_lines<kC2cY>,
+                                (const void*)k_dft_lines<kC2cZ>, (const void*)k_dft_lines<kC2rX> },
+                              { (const void*)k_dft_lines_r<kR2cX>, (const void*)k_dft_lines_r<kC2cY>,
+                                (const void*)k_dft_lines_r<kC2cZ>, (const void*)k_dft_lines_r<kC2rX> } };
     const int fn_axis[4] = { 0, 1, 2, 0 };
     for (int f = 0; f < 4; ++f)
-        HIP_TRY(c, hipFuncSetAttribute(fns[f], hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)c->dft_lds[fn_axis[f]]));
+        HIP_TRY(c, hipFuncSetAttribute(fns[c->dft_reg[fn_axis[f]] ? 1 : 0][f],
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->dft_lds[fn_axis[f]]));
     HIP_TRY(c, hipMemcpyAsync(c->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_dreps, c->dreps.data(), c->dreps.size() * sizeof(DemagRep), hipMemcpyHostToDevice,
                               c->stream));
@@ -2591,20 +2814,29 @@ int launch_demag(ising_ctx* c, int only)
     hipLaunchKernelGGL(k_demag_deposit, grid(c->dmax_cells), dim3(kBlock), 0, s, (const float*)cur_spins(c),
                        c->d_reps, c->d_lads, dr, only, c->d_dmesh);
     auto lines = [&](int a) { return dim3((unsigned)std::max<int64_t>(1, c->dft_tiles[a]), R); };
-    hipLaunchKernelGGL(k_dft_lines<kR2cX>, lines(0), dim3(kBlock), c->dft_lds[0], s, (const double*)c->d_dmesh,
-                       c->d_reps, c->d_lads, dr, only, tw, -1, c->d_dk1);
-    hipLaunchKernelGGL(k_dft_lines<kC2cY>, lines(1), dim3(kBlock), c->dft_lds[1], s, (const double2*)c->d_dk1,
-                       c->d_reps, c->d_lads, dr, only, tw, -1, c->d_dk2);
-    hipLaunchKernelGGL(k_dft_lines<kC2cZ>, lines(2), dim3(kBlock), c->dft_lds[2], s, (const double2*)c->d_dk2,
-                       c->d_reps, c->d_lads, dr, only, tw, -1, c->d_dk1);
+    // one line transform; k_dft_lines_r where every slot's factors fit
+    auto dft = [&](auto mode, const auto* in, int sign, auto* out) {
+        constexpr int m = decltype(mode)::value;
+        constexpr int a = kDftAxis<m>;
+        if (c->dft_reg[a])
+            hipLaunchKernelGGL(k_dft_lines_r<m>, lines(a), dim3(kBlock), c->dft_lds[a], s, in, c->d_reps, c->d_lads,
+                               dr, only, tw, sign, out);
+        else
+            hipLaunchKernelGGL(k_dft_lines<m>, lines(a), dim3(kBlock), c->dft_lds[a], s, in, c->d_reps, c->d_lads,
+                               dr, only, tw, sign, out);
+    };
+    using R2c = std::integral_constant<int, kR2cX>;
+    using C2cY = std::integral_constant<int, kC2cY>;
+    using C2cZ = std::integral_constant<int, kC2cZ>;
+    using C2r = std::integral_constant<int, kC2rX>;
+    dft(R2c{}, (const double*)c->d_dmesh, -1, c->d_dk1);
+    dft(C2cY{}, (const double2*)c->d_dk1, -1, c->d_dk2);
+    dft(C2cZ{}, (const double2*)c->d_dk2, -1, c->d_dk1);
     hipLaunchKernelGGL(k_demag_project, grid(c->dmax_modes), dim3(kBlock), 0, s, c->d_dk1, c->d_reps, c->d_lads,
                        dr, only);
-    hipLaunchKernelGGL(k_dft_lines<kC2cZ>, lines(2), dim3(kBlock), c->dft_lds[2], s, (const double2*)c->d_dk1,
-                       c->d_reps, c->d_lads, dr, only, tw, +1, c->d_dk2);
-    hipLaunchKernelGGL(k_dft_lines<kC2cY>, lines(1), dim3(kBlock), c->dft_lds[1], s, (const double2*)c->d_dk2,
-                       c->d_reps, c->d_lads, dr, only, tw, +1, c->d_dk1);
-    hipLaunchKernelGGL(k_dft_lines<kC2rX>, lines(0), dim3(kBlock), c->dft_lds[0], s, (const double2*)c->d_dk1,
-                       c->d_reps, c->d_lads, dr, only, tw, +1, c->d_dmesh);
+    dft(C2cZ{}, (const double2*)c->d_dk1, +1, c->d_dk2);
+    dft(C2cY{}, (const double2*)c->d_dk2, +1, c->d_dk1);
+    dft(C2r{}, (const double2*)c->d_dk1, +1, c->d_dmesh);
     hipLaunchKernelGGL(k_demag_gather, grid(c->dmax_sites), dim3(kBlock), 0, s, (const double*)c->d_dmesh,
                        c->d_reps, c->d_lads, dr, only, c->d_demag);
     LAUNCH_CHECK(c);

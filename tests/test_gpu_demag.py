@@ -30,6 +30,7 @@ LATS = {
     # line transforms L = P Q: P = 3, 3, 4 and several workgroups per axis
     "sc_12x18x16": Lattice(12, 18, 16),
     "bcc_7x13x5": Lattice(7, 13, 5, cell="bcc"),    # prime lengths: P = 1
+    "sc_64x16x8": Lattice(64, 16, 8),                # 8 x 8, 4 x 4, 2 x 4: register FFTs
     # one line per workgroup; 1024 takes 80 KB of LDS
     "bcc_600x2x2": Lattice(600, 2, 2, cell="bcc"),
     "bcc_2x2x1024": Lattice(2, 2, 1024, cell="bcc"),
