@@ -37,7 +37,16 @@ colour launches per sweep, table-driven neighbours); 2000 timed sweeps,
 in-schedule kernel timing, and the FP64 CPU oracle on one replica of the
 same cell. With `--profile-sweeps N` they run only N plain sweeps.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|bcc|hk]
+`--workload demag` measures Metropolis_mcs with the demagnetising term on:
+64 replicas of 64^3 (T 0.2..2.0 geometric, J = 1, bulk D = 0.5, Bz = 0.2),
+the field of every replica recomputed before the first sweep and after every
+7th (the reference's int(pi sqrt 5 / (2 r)) at r = 0.5); 140 timed sweeps in
+one ising_sweep call. The roofline line is the field recomputation as a
+whole (deposit, nine transform and projection passes, gather), timed with a
+hipEvent pair (ising_time_demag), against the bytes its passes read and
+write once each.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|bcc|hk|demag]
 """
 from __future__ import annotations
 
@@ -119,7 +128,7 @@ def main() -> int:
                     help="with --profile-sweeps: use these launch-tuning flags instead of autotuning")
     ap.add_argument("--eager", action="store_true",
                     help="launch every period eagerly instead of replaying a captured hipGraph (same kernels)")
-    ap.add_argument("--workload", choices=("c5", "c3", "bcc", "hk"), default="c5",
+    ap.add_argument("--workload", choices=("c5", "c3", "bcc", "hk", "demag"), default="c5",
                     help="c5: 64^3 x 64 PT, learned vs fixed (default); c3: 256 replicas of mixed lattices; "
                          "bcc / hk: 64-replica PT on body-centred-cubic / hyper-kagome cells")
     ap.add_argument("--profile-sweeps", type=int, default=0,
@@ -139,6 +148,8 @@ def main() -> int:
         return bench_c3(args)
     if args.workload in ("bcc", "hk"):
         return bench_cell(args, args.workload)
+    if args.workload == "demag":
+        return bench_demag(args)
 
     T = geometric_temperatures(T_MIN, T_MAX, N_REP)
     ham = Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2, Kz=0.0)
@@ -440,6 +451,100 @@ def bench_cell(args, name: str) -> int:
                      "ms_per_launch": ms_launch, "algorithmic_bytes_per_launch": bytes_launch,
                      "tuning_flags": tune_flags,
                      "tuning_ms_per_launch": {str(f): float(t) for f, t in tune_ms.items()}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+DEMAG_INTERVAL = 7           # int(pi sqrt(5) / (2 r)) at r = 0.5 (src/energyMin.f90:258-259,362-369)
+DEMAG_SWEEPS = 140
+
+
+def demag_bytes_per_field(n_sites: int, lx: int, ly: int, lz: int) -> int:
+    """HBM bytes of one field recomputation of one replica when every pass
+    reads its input once and writes its output once: FP32 spins in (12 B per
+    site) and the FP64 field out (24 B per site); the FP64 real mesh written
+    by the deposit, read by the forward x transform, written by the inverse
+    one and read by the gather (4 x 24 B per cell); the half-complex mesh
+    written by the forward x transform, read and written by the y, z,
+    projection, z and y passes and read by the inverse x transform
+    (12 x 48 B per mode, (lx/2+1) ly lz modes)."""
+    cells = lx * ly * lz
+    modes = (lx // 2 + 1) * ly * lz
+    return 36 * n_sites + 96 * cells + 576 * modes
+
+
+def cpu_baseline_demag(seconds_hint: float = 10.0) -> dict:
+    """FP64 OpenMP oracle: Metropolis_mcs units of one field + 7 sweeps on one
+    64^3 replica (the field of the oracle is direct per-axis DFTs)."""
+    from oracle import pyoracle as orc
+    cores = min(16, os.cpu_count() or 1)
+    orc.set_threads(cores)
+    lat = orc.lattice(L, L, L)
+    h = orc.ham(1.0, 0.5, "bulk", 0.2, 0.0)
+    s = orc.init_random(lat, SEED, 0)
+    n = orc.nsites(lat)
+    units, dt = 0, 0.0
+    while dt < seconds_hint or units < 1:
+        t0 = time.perf_counter()
+        orc.sweeps_demag(lat, h, s, 2.0, CONE, SEED, 0, units * DEMAG_INTERVAL, DEMAG_INTERVAL, interval=0)
+        dt += time.perf_counter() - t0
+        units += 1
+    return {"value": n * DEMAG_INTERVAL * units / dt, "unit": "site updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 C/OpenMP oracle, 1 replica of 64^3, T=0.5, r=0.5: {units} x (field + "
+                      f"{DEMAG_INTERVAL} sweeps) in {dt:.1f} s"}
+
+
+def bench_demag(args) -> int:
+    """64 replicas of 64^3 with the demagnetising term: 140 timed sweeps with
+    the field recomputed every 7, and the field recomputation timed alone."""
+    from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
+    T = geometric_temperatures(T_MIN, T_MAX, N_REP)
+    lat = Lattice(L, L, L)
+    eng = Engine([Ladder(lat, Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2, demag=True), list(1.0 / T), CONE)],
+                 seed=SEED)
+    eng.init_random()
+    eng.set_demag_interval(DEMAG_INTERVAL)
+    n_sites = lat.sites * N_REP
+    eng.sweep(2 * DEMAG_INTERVAL)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.sweep(DEMAG_SWEEPS)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    progress(f"demag timed: {wall * 1e3:.1f} ms for {DEMAG_SWEEPS} sweeps")
+    eng.time_demag(2)
+    ms_field = eng.time_demag(10)
+    e = eng.energy()
+    eng.close()
+    bytes_field = demag_bytes_per_field(lat.sites, L, L, L) * N_REP
+    achieved = bytes_field / (ms_field * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline_demag()
+    line = {
+        "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
+        "value": n_sites * DEMAG_SWEEPS / wall,
+        "unit": "site updates/s",
+        "n_gpus": 1,
+        "steps": DEMAG_SWEEPS,
+        "warmup": 2 * DEMAG_INTERVAL,
+        "ms_per_step": wall * 1e3 / DEMAG_SWEEPS,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 spins, fp64 field and reductions",
+        "data": "synthetic (seeded random-sphere start)",
+        "config": {"workload": "Metropolis_mcs with the demagnetising term: 64 replicas of 64^3, T 0.2-2.0 "
+                               "geometric, J=1 D=0.5 (bulk) Bz=0.2, r=0.5, field recomputed before the first "
+                               f"sweep and after every {DEMAG_INTERVAL}th", "sites": n_sites,
+                   "step": "one sweep of every replica with its share of field recomputations"},
+        "ms_per_field": ms_field,
+        "energy_per_site": (np.asarray(e)[::16] / lat.sites).tolist(),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "field recomputation (k_demag_deposit, k_dft_lines_r x6, k_demag_project, "
+                               "k_demag_gather)", "ms_per_launch": ms_field,
+                     "algorithmic_bytes_per_launch": bytes_field},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -248,6 +248,12 @@ int  ising_time_sweeps(ising_ctx* ctx, int n_sweeps, double* ms_per_launch);
  * inside the workload it belongs to (back-to-back launches alone run at a
  * lower clock under the board's power limit). */
 int  ising_time_pt_sweeps(ising_ctx* ctx, const ising_schedule* sched, int n_sweeps, double* ms_per_launch);
+/* Recomputes the demagnetising field of every replica whose ladder has
+ * demag = 1 from its current spins, n_fields times back to back, with one
+ * hipEvent pair around them on the context's stream; *ms_per_field = elapsed
+ * / n_fields. The stored fields are left as one recomputation leaves them.
+ * ISING_ERR_INVALID_ARG when no ladder has the term. */
+int  ising_time_demag(ising_ctx* ctx, int n_fields, double* ms_per_field);
 /* hipDeviceSynchronize on the context's device. */
 int  ising_synchronize(ising_ctx* ctx);
 /* Plain device-to-device copy kernel over `bytes` bytes (two scratch buffers

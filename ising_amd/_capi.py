@@ -108,6 +108,7 @@ SIGNATURES = {
     "ising_accept_counts": (C.c_int, [_P, _i64p]),
     "ising_time_sweeps": (C.c_int, [_P, C.c_int, _dp]),
     "ising_time_pt_sweeps": (C.c_int, [_P, C.POINTER(ising_schedule), C.c_int, _dp]),
+    "ising_time_demag": (C.c_int, [_P, C.c_int, _dp]),
     "ising_synchronize": (C.c_int, [_P]),
     "ising_bandwidth_probe": (C.c_int, [_P, C.c_int64, _dp]),
     "ising_set_tuning": (C.c_int, [_P, C.c_uint32]),

@@ -3704,6 +3704,29 @@ int ising_time_sweeps(ising_ctx* c, int n_sweeps, double* ms_per_launch)
     return ISING_OK;
 }
 
+int ising_time_demag(ising_ctx* c, int n_fields, double* ms_per_field)
+{
+    if (!c || !ms_per_field || n_fields < 1) return ISING_ERR_INVALID_ARG;
+    if (!c->any_demag) return set_err(c, ISING_ERR_INVALID_ARG, "no ladder has the demagnetising term");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = demag_setup(c);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
+    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
+    for (int k = 0; k < n_fields && !rc; ++k) rc = launch_demag(c, kDemagLadders);
+    if (!rc && hipEventRecord(e1, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
+    float ms = 0.f;
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "sync");
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *ms_per_field = (double)ms / n_fields;
+    return ISING_OK;
+}
+
 int ising_time_pt_sweeps(ising_ctx* c, const ising_schedule* sched, int n_sweeps, double* ms_per_launch)
 {
     if (!c || !sched || !ms_per_launch || n_sweeps < 1) return ISING_ERR_INVALID_ARG;

@@ -350,6 +350,13 @@ class Engine:
         self._check(self._lib.ising_time_pt_sweeps(self._h, C.byref(sch), int(n_sweeps), C.byref(ms)))
         return ms.value
 
+    def time_demag(self, n_fields: int) -> float:
+        """Mean device time (ms) of one recomputation of the demagnetising
+        field of every replica whose ladder has the term."""
+        ms = C.c_double()
+        self._check(self._lib.ising_time_demag(self._h, int(n_fields), C.byref(ms)))
+        return ms.value
+
     def bandwidth_probe(self, nbytes: int = 256 << 20) -> float:
         """Device time (ms) of one plain copy of nbytes (read nbytes + write nbytes)."""
         ms = C.c_double()

@@ -401,6 +401,13 @@ module ising_gpu_c
                         real(c_double), intent(out) :: ms
                         integer(c_int) :: rc
                 end function
+                function ising_time_demag(ctx, n_fields, ms) bind(C, name="ising_time_demag") result(rc)
+                        import :: c_ptr, c_int, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: n_fields
+                        real(c_double), intent(out) :: ms
+                        integer(c_int) :: rc
+                end function
                 function ising_demag_field(ctx, replica, h, n) bind(C, name="ising_demag_field") result(rc)
                         import :: c_ptr, c_int, c_int64_t, c_double
                         type(c_ptr), value :: ctx

@@ -195,6 +195,23 @@ def test_fused_flag_then_demag_sweeps_in_place():
     assert int(np.sum(np.max(np.abs(ref2 - got2), axis=0) > 1e-3)) <= 2
 
 
+def test_time_demag():
+    """ising_time_demag recomputes the field of every replica with the term
+    from its current spins and leaves it stored; refused without the term."""
+    lat = Lattice(8, 6, 4)
+    lads = [Ladder(lat, Hamiltonian(J=1.0, D=0.3, demag=True), [1.0, 0.5]), Ladder(lat, Hamiltonian(J=1.0), [0.7])]
+    with Engine(lads, seed=5) as e:
+        e.init_random()
+        assert e.time_demag(3) > 0
+        for r in range(2):
+            want = orc.demag_field(olat(lat), e.get_spins(r))
+            assert np.max(np.abs(e.stored_demag_field(r) - want)) < 1e-12 * max(1.0, np.max(np.abs(want)))
+        assert np.max(np.abs(e.stored_demag_field(2))) == 0.0
+    with engine(lat, Hamiltonian(J=1.0), [1.0]) as e:
+        with pytest.raises(IsingError):
+            e.time_demag(1)
+
+
 def test_refusals():
     with pytest.raises(IsingError) as ex:
         engine(Lattice(4, 4, 4, (True, True, False)), Hamiltonian(demag=True), [1.0])
