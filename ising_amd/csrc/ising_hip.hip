@@ -2789,8 +2789,9 @@ int demag_setup(ising_ctx* c)
                                 (const void*)k_dft_lines_r<kC2cZ>, (const void*)k_dft_lines_r<kC2rX> } };
     const int fn_axis[4] = { 0, 1, 2, 0 };
     for (int f = 0; f < 4; ++f)
-        HIP_TRY(c, hipFuncSetAttribute(fns[c->dft_reg[fn_axis[f]] ? 1 : 0][f],
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->dft_lds[fn_axis[f]]));
+        if (c->dft_lds[fn_axis[f]] > 0)
+            HIP_TRY(c, hipFuncSetAttribute(fns[c->dft_reg[fn_axis[f]] ? 1 : 0][f],
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->dft_lds[fn_axis[f]]));
     HIP_TRY(c, hipMemcpyAsync(c->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->d_dreps, c->dreps.data(), c->dreps.size() * sizeof(DemagRep), hipMemcpyHostToDevice,
                               c->stream));
