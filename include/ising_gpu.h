@@ -35,7 +35,8 @@
  * field of calculate_demagnetisation_field (src/reciprocalSpaceProcesses.f90:
  * 202-281) in lattice units: cloud-in-cell deposit onto one mesh point per
  * cell centre, H(k) = -(k.M(k)) k / k^2 with the k = 0 mode removed, the
- * same weights back to the atoms. It needs a fully periodic lattice.
+ * same weights back to the atoms. It needs a fully periodic lattice of at
+ * most 1024 cells per axis (ISING_ERR_UNSUPPORTED otherwise).
  */
 #ifndef ISING_GPU_H
 #define ISING_GPU_H
