@@ -1684,18 +1684,17 @@ __device__ __forceinline__ bool demag_slot(const Rep& R, const Lad* __restrict__
 }
 
 // One step of getNeighboringCells (src/chainMesh.f90:98-117): n -> 0, -1 -> n-1.
-__device__ __forceinline__ int wrap_cell(int i, int n) { return i == n ? 0 : (i == -1 ? n - 1 : i); }
+__host__ __device__ inline int wrap_cell(int i, int n) { return i == n ? 0 : (i == -1 ? n - 1 : i); }
 
-// Weight factor of one axis: d = atom - centre in lattice units, minimum
-// image d - n nint(d/n) (nint half away from zero; distance_points_vec_bravais,
-// src/chainMesh.f90:268-270), then max(1 - |d|, 0). nint(d/n) is 0 unless
-// |d| >= n/2 (d is a multiple of 1/4 on every cell here, so d/n rounds to 1/2
-// only when it is 1/2): the division runs only on wrapped bonds.
-__device__ __forceinline__ double cic_axis(int atom_cell, double frac, int centre_cell, int n)
+// Weight factor of one axis (host): d = atom - centre in lattice units,
+// minimum image d - n nint(d/n) (nint half away from zero;
+// distance_points_vec_bravais, src/chainMesh.f90:268-270), then
+// max(1 - |d|, 0).
+inline double cic_axis(int atom_cell, double frac, int centre_cell, int n)
 {
     double d = ((double)atom_cell + frac) - ((double)centre_cell + 0.5);
-    if (fabs(d) >= 0.5 * (double)n) d -= (double)n * round(d / (double)n);
-    return fmax(1.0 - fabs(d), 0.0);
+    d -= (double)n * std::round(d / (double)n);
+    return std::fmax(1.0 - std::fabs(d), 0.0);
 }
 
 // Colour-major index colour*half + q of atom a of cell (i, j, k).
@@ -1708,23 +1707,27 @@ __device__ __forceinline__ uint32_t atom_index(const Rep& R, int i, int j, int k
     return (uint32_t)((i + j + k) & 1) * R.half + row * (uint32_t)R.hx + ((uint32_t)i >> 1);
 }
 
-// Atoms per cell and the position of atom a in its cell along `axis` (units
-// of the edge): the corner on SC, corner and body centre on bcc, the table's
-// fractions on hyper-kagome (down to -3/4: such an atom still deposits only
-// on the 27 cells around its own, as the reference's loop does).
+// Atoms per cell.
 __device__ __forceinline__ int atoms_per_cell(const Rep& R)
 {
     return R.cell == ISING_CELL_HYPERKAGOME ? kHkAtoms : (R.cell == ISING_CELL_BCC ? 2 : 1);
 }
-__device__ __forceinline__ double atom_frac(const Rep& R, int a, int axis)
-{
-    if (R.cell == ISING_CELL_HYPERKAGOME) return c_hk.frac[a][axis];
-    return 0.5 * (double)a;
-}
+
+// Weight factors per slot, atom type and axis: cic[((slot * kHkAtoms + a) * 3
+// + axis) * 3 + di + 1] = cic_axis(c + di, frac, c, n) for an atom of type a
+// in cell c + di (wrapped) and the mesh point of cell c. The minimum image
+// takes the wrap back out, so the factor is the same for every cell c; the
+// host fills the table with the same formula (every operand a multiple of
+// 1/4: exact). Atom positions: the corner on SC, corner and body centre on
+// bcc, the table's fractions on hyper-kagome (down to -3/4: such an atom
+// still deposits only on the 27 cells around its own, as the reference's
+// loop does). The gather's factor for the mesh point of cell c + di seen
+// from an atom in cell c is the entry of -di.
+constexpr int kCicPerSlot = kHkAtoms * 9;
 
 __global__ __launch_bounds__(kBlock) void k_demag_deposit(
     const float* __restrict__ spins, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
-    const DemagRep* __restrict__ dreps, int only, double* __restrict__ mesh)
+    const DemagRep* __restrict__ dreps, const double* __restrict__ cic, int only, double* __restrict__ mesh)
 {
     const Rep R = reps[blockIdx.y];
     if (!demag_slot(R, lads, only)) return;
@@ -1739,12 +1742,13 @@ __global__ __launch_bounds__(kBlock) void k_demag_deposit(
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
 #pragma unroll 1
     for (int a = 0; a < na; ++a) {
-        // the weight is a product of one factor per axis: 9 factors per atom
+        // the weight is a product of one factor per axis (uniform loads)
+        const double* w = cic + ((size_t)blockIdx.y * kHkAtoms + a) * 9;
         double fx[3], fy[3], fz[3];
         for (int d = 0; d < 3; ++d) {
-            fx[d] = cic_axis(wrap_cell(i + d - 1, nx), atom_frac(R, a, 0), i, nx);
-            fy[d] = cic_axis(wrap_cell(j + d - 1, ny), atom_frac(R, a, 1), j, ny);
-            fz[d] = cic_axis(wrap_cell(k + d - 1, nz), atom_frac(R, a, 2), k, nz);
+            fx[d] = w[d];
+            fy[d] = w[3 + d];
+            fz[d] = w[6 + d];
         }
 #pragma unroll 1
         for (int di = 0; di < 3; ++di) {
@@ -2133,65 +2137,84 @@ __global__ __launch_bounds__(kBlock) void k_demag_project(
     f[2 * Nk] = double2{ -dr * k2, -di * k2 };
 }
 
+// Sum of the mesh field over the 27 cells around the atom's cell (ix, iy,
+// iz) with the factors cw of its type.
+__device__ __forceinline__ void gather_sum(const Rep& R, const double* __restrict__ m, const double* __restrict__ cw,
+                                           int ix, int iy, int iz, double (&h)[3])
+{
+    const int nx = R.lx, ny = R.ly, nz = R.lz;
+    const int64_t Nc = (int64_t)nx * ny * nz;
+    h[0] = h[1] = h[2] = 0.0;
+    for (int di = -1; di <= 1; ++di) {
+        const double wx = cw[1 - di];
+        if (wx == 0.0) continue;
+        const int ci = wrap_cell(ix + di, nx);
+        for (int dj = -1; dj <= 1; ++dj) {
+            const double wy = cw[3 + 1 - dj];
+            if (wy == 0.0) continue;
+            const int cj = wrap_cell(iy + dj, ny);
+            for (int dk = -1; dk <= 1; ++dk) {
+                const double w = wx * wy * cw[6 + 1 - dk];
+                if (w == 0.0) continue;
+                const int64_t cell = ((int64_t)wrap_cell(iz + dk, nz) * ny + cj) * nx + ci;
+                h[0] += w * m[cell];
+                h[1] += w * m[Nc + cell];
+                h[2] += w * m[2 * Nc + cell];
+            }
+        }
+    }
+}
+
 // H of every atom: the mesh field of the 27 cells around the atom's cell with
 // the deposit's weights; written in the spin layout (colour-major t).
 __global__ __launch_bounds__(kBlock) void k_demag_gather(
     const double* __restrict__ mesh, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
-    const DemagRep* __restrict__ dreps, int only, double* __restrict__ H)
+    const DemagRep* __restrict__ dreps, const double* __restrict__ cic, int only, double* __restrict__ H)
 {
     const Rep R = reps[blockIdx.y];
     if (!demag_slot(R, lads, only)) return;
     const int64_t tt = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (tt >= R.n) return;
-    const uint32_t t = (uint32_t)tt;
-    int ix, iy, iz;
-    double f[3];
-    if (R.cell == ISING_CELL_HYPERKAGOME) {
+    uint32_t t = (uint32_t)tt;    // colour-major index of the atom
+    int ix, iy, iz, type;
+    if (R.cell == ISING_CELL_SC) {
+        // lanes in x-fastest site order, so that each wavefront reads whole
+        // mesh rows (in colour-major order they would take every other cell)
+        const uint32_t row = t / (uint32_t)R.lx;
+        ix = (int)(t - row * (uint32_t)R.lx);
+        iz = (int)(row / (uint32_t)R.ly);
+        iy = (int)(row - (uint32_t)iz * (uint32_t)R.ly);
+        type = 0;
+        t = atom_index(R, ix, iy, iz, 0);
+    } else if (R.cell == ISING_CELL_HYPERKAGOME) {
         const uint32_t C = (uint32_t)R.plane * (uint32_t)R.lz;
         const uint32_t rank = t / C, cell = t - rank * C;
         const uint32_t row = cell / (uint32_t)R.lx;
         ix = (int)(cell - row * (uint32_t)R.lx);
         iz = (int)(row / (uint32_t)R.ly);
         iy = (int)(row - (uint32_t)iz * (uint32_t)R.ly);
-        const int type = c_hk.type_of_rank[rank];
-        for (int a = 0; a < 3; ++a) f[a] = c_hk.frac[type][a];
-    } else {
+        type = c_hk.type_of_rank[rank];
+    } else {                      // bcc: atom = colour, rows of lx cells
         const uint32_t colour = t >= R.half ? 1u : 0u;
         const uint32_t q = t - colour * R.half;
-        const uint32_t row = q / (uint32_t)R.hx, h = q - row * (uint32_t)R.hx;
+        const uint32_t row = q / (uint32_t)R.hx;
+        ix = (int)(q - row * (uint32_t)R.hx);
         iz = (int)(row / (uint32_t)R.ly);
         iy = (int)(row - (uint32_t)iz * (uint32_t)R.ly);
-        const bool bcc = R.cell == ISING_CELL_BCC;
-        ix = bcc ? (int)h : (int)(2u * h + (((uint32_t)(iy + iz) + colour) & 1u));
-        f[0] = f[1] = f[2] = bcc && colour ? 0.5 : 0.0;
+        type = (int)colour;
     }
-    const int nx = R.lx, ny = R.ly, nz = R.lz;
-    const int64_t Nc = (int64_t)nx * ny * nz;
     const double* m = mesh + dreps[blockIdx.y].moff;
-    double h0 = 0.0, h1 = 0.0, h2 = 0.0;
-    for (int di = -1; di <= 1; ++di) {
-        const int ci = wrap_cell(ix + di, nx);
-        const double wx = cic_axis(ix, f[0], ci, nx);
-        if (wx == 0.0) continue;
-        for (int dj = -1; dj <= 1; ++dj) {
-            const int cj = wrap_cell(iy + dj, ny);
-            const double wy = cic_axis(iy, f[1], cj, ny);
-            if (wy == 0.0) continue;
-            for (int dk = -1; dk <= 1; ++dk) {
-                const int ck = wrap_cell(iz + dk, nz);
-                const double w = wx * wy * cic_axis(iz, f[2], ck, nz);
-                if (w == 0.0) continue;
-                const int64_t cell = ((int64_t)ck * ny + cj) * nx + ci;
-                h0 += w * m[cell];
-                h1 += w * m[Nc + cell];
-                h2 += w * m[2 * Nc + cell];
-            }
-        }
-    }
+    const double* base = cic + (size_t)blockIdx.y * kHkAtoms * 9;
+    // the atom type is the same on the whole wavefront except where it
+    // straddles two colours or ranks: there the factors are read per lane
+    const int t0 = __builtin_amdgcn_readfirstlane(type);
+    double h[3];
+    if (__builtin_amdgcn_ballot_w64(type != t0) == 0) gather_sum(R, m, base + t0 * 9, ix, iy, iz, h);
+    else gather_sum(R, m, base + type * 9, ix, iy, iz, h);
     double* o = H + R.off + t;
-    o[0] = h0;
-    o[R.n] = h1;
-    o[2 * R.n] = h2;
+    o[0] = h[0];
+    o[R.n] = h[1];
+    o[2 * R.n] = h[2];
 }
 
 } // namespace ising
@@ -2245,6 +2268,7 @@ struct ising_ctx {
     double2* d_dk1 = nullptr;    // half-complex meshes [3][modes] per slot (ping-pong)
     double2* d_dk2 = nullptr;
     double2* d_tw = nullptr;     // twiddle tables e^{-2 pi i m/n}
+    double* d_cic = nullptr;     // weight factors, kCicPerSlot per slot
     DemagRep* d_dreps = nullptr;
 
     // learned proposal
@@ -2755,6 +2779,20 @@ int demag_setup(ising_ctx* c)
         c->dmax_sites = std::max(c->dmax_sites, r.n);
         c->dreps.push_back(d);
     }
+    // weight factors of every slot, from cell 1 (any cell gives the same)
+    std::vector<double> cic(c->reps.size() * kCicPerSlot, 0.0);
+    for (size_t k = 0; k < c->reps.size(); ++k) {
+        const Rep& r = c->reps[k];
+        const int na = r.cell == ISING_CELL_HYPERKAGOME ? kHkAtoms : (r.cell == ISING_CELL_BCC ? 2 : 1);
+        for (int a = 0; a < na; ++a)
+            for (int ax = 0; ax < 3; ++ax) {
+                const int n = ax == 0 ? r.lx : (ax == 1 ? r.ly : r.lz);
+                const double frac = r.cell == ISING_CELL_HYPERKAGOME ? hk_table()->frac[a][ax] : 0.5 * a;
+                const int centre = 1 % n;
+                for (int d = 0; d < 3; ++d)
+                    cic[(k * kHkAtoms + a) * 9 + ax * 3 + d] = cic_axis(wrap_cell(centre + d - 1, n), frac, centre, n);
+            }
+    }
     // lines per workgroup, once the kernel of each axis is known: one item per
     // thread (k_dft_lines_r) or two buffers within kDftLdsTarget (k_dft_lines)
     for (size_t k = 0; k < c->reps.size(); ++k) {
@@ -2780,8 +2818,10 @@ int demag_setup(ising_ctx* c)
     int rc;
     if ((rc = dalloc(c, &c->d_demag, (size_t)c->total_floats)) || (rc = dalloc(c, &c->d_dmesh, (size_t)moff)) ||
         (rc = dalloc(c, &c->d_dk1, (size_t)koff)) || (rc = dalloc(c, &c->d_dk2, (size_t)koff)) ||
-        (rc = dalloc(c, &c->d_tw, tw.size())) || (rc = dalloc(c, &c->d_dreps, c->dreps.size())))
+        (rc = dalloc(c, &c->d_tw, tw.size())) || (rc = dalloc(c, &c->d_dreps, c->dreps.size())) ||
+        (rc = dalloc(c, &c->d_cic, cic.size())))
         return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->d_cic, cic.data(), cic.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     // LDS beyond the default 64 KB per workgroup (axes longer than 512)
     const void* fns[2][4] = { { (const void*)k_dft_lines<kR2cX>, (const void*)k_dft_lines<kC2cY>,
                                 (const void*)k_dft_lines<kC2cZ>, (const void*)k_dft_lines<kC2rX> },
@@ -2813,7 +2853,7 @@ int launch_demag(ising_ctx* c, int only)
     const DemagRep* dr = c->d_dreps;
     const double2* tw = c->d_tw;
     hipLaunchKernelGGL(k_demag_deposit, grid(c->dmax_cells), dim3(kBlock), 0, s, (const float*)cur_spins(c),
-                       c->d_reps, c->d_lads, dr, only, c->d_dmesh);
+                       c->d_reps, c->d_lads, dr, (const double*)c->d_cic, only, c->d_dmesh);
     auto lines = [&](int a) { return dim3((unsigned)std::max<int64_t>(1, c->dft_tiles[a]), R); };
     // one line transform; k_dft_lines_r where every slot's factors fit
     auto dft = [&](auto mode, const auto* in, int sign, auto* out) {
@@ -2839,7 +2879,7 @@ int launch_demag(ising_ctx* c, int only)
     dft(C2cY{}, (const double2*)c->d_dk2, +1, c->d_dk1);
     dft(C2r{}, (const double2*)c->d_dk1, +1, c->d_dmesh);
     hipLaunchKernelGGL(k_demag_gather, grid(c->dmax_sites), dim3(kBlock), 0, s, (const double*)c->d_dmesh,
-                       c->d_reps, c->d_lads, dr, only, c->d_demag);
+                       c->d_reps, c->d_lads, dr, (const double*)c->d_cic, only, c->d_demag);
     LAUNCH_CHECK(c);
     return ISING_OK;
 }
@@ -3071,7 +3111,7 @@ void ising_destroy(ising_ctx* c)
                      c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_grad_part, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de,
                      c->ck_spins, c->ck_sot, c->ck_bslot, c->ck_btemp, c->ck_ctr,
                      c->d_fblock_rep, c->d_acc_wave_f, c->d_acc_base, c->d_wave_part_f,
-                     c->d_demag, c->d_dmesh, c->d_dk1, c->d_dk2, c->d_tw, c->d_dreps };
+                     c->d_demag, c->d_dmesh, c->d_dk1, c->d_dk2, c->d_tw, c->d_dreps, c->d_cic };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
