@@ -352,8 +352,9 @@ def bench_c3(args) -> int:
                             "min": float((acc / np.maximum(att, 1))[att > 0].min())},
         "energy_per_site_first_ladder": (out["energy"][-1][:32:8] / ladders[0].lattice.sites).tolist(),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_half_sweep",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_half_sweep",
                      "ms_per_launch": ms_launch, "algorithmic_bytes_per_launch": bytes_launch,
+                     "traffic_source": "profiles/sweep_multiatom_cells.md (FETCH_SIZE x2, WRITE_SIZE x1)",
                      "tuning_flags": tune_flags,
                      "tuning_ms_per_half_sweep": {str(f): float(t) for f, t in tune_ms.items()}},
         "cpu_baseline": None,
@@ -426,6 +427,7 @@ def bench_cell(args, name: str) -> int:
     eng.close()
     bytes_launch = BYTES_PER_UPDATE * n_sites / launches
     achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
+    traffic = counters_from_profiles().get("cells", {}).get(name, {}).get("hbm_bytes_per_launch")
     cpu = None if args.no_cpu else cpu_baseline_cell(cell, {"bcc": 24, "hk": 12}[name])
     line = {
         "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
@@ -520,6 +522,7 @@ def bench_demag(args) -> int:
     eng.close()
     bytes_field = demag_bytes_per_field(lat.sites, L, L, L) * N_REP
     achieved = bytes_field / (ms_field * 1e-3) / 1e9
+    traffic = counters_from_profiles().get("demag", {}).get("hbm_bytes_per_field")
     cpu = None if args.no_cpu else cpu_baseline_demag()
     line = {
         "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
@@ -541,7 +544,8 @@ def bench_demag(args) -> int:
         "ms_per_field": ms_field,
         "energy_per_site": (np.asarray(e)[::16] / lat.sites).tolist(),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/demag_field_pt64x64.md (FETCH_SIZE x2, WRITE_SIZE x1)",
                      "kernel": "field recomputation (k_demag_deposit, k_dft_lines_r x6, k_demag_project, "
                                "k_demag_gather)", "ms_per_launch": ms_field,
                      "algorithmic_bytes_per_launch": bytes_field},
