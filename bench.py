@@ -352,9 +352,8 @@ def bench_c3(args) -> int:
                             "min": float((acc / np.maximum(att, 1))[att > 0].min())},
         "energy_per_site_first_ladder": (out["energy"][-1][:32:8] / ladders[0].lattice.sites).tolist(),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_half_sweep",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_half_sweep",
                      "ms_per_launch": ms_launch, "algorithmic_bytes_per_launch": bytes_launch,
-                     "traffic_source": "profiles/sweep_multiatom_cells.md (FETCH_SIZE x2, WRITE_SIZE x1)",
                      "tuning_flags": tune_flags,
                      "tuning_ms_per_half_sweep": {str(f): float(t) for f, t in tune_ms.items()}},
         "cpu_baseline": None,
@@ -449,8 +448,9 @@ def bench_cell(args, name: str) -> int:
                             "min": float((acc / np.maximum(att, 1))[att > 0].min())},
         "energy_per_site": (out["energy"][-1][::16] / lat.sites).tolist(),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_half_sweep",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_half_sweep",
                      "ms_per_launch": ms_launch, "algorithmic_bytes_per_launch": bytes_launch,
+                     "traffic_source": "profiles/sweep_multiatom_cells.md (FETCH_SIZE x2, WRITE_SIZE x1)",
                      "tuning_flags": tune_flags,
                      "tuning_ms_per_launch": {str(f): float(t) for f, t in tune_ms.items()}},
         "cpu_baseline": cpu,
