@@ -1804,16 +1804,17 @@ template <int kMode> using DftIn = std::conditional_t<kMode == kR2cX, double, do
 template <int kMode> constexpr int kDftAxis = kMode == kC2cY ? 1 : (kMode == kC2cZ ? 2 : 0);
 template <int kMode> using DftOut = std::conditional_t<kMode == kC2rX, double, double2>;
 
-// Lines of this workgroup: sets L, P, Q, T, nt and the element stride,
-// fills the twiddles w[L] and the first element of every line (s_in in the
-// source mesh, s_out in the destination), then loads the lines into
-// a[n * (T + 1) + t]. False (for the whole workgroup) when the slot is
-// skipped or has no line here.
+// Geometry of a workgroup's lines: length, factors L = P Q, lines per
+// workgroup T, lines present nt, element stride.
 struct DftLines {
     int L, P, Q, T, nt;
     int64_t stride;
 };
 
+// Lines of this workgroup: sets g, fills the twiddles w[L] and the first
+// element of every line (s_in in the source mesh, s_out in the destination),
+// then loads the lines into a[n * (T + 1) + t]. False (for the whole
+// workgroup) when it has no line.
 template <int kMode>
 __device__ __forceinline__ bool dft_load(const Rep& R, const DemagRep& D, const DftIn<kMode>* __restrict__ in,
                                          const double2* __restrict__ tw, double2* a, double2* w, int64_t* s_in,
