@@ -1665,6 +1665,7 @@ struct DemagRep {
     int32_t hx;          // lx/2 + 1
     int32_t fp[3];       // P of L = P Q per axis
     int32_t tl[3];       // lines per workgroup per axis
+    int32_t tz;          // z lines per workgroup of k_demag_zproject
 };
 
 constexpr int kDftMaxLen = 1024;     // longest axis of a lattice with the field
@@ -2107,6 +2108,26 @@ __global__ __launch_bounds__(kBlock) void k_dft_lines_r(
 // wx = kx, wy and wz folded to (-n/2, n/2]; k = 2 pi w / n per axis (cubic
 // cells in lattice units, so the reference's cutoff |k_a| > pi/a never
 // removes a mode).
+__device__ __forceinline__ void project_mode(int kx, int ky, int kz, int nx, int ny, int nz, double2& a, double2& b,
+                                             double2& c)
+{
+    if (kx == 0 && ky == 0 && kz == 0) {
+        a = b = c = double2{ 0.0, 0.0 };
+        return;
+    }
+    const int wy = ky > ny / 2 ? ky - ny : ky, wz = kz > nz / 2 ? kz - nz : kz;
+    const double two_pi = 6.283185307179586476925;
+    const double k0 = two_pi * ((double)kx / (double)nx);
+    const double k1 = two_pi * ((double)wy / (double)ny);
+    const double k2 = two_pi * ((double)wz / (double)nz);
+    const double inv = 1.0 / (k0 * k0 + k1 * k1 + k2 * k2);
+    const double dr = (k0 * a.x + k1 * b.x + k2 * c.x) * inv;
+    const double di = (k0 * a.y + k1 * b.y + k2 * c.y) * inv;
+    a = double2{ -dr * k0, -di * k0 };
+    b = double2{ -dr * k1, -di * k1 };
+    c = double2{ -dr * k2, -di * k2 };
+}
+
 __global__ __launch_bounds__(kBlock) void k_demag_project(
     double2* __restrict__ F, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const DemagRep* __restrict__ dreps, int only)
@@ -2120,22 +2141,130 @@ __global__ __launch_bounds__(kBlock) void k_demag_project(
     if (e >= Nk) return;
     const int kx = (int)(e % D.hx), ky = (int)((e / D.hx) % ny), kz = (int)(e / ((int64_t)D.hx * ny));
     double2* f = F + D.koff + e;
-    if (kx == 0 && ky == 0 && kz == 0) {
-        f[0] = double2{ 0.0, 0.0 }; f[Nk] = double2{ 0.0, 0.0 }; f[2 * Nk] = double2{ 0.0, 0.0 };
-        return;
+    project_mode(kx, ky, kz, nx, ny, nz, f[0], f[Nk], f[2 * Nk]);
+}
+
+// One transform of nc components x T lines in LDS, in place and in natural
+// order: a[(c L + n) TP + t] -> X[k] at (c L + k) TP + t, by the two steps of
+// k_dft_lines_r (P, Q <= kDftRadix, nc Q T <= kBlock). Every thread of the
+// workgroup calls it (it holds barriers).
+__device__ __forceinline__ void dft_lines_lds(double2* a, const double2* w, int L, int P, int Q, int T, int nc,
+                                              double sg)
+{
+    const int TP = T + 1;
+    double2 x[kDftRadix];
+    // first step: item (c, n2, t) takes x[Q n1 + n2], n1 < P
+    {
+        const int t = threadIdx.x % T, r = threadIdx.x / T;
+        const int c = r / Q, n2 = r - c * Q;
+        const bool on = c < nc;
+        double2* line = a + (size_t)c * L * TP + t;
+#pragma unroll
+        for (int n1 = 0; n1 < kDftRadix; ++n1)
+            if (on && n1 < P) x[n1] = line[(Q * n1 + n2) * TP];
+        __syncthreads();
+        if (on && fft_reg(x, P, w, Q, sg)) {
+#pragma unroll
+            for (int k1 = 0; k1 < kDftRadix; ++k1) {
+                if (k1 < P) {
+                    const double2 u = w[n2 * k1];
+                    const double ui = sg * u.y;
+                    line[(n2 * P + k1) * TP] = double2{ x[k1].x * u.x - x[k1].y * ui, x[k1].x * ui + x[k1].y * u.x };
+                }
+            }
+        } else if (on) {
+            for (int k1 = 0; k1 < P; ++k1) {
+                double re = 0.0, im = 0.0;
+                int m = 0;
+#pragma unroll
+                for (int n1 = 0; n1 < kDftRadix; ++n1) {
+                    if (n1 < P) {
+                        cmac(re, im, x[n1], w[m * Q], sg);
+                        m += k1;
+                        if (m >= P) m -= P;
+                    }
+                }
+                const double2 u = w[n2 * k1];
+                const double ui = sg * u.y;
+                line[(n2 * P + k1) * TP] = double2{ re * u.x - im * ui, re * ui + im * u.x };
+            }
+        }
+        __syncthreads();
     }
-    const int wy = ky > ny / 2 ? ky - ny : ky, wz = kz > nz / 2 ? kz - nz : kz;
-    const double two_pi = 6.283185307179586476925;
-    const double k0 = two_pi * ((double)kx / (double)nx);
-    const double k1 = two_pi * ((double)wy / (double)ny);
-    const double k2 = two_pi * ((double)wz / (double)nz);
-    const double2 a = f[0], b = f[Nk], c = f[2 * Nk];
-    const double inv = 1.0 / (k0 * k0 + k1 * k1 + k2 * k2);
-    const double dr = (k0 * a.x + k1 * b.x + k2 * c.x) * inv;
-    const double di = (k0 * a.y + k1 * b.y + k2 * c.y) * inv;
-    f[0] = double2{ -dr * k0, -di * k0 };
-    f[Nk] = double2{ -dr * k1, -di * k1 };
-    f[2 * Nk] = double2{ -dr * k2, -di * k2 };
+    // second step: item (c, k1, t) takes Y[n2][k1], n2 < Q, writes X[k1 + P k2]
+    {
+        const int t = threadIdx.x % T, r = threadIdx.x / T;
+        const int c = r / P, k1 = r - c * P;
+        const bool on = c < nc;
+        double2* line = a + (size_t)c * L * TP + t;
+#pragma unroll
+        for (int j = 0; j < kDftRadix; ++j)
+            if (on && j < Q) x[j] = line[(j * P + k1) * TP];
+        __syncthreads();
+        if (on && fft_reg(x, Q, w, P, sg)) {
+#pragma unroll
+            for (int k2 = 0; k2 < kDftRadix; ++k2)
+                if (k2 < Q) line[(k1 + P * k2) * TP] = x[k2];
+        } else if (on) {
+            for (int k2 = 0; k2 < Q; ++k2) {
+                double re = 0.0, im = 0.0;
+                int m = 0;
+#pragma unroll
+                for (int j = 0; j < kDftRadix; ++j) {
+                    if (j < Q) {
+                        cmac(re, im, x[j], w[m * P], sg);
+                        m += k2;
+                        if (m >= Q) m -= Q;
+                    }
+                }
+                line[(k1 + P * k2) * TP] = double2{ re, im };
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The forward z transform, the projection and the inverse z transform in one
+// pass: T z lines (ky, kx) of all three components in LDS (D.tz lines per
+// workgroup, 3 Q T <= kBlock), the complex mesh overwritten in place with H.
+__global__ __launch_bounds__(kBlock) void k_demag_zproject(
+    double2* __restrict__ F, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
+    const DemagRep* __restrict__ dreps, int only, const double2* __restrict__ tw)
+{
+    const Rep R = reps[blockIdx.y];
+    if (!demag_slot(R, lads, only)) return;
+    const DemagRep D = dreps[blockIdx.y];
+    const int L = R.lz, P = D.fp[2], Q = L / P, T = D.tz, TP = T + 1;
+    const int64_t krow = (int64_t)R.ly * D.hx, Nk = krow * R.lz;
+    const int64_t first = (int64_t)blockIdx.x * T;
+    if (first >= krow) return;    // uniform per workgroup
+    const int nt = (int)min((int64_t)T, krow - first);
+    extern __shared__ double2 dft_lds[];
+    double2* a = dft_lds;                     // [3][L][TP]
+    double2* w = a + 3 * L * TP;              // [L]
+    for (int i = threadIdx.x; i < L; i += kBlock) w[i] = tw[D.tw[2] + i];
+    double2* f = F + D.koff + first;
+    for (int i = threadIdx.x; i < 3 * L * T; i += kBlock) {
+        const int t = i % T, r = i / T;       // r = c L + n
+        const int c = r / L, n = r - c * L;
+        a[r * TP + t] = t < nt ? f[c * Nk + n * krow + t] : double2{ 0.0, 0.0 };
+    }
+    __syncthreads();
+    dft_lines_lds(a, w, L, P, Q, T, 3, 1.0);
+    for (int i = threadIdx.x; i < L * T; i += kBlock) {
+        const int t = i % T, kz = i / T;
+        if (t >= nt) continue;
+        const int64_t r = first + t;
+        const int ky = (int)(r / D.hx), kx = (int)(r - (int64_t)ky * D.hx);
+        project_mode(kx, ky, kz, R.lx, R.ly, R.lz, a[kz * TP + t], a[(L + kz) * TP + t], a[(2 * L + kz) * TP + t]);
+    }
+    __syncthreads();
+    dft_lines_lds(a, w, L, P, Q, T, 3, -1.0);
+    for (int i = threadIdx.x; i < 3 * L * T; i += kBlock) {
+        const int t = i % T, r = i / T;
+        const int c = r / L, n = r - c * L;
+        if (t < nt) f[c * Nk + n * krow + t] = a[r * TP + t];
+    }
 }
 
 // Sum of the mesh field over the 27 cells around the atom's cell (ix, iy,
@@ -2262,6 +2391,8 @@ struct ising_ctx {
     std::vector<DemagRep> dreps;
     int64_t dmax_cells = 0, dmax_modes = 0, dmax_sites = 0;
     int64_t dft_tiles[3] = { 0, 0, 0 };   // workgroups per slot of the line transforms, per axis
+    int64_t dft_tiles_zp = 0;             // the same for k_demag_zproject (when dft_reg[2])
+    size_t dft_lds_zp = 0;
     bool dft_reg[3] = { true, true, true };   // k_dft_lines_r for that axis (every P, Q <= kDftRadix)
     size_t dft_lds[3] = { 0, 0, 0 };      // their dynamic LDS bytes
     double* d_demag = nullptr;   // FP64 field per site, spin layout (offsets of d_spins)
@@ -2815,6 +2946,12 @@ int demag_setup(ising_ctx* c)
             c->dft_tiles[a] = std::max(c->dft_tiles[a], (lines[a] + d.tl[a] - 1) / d.tl[a]);
             c->dft_lds[a] = std::max(c->dft_lds[a], lds);
         }
+        if (c->dft_reg[2]) {      // k_demag_zproject: 3 components x Q x tz items
+            d.tz = std::min(kDftMaxLines, kBlock / (3 * (r.lz / d.fp[2])));
+            const int64_t krow = (int64_t)r.ly * d.hx;
+            c->dft_tiles_zp = std::max(c->dft_tiles_zp, (krow + d.tz - 1) / d.tz);
+            c->dft_lds_zp = std::max(c->dft_lds_zp, (size_t)(3 * r.lz * (d.tz + 1) + r.lz) * sizeof(double2));
+        }
     }
     int rc;
     if ((rc = dalloc(c, &c->d_demag, (size_t)c->total_floats)) || (rc = dalloc(c, &c->d_dmesh, (size_t)moff)) ||
@@ -2829,6 +2966,9 @@ int demag_setup(ising_ctx* c)
                               { (const void*)k_dft_lines_r<kR2cX>, (const void*)k_dft_lines_r<kC2cY>,
                                 (const void*)k_dft_lines_r<kC2cZ>, (const void*)k_dft_lines_r<kC2rX> } };
     const int fn_axis[4] = { 0, 1, 2, 0 };
+    if (c->dft_reg[2] && c->dft_lds_zp > 0)
+        HIP_TRY(c, hipFuncSetAttribute((const void*)k_demag_zproject, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)c->dft_lds_zp));
     for (int f = 0; f < 4; ++f)
         if (c->dft_lds[fn_axis[f]] > 0)
             HIP_TRY(c, hipFuncSetAttribute(fns[c->dft_reg[fn_axis[f]] ? 1 : 0][f],
@@ -2873,10 +3013,15 @@ int launch_demag(ising_ctx* c, int only)
     using C2r = std::integral_constant<int, kC2rX>;
     dft(R2c{}, (const double*)c->d_dmesh, -1, c->d_dk1);
     dft(C2cY{}, (const double2*)c->d_dk1, -1, c->d_dk2);
-    dft(C2cZ{}, (const double2*)c->d_dk2, -1, c->d_dk1);
-    hipLaunchKernelGGL(k_demag_project, grid(c->dmax_modes), dim3(kBlock), 0, s, c->d_dk1, c->d_reps, c->d_lads,
-                       dr, only);
-    dft(C2cZ{}, (const double2*)c->d_dk1, +1, c->d_dk2);
+    if (c->dft_reg[2]) {          // z transforms and projection in one pass, in place
+        hipLaunchKernelGGL(k_demag_zproject, dim3((unsigned)std::max<int64_t>(1, c->dft_tiles_zp), R), dim3(kBlock),
+                           c->dft_lds_zp, s, c->d_dk2, c->d_reps, c->d_lads, dr, only, tw);
+    } else {
+        dft(C2cZ{}, (const double2*)c->d_dk2, -1, c->d_dk1);
+        hipLaunchKernelGGL(k_demag_project, grid(c->dmax_modes), dim3(kBlock), 0, s, c->d_dk1, c->d_reps,
+                           c->d_lads, dr, only);
+        dft(C2cZ{}, (const double2*)c->d_dk1, +1, c->d_dk2);
+    }
     dft(C2cY{}, (const double2*)c->d_dk2, +1, c->d_dk1);
     dft(C2r{}, (const double2*)c->d_dk1, +1, c->d_dmesh);
     hipLaunchKernelGGL(k_demag_gather, grid(c->dmax_sites), dim3(kBlock), 0, s, (const double*)c->d_dmesh,
