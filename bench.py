@@ -42,9 +42,10 @@ same cell. With `--profile-sweeps N` they run only N plain sweeps.
 the field of every replica recomputed before the first sweep and after every
 7th (the reference's int(pi sqrt 5 / (2 r)) at r = 0.5); 140 timed sweeps in
 one ising_sweep call. The roofline line is the field recomputation as a
-whole (deposit, nine transform and projection passes, gather), timed with a
-hipEvent pair (ising_time_demag), against the bytes its passes read and
-write once each.
+whole (deposit, x and y transforms, the fused z-transform/projection pass,
+inverse y and x transforms, gather), timed with a hipEvent pair
+(ising_time_demag), against the bytes its seven passes read and write once
+each.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|bcc|hk|demag]
 """
@@ -464,17 +465,18 @@ DEMAG_SWEEPS = 140
 
 
 def demag_bytes_per_field(n_sites: int, lx: int, ly: int, lz: int) -> int:
-    """HBM bytes of one field recomputation of one replica when every pass
-    reads its input once and writes its output once: FP32 spins in (12 B per
-    site) and the FP64 field out (24 B per site); the FP64 real mesh written
-    by the deposit, read by the forward x transform, written by the inverse
-    one and read by the gather (4 x 24 B per cell); the half-complex mesh
-    written by the forward x transform, read and written by the y, z,
-    projection, z and y passes and read by the inverse x transform
-    (12 x 48 B per mode, (lx/2+1) ly lz modes)."""
+    """HBM bytes of one field recomputation of one replica when each of its
+    seven passes reads its input once and writes its output once: FP32 spins
+    in (12 B per site) and the FP64 field out (24 B per site); the FP64 real
+    mesh written by the deposit, read by the forward x transform, written by
+    the inverse one and read by the gather (4 x 24 B per cell); the
+    half-complex mesh written by the forward x transform, read and written by
+    the forward y pass, the fused z-transform/projection pass and the
+    inverse y pass, and read by the inverse x transform (8 x 48 B per mode,
+    (lx/2+1) ly lz modes)."""
     cells = lx * ly * lz
     modes = (lx // 2 + 1) * ly * lz
-    return 36 * n_sites + 96 * cells + 576 * modes
+    return 36 * n_sites + 96 * cells + 384 * modes
 
 
 def cpu_baseline_demag(seconds_hint: float = 10.0) -> dict:
@@ -546,7 +548,7 @@ def bench_demag(args) -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": "profiles/demag_field_pt64x64.md (FETCH_SIZE x2, WRITE_SIZE x1)",
-                     "kernel": "field recomputation (k_demag_deposit, k_dft_lines_r x6, k_demag_project, "
+                     "kernel": "field recomputation (k_demag_deposit, k_dft_lines_r x4, k_demag_zproject, "
                                "k_demag_gather)", "ms_per_launch": ms_field,
                      "algorithmic_bytes_per_launch": bytes_field},
         "cpu_baseline": cpu,
