@@ -31,18 +31,38 @@ def autocorrelation(x: np.ndarray) -> np.ndarray:
     return acf / acf[0]
 
 
-def tau_int(x: np.ndarray, c: float = 5.0) -> float:
-    """Integrated autocorrelation time in units of the sampling interval."""
+# A window is trusted only when the series spans at least this many
+# autocorrelation times (Sokal's estimator is biased low on shorter ones).
+MIN_LENGTH_OVER_TAU = 50.0
+
+
+def tau_int_checked(x: np.ndarray, c: float = 5.0, min_ratio: float = MIN_LENGTH_OVER_TAU) -> tuple[float, bool]:
+    """(tau_int, converged): Sokal's windowed estimate in units of the
+    sampling interval, and whether it can be trusted.
+
+    On a mean-subtracted series the biased autocorrelations sum to exactly
+    -1/2 over all lags t >= 1, so the partial sums fall back to 0 at the tail
+    and some window W >= c tau(W) is always found, even when tau is far above
+    the series length. The estimate is marked converged only when the window
+    closes while the series is at least min_ratio times longer than the
+    estimate (n >= min_ratio tau); otherwise tau is a lower bound."""
     rho = autocorrelation(x)
     n = len(rho)
     if n < 2:
-        return 0.5
+        return 0.5, False
     taus = 0.5 + np.cumsum(rho[1:])
     w = np.arange(1, n)
     ok = w >= c * taus
     if not ok.any():
-        return float(max(taus[-1], 0.5))
-    return float(max(taus[np.argmax(ok)], 0.5))
+        return float(max(taus[-1], 0.5)), False
+    t = float(max(taus[np.argmax(ok)], 0.5))
+    return t, bool(n >= min_ratio * t)
+
+
+def tau_int(x: np.ndarray, c: float = 5.0) -> float:
+    """Integrated autocorrelation time in units of the sampling interval
+    (Sokal's window; see tau_int_checked for whether it converged)."""
+    return tau_int_checked(x, c)[0]
 
 
 def mean_error(x: np.ndarray, c: float = 5.0) -> tuple[float, float, float]:
@@ -75,13 +95,31 @@ def specific_heat_error(e: np.ndarray, beta: float, n_sites: int, n_blocks: int 
     return float(np.sqrt((nb - 1) / nb * np.sum((vals - vals.mean()) ** 2)))
 
 
-def independent_samples(series: np.ndarray, c: float = 5.0) -> tuple[float, np.ndarray]:
-    """Sum over columns (replicas) of n / (2 tau_int); returns (total, tau per column)."""
+def independent_samples(series: np.ndarray, c: float = 5.0) -> tuple[float, np.ndarray, np.ndarray]:
+    """Per column (replica) n / (2 tau_int); returns (sum over the columns
+    whose tau converged, tau per column, converged flag per column)."""
     s = np.asarray(series, dtype=np.float64)
     if s.ndim == 1:
         s = s[:, None]
-    taus = np.array([tau_int(s[:, k], c) for k in range(s.shape[1])])
-    return float(np.sum(s.shape[0] / (2.0 * taus))), taus
+    res = [tau_int_checked(s[:, k], c) for k in range(s.shape[1])]
+    taus = np.array([r[0] for r in res])
+    conv = np.array([r[1] for r in res], dtype=bool)
+    return float(np.sum(s.shape[0] / (2.0 * taus[conv]))), taus, conv
+
+
+def ratio_bootstrap(num: np.ndarray, den: np.ndarray, n_boot: int = 2000, seed: int = 0) -> tuple[float, float, float]:
+    """sum(num) / sum(den) over columns, with the 16th and 84th percentiles
+    of the same ratio over columns resampled with replacement (a 1-sigma
+    band over replicas)."""
+    num = np.asarray(num, dtype=np.float64)
+    den = np.asarray(den, dtype=np.float64)
+    r = float(num.sum() / den.sum())
+    if len(num) < 2:
+        return r, float("nan"), float("nan")
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, len(num), size=(n_boot, len(num)))
+    b = num[idx].sum(axis=1) / den[idx].sum(axis=1)
+    return r, float(np.percentile(b, 16)), float(np.percentile(b, 84))
 
 
 def consistent(a: float, ea: float, b: float, eb: float, nsigma: float = 3.0) -> bool:

@@ -1065,18 +1065,28 @@ __global__ __launch_bounds__(kBlock) void k_acc_fold(
 }
 
 // ---------------------------------------------------------------------------
-// One record of a measurement series, written by the kernel that finishes
-// the value: entry *ctr + local of an array of cap entries indexed by
-// temperature position. ctr advances once per period (k_advance), local
-// counts this series' records earlier in the period, so one captured period
-// replays correctly. a: energies [cap][R] or FP64 Q [cap][R][zmax];
-// b: magnetisation [cap][R][3]; qi: integer Q [cap][R][zmax].
-struct SeriesRec {
-    const uint32_t* ctr;
-    uint32_t local, cap;
-    double* a;
-    double* b;
+// Measurement series of one run: base pointers and capacities in a device
+// descriptor, written before the run by a one-thread kernel, so launches
+// captured in a graph stay valid when the series buffers are reallocated or
+// placed elsewhere. e: energies [cap_e][R]; m: magnetisation [cap_e][R][3];
+// qr / qi: FP64 and integer Q [cap_q][R][zmax], indexed by temperature
+// position.
+struct SeriesDesc {
+    double* e;
+    double* m;
+    double* qr;
     long long* qi;
+    uint32_t cap_e, cap_q;
+};
+
+// One record of a series, written by the kernel that finishes the value:
+// entry *ctr + local. ctr advances once per period (k_advance), local counts
+// this series' records earlier in the period, so one captured period replays
+// correctly. desc == nullptr: nothing is recorded.
+struct SeriesRec {
+    const SeriesDesc* desc;
+    const uint32_t* ctr;
+    uint32_t local;
 };
 
 // Temperature position of `slot` (the inverse of slot_of_temp), found by the
@@ -1253,13 +1263,14 @@ __device__ __forceinline__ void wave_finalize_slot(
         energy[slot] = v[0];
         mag[3 * slot] = v[1] * inv; mag[3 * slot + 1] = v[2] * inv; mag[3 * slot + 2] = v[3] * inv;
     }
-    if (rec.a) {       // uniform
+    if (rec.desc) {       // uniform
         const int t = temp_of_slot(slot_of_temp, nrep, slot);
         const uint32_t m = *rec.ctr + rec.local;
-        if (threadIdx.x == 0 && m < rec.cap && t >= 0) {
+        const SeriesDesc& D = *rec.desc;
+        if (threadIdx.x == 0 && m < D.cap_e && t >= 0) {
             const int64_t e = (int64_t)m * nrep + t;
-            rec.a[e] = v[0];
-            rec.b[3 * e] = v[1] * inv; rec.b[3 * e + 1] = v[2] * inv; rec.b[3 * e + 2] = v[3] * inv;
+            D.e[e] = v[0];
+            D.m[3 * e] = v[1] * inv; D.m[3 * e + 1] = v[2] * inv; D.m[3 * e + 2] = v[3] * inv;
         }
     }
 }
@@ -1295,13 +1306,14 @@ __global__ __launch_bounds__(kBlock) void k_measure_finalize(
         energy[slot] = v[0];
         mag[3 * slot] = v[1] * inv; mag[3 * slot + 1] = v[2] * inv; mag[3 * slot + 2] = v[3] * inv;
     }
-    if (rec.a) {       // uniform
+    if (rec.desc) {       // uniform
         const int t = temp_of_slot(slot_of_temp, nrep, slot);
         const uint32_t m = *rec.ctr + rec.local;
-        if (threadIdx.x == 0 && m < rec.cap && t >= 0) {
+        const SeriesDesc& D = *rec.desc;
+        if (threadIdx.x == 0 && m < D.cap_e && t >= 0) {
             const int64_t e = (int64_t)m * nrep + t;
-            rec.a[e] = v[0];
-            rec.b[3 * e] = v[1] * inv; rec.b[3 * e + 1] = v[2] * inv; rec.b[3 * e + 2] = v[3] * inv;
+            D.e[e] = v[0];
+            D.m[3 * e] = v[1] * inv; D.m[3 * e + 1] = v[2] * inv; D.m[3 * e + 2] = v[3] * inv;
         }
     }
 }
@@ -1442,13 +1454,14 @@ __device__ __forceinline__ void winding_finalize_block(
         qraw[b] = q;
         qint[b] = llrint(q);
     }
-    if (rec.a) {       // uniform
+    if (rec.desc) {       // uniform
         const int t = temp_of_slot(slot_of_temp, nrep, slot);
         const uint32_t m = *rec.ctr + rec.local;
-        if (threadIdx.x == 0 && m < rec.cap && t >= 0) {
+        const SeriesDesc& D = *rec.desc;
+        if (threadIdx.x == 0 && m < D.cap_q && t >= 0) {
             const int64_t e = ((int64_t)m * nrep + t) * zmax + z;
-            rec.a[e] = q;
-            rec.qi[e] = llrint(q);
+            D.qr[e] = q;
+            D.qi[e] = llrint(q);
         }
     }
 }
@@ -1514,13 +1527,14 @@ __global__ __launch_bounds__(kBlock) void k_swap(
     __syncthreads();
     if (threadIdx.x == 0) *round_ctr = round + 1u;
     // E and M after the round, at each temperature (configurations stay put)
-    const uint32_t m = rec.a ? *rec.ctr + rec.local : 0u;
-    if (rec.a && m < rec.cap) {
+    const uint32_t m = rec.desc ? *rec.ctr + rec.local : 0u;
+    if (rec.desc && m < rec.desc->cap_e) {
+        const SeriesDesc& D = *rec.desc;
         for (int t = threadIdx.x; t < R; t += blockDim.x) {
             const int s = slot_of_temp[t];
             const int64_t e = (int64_t)m * R + t;
-            rec.a[e] = energy[s];
-            rec.b[3 * e] = mag[3 * s]; rec.b[3 * e + 1] = mag[3 * s + 1]; rec.b[3 * e + 2] = mag[3 * s + 2];
+            D.e[e] = energy[s];
+            D.m[3 * e] = mag[3 * s]; D.m[3 * e + 1] = mag[3 * s + 1]; D.m[3 * e + 2] = mag[3 * s + 2];
         }
     }
 }
@@ -1541,6 +1555,12 @@ __global__ void k_advance(uint32_t* __restrict__ ctr, uint32_t sweeps, uint32_t 
 __global__ void k_set_u32(uint32_t* __restrict__ p, int n, uint32_t v)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+// Writes the series descriptor of the next run (a kernel rather than a copy:
+// it needs no host staging buffer and may be recorded in a graph).
+__global__ void k_set_series_desc(SeriesDesc* __restrict__ d, SeriesDesc v)
+{
+    if (threadIdx.x == 0) *d = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -2362,6 +2382,24 @@ struct LadHost {
     std::vector<double> betas;
 };
 
+// What a captured schedule period depends on besides the context's fixed
+// buffers: the schedule, the launch tuning (kernel shapes and the fused
+// layout), the kernel variants (learned, demagnetisation) and the field
+// buffer the half sweeps read.
+struct PtGraphKey {
+    int32_t swap_every = -1, measure_every = -1, winding_every = -1, winding_layer = -1;
+    int64_t period = 0;
+    uint32_t tune = 0;
+    int32_t learned = 0, demag = 0, records = 0;
+    const void* demag_buf = nullptr;
+    bool operator==(const PtGraphKey& o) const
+    {
+        return swap_every == o.swap_every && measure_every == o.measure_every && winding_every == o.winding_every &&
+               winding_layer == o.winding_layer && period == o.period && tune == o.tune && learned == o.learned &&
+               demag == o.demag && records == o.records && demag_buf == o.demag_buf;
+    }
+};
+
 struct ising_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -2447,6 +2485,15 @@ struct ising_ctx {
     unsigned long long* d_swap_att = nullptr;
     unsigned long long* d_swap_acc = nullptr;
     int32_t* d_sites = nullptr; float* d_props = nullptr; double* d_de = nullptr; int de_cap = 0;
+    // ising_pt_run: series buffers (grown, never shrunk), their descriptor,
+    // and the instantiated graph of the last schedule period (replaced when
+    // its key changes, released with the context)
+    double* ser_e = nullptr; double* ser_m = nullptr; double* ser_qr = nullptr; long long* ser_qi = nullptr;
+    size_t ser_cap_e = 0, ser_cap_q = 0;         // entries of ser_e and of ser_qr
+    SeriesDesc* d_series = nullptr;
+    hipGraph_t pt_graph = nullptr;
+    hipGraphExec_t pt_exec = nullptr;
+    PtGraphKey pt_key;
     // checkpoint (ising_save_state)
     float* ck_spins = nullptr; int32_t* ck_sot = nullptr; float* ck_bslot = nullptr; double* ck_btemp = nullptr;
     uint32_t* ck_ctr = nullptr; uint32_t ck_host_sweep = 0, ck_host_fused = 0; bool ck_valid = false;
@@ -2763,7 +2810,7 @@ void launch_fused_kernel(ising_ctx* c, uint32_t off, hipEvent_t t0 = nullptr, hi
 // colour 0 then colour 1 of every replica, from the buffer of the current
 // parity into the other one; with `measure`, E and M of the new state are
 // recorded by the same launches.
-constexpr SeriesRec kNoRecord{ nullptr, 0u, 0u, nullptr, nullptr, nullptr };
+constexpr SeriesRec kNoRecord{ nullptr, nullptr, 0u };
 
 // Start/stop events carried by the dispatch packets of the plain
 // (non-measuring) sweep launches of an eagerly enqueued schedule
@@ -3097,9 +3144,10 @@ void advance_host(ising_ctx* c, uint32_t n)
     if (fused_active(c)) c->host_fused += n;
 }
 
+// Where the records of a run go: the device descriptor (nullptr: records
+// are computed, not stored).
 struct SeriesDev {
-    double* e = nullptr; double* m = nullptr; double* qr = nullptr; long long* qi = nullptr;
-    uint32_t n_meas = 0, n_q = 0;
+    const SeriesDesc* desc = nullptr;
 };
 
 // Sweeps [0, count) of a schedule period, with swaps / records after every
@@ -3115,7 +3163,7 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
         const bool do_swap = s.swap_every > 0 && done % s.swap_every == 0;
         const bool do_meas = s.measure_every > 0 && done % s.measure_every == 0;
         const bool do_q = s.winding_every > 0 && done % s.winding_every == 0;
-        const SeriesRec rm = (do_meas && sd.e) ? SeriesRec{ c->d_ctr + 1, nm, sd.n_meas, sd.e, sd.m, nullptr } : kNoRecord;
+        const SeriesRec rm = (do_meas && sd.desc) ? SeriesRec{ sd.desc, c->d_ctr + 1, nm } : kNoRecord;
         // Measurements ride on the sweep launches. Before a swap, whose
         // exponent takes FP64 per-site energies, the half sweep's colour-1
         // launch computes them (the fused sweep and three-colour sweeps leave
@@ -3133,7 +3181,7 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
         if (do_swap)
             if ((rc = launch_swap(c, rm))) return rc;
         if (do_q) {
-            const SeriesRec rq = sd.qr ? SeriesRec{ c->d_ctr + 2, nq, sd.n_q, sd.qr, nullptr, sd.qi } : kNoRecord;
+            const SeriesRec rq = sd.desc ? SeriesRec{ sd.desc, c->d_ctr + 2, nq } : kNoRecord;
             if ((rc = launch_winding_all(c, s.winding_layer - 1, done, rq, merge, rm))) return rc;
         }
         nm += do_meas ? 1u : 0u;
@@ -3213,6 +3261,168 @@ int fused_layout(ising_ctx* c, int lc)
 
 int64_t gcd64(int64_t a, int64_t b) { while (b) { int64_t t = a % b; a = b; b = t; } return a; }
 
+int64_t schedule_period(const ising_schedule& s)
+{
+    int64_t period = 1;
+    for (int v : { s.swap_every, s.measure_every, s.winding_every })
+        if (v > 0) period = period / gcd64(period, v) * v;
+    return period;
+}
+
+// Byte layout of one run's series in a staging area: the descriptor, then
+// energies, magnetisations, FP64 and integer charges (each 256-byte aligned).
+struct SeriesLayout {
+    size_t e, m, qr, qi, total;
+};
+
+SeriesLayout series_layout(size_t n_meas, size_t n_q, size_t R, size_t Z)
+{
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    SeriesLayout L;
+    L.e = up(sizeof(SeriesDesc));
+    L.m = L.e + up(n_meas * R * sizeof(double));
+    L.qr = L.m + up(n_meas * R * 3 * sizeof(double));
+    L.qi = L.qr + up(n_q * R * Z * sizeof(double));
+    L.total = L.qi + up(n_q * R * Z * sizeof(long long));
+    return L;
+}
+
+// Series buffers of the context, grown to hold n_meas E/M and n_q charge
+// records (they are never shrunk; a reallocation needs no new graph, the
+// captured launches read the buffers through the descriptor).
+int series_reserve(ising_ctx* c, size_t n_meas, size_t n_q)
+{
+    const size_t R = (size_t)c->R, Z = (size_t)c->zmax;
+    int rc;
+    if (n_meas * R > c->ser_cap_e) {
+        if (c->ser_e) (void)hipFree(c->ser_e);
+        if (c->ser_m) (void)hipFree(c->ser_m);
+        c->ser_e = nullptr; c->ser_m = nullptr; c->ser_cap_e = 0;
+        if ((rc = dalloc(c, &c->ser_e, n_meas * R)) || (rc = dalloc(c, &c->ser_m, 3 * n_meas * R))) return rc;
+        c->ser_cap_e = n_meas * R;
+    }
+    if (n_q * R * Z > c->ser_cap_q) {
+        if (c->ser_qr) (void)hipFree(c->ser_qr);
+        if (c->ser_qi) (void)hipFree(c->ser_qi);
+        c->ser_qr = nullptr; c->ser_qi = nullptr; c->ser_cap_q = 0;
+        if ((rc = dalloc(c, &c->ser_qr, n_q * R * Z)) || (rc = dalloc(c, &c->ser_qi, n_q * R * Z))) return rc;
+        c->ser_cap_q = n_q * R * Z;
+    }
+    return ISING_OK;
+}
+
+void drop_pt_graph(ising_ctx* c)
+{
+    if (c->pt_exec) (void)hipGraphExecDestroy(c->pt_exec);
+    if (c->pt_graph) (void)hipGraphDestroy(c->pt_graph);
+    c->pt_exec = nullptr;
+    c->pt_graph = nullptr;
+    c->pt_key = PtGraphKey{};
+}
+
+// The body of the PT entry points. staging == nullptr: the series live in
+// the context's buffers and whole periods replay one cached graph;
+// otherwise the descriptor and the series sit in `staging` (device memory of
+// at least series_layout(...).total bytes) and every period is launched
+// eagerly, so nothing is allocated.
+int pt_run_impl(ising_ctx* c, const ising_schedule* sched, const ising_series* out, unsigned char* staging)
+{
+    if (!c || !sched) return ISING_ERR_INVALID_ARG;
+    if (c->any_demag)
+        return set_err(c, ISING_ERR_UNSUPPORTED, "parallel tempering with the demagnetisation term (the reference's "
+                       "PT runs without it)");
+    const ising_schedule s = *sched;
+    if (s.n_sweeps < 0 || s.swap_every < 0 || s.measure_every < 0 || s.winding_every < 0)
+        return set_err(c, ISING_ERR_INVALID_ARG, "negative schedule entry");
+    if (s.winding_layer < 0 || s.winding_layer > c->zmax)
+        return set_err(c, ISING_ERR_INVALID_ARG, "winding_layer %d outside 0..%d", s.winding_layer, c->zmax);
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc;
+    const size_t n_meas = s.measure_every ? (size_t)(s.n_sweeps / s.measure_every) : 0u;
+    const size_t n_q = s.winding_every ? (size_t)(s.n_sweeps / s.winding_every) : 0u;
+    const size_t R = (size_t)c->R, Z = (size_t)c->zmax;
+    SeriesDesc desc{};
+    SeriesDesc* d_desc = nullptr;
+    if (staging) {
+        const SeriesLayout lay = series_layout(n_meas, n_q, R, Z);
+        d_desc = (SeriesDesc*)staging;
+        desc.e = (double*)(staging + lay.e);
+        desc.m = (double*)(staging + lay.m);
+        desc.qr = (double*)(staging + lay.qr);
+        desc.qi = (long long*)(staging + lay.qi);
+    } else {
+        if ((rc = series_reserve(c, n_meas, n_q))) return rc;
+        d_desc = c->d_series;
+        desc.e = c->ser_e; desc.m = c->ser_m; desc.qr = c->ser_qr; desc.qi = c->ser_qi;
+    }
+    desc.cap_e = (uint32_t)n_meas;
+    desc.cap_q = (uint32_t)n_q;
+    SeriesDev sd;
+    sd.desc = d_desc;
+    // The descriptor and the record counters of this run (plain kernels: no
+    // staging copy, no memset).
+    hipLaunchKernelGGL(k_set_series_desc, dim3(1), dim3(64), 0, c->stream, d_desc, desc);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, c->d_ctr + 1, 2, 0u);
+    LAUNCH_CHECK(c);
+
+    const int64_t period = schedule_period(s);
+    const bool graph = !staging && s.use_graph && period <= 512 && s.n_sweeps >= 2 * period;
+    int done = 0;
+    if (graph) {
+        PtGraphKey key;
+        key.swap_every = s.swap_every; key.measure_every = s.measure_every; key.winding_every = s.winding_every;
+        key.winding_layer = s.winding_layer; key.period = period; key.tune = c->tune;
+        key.learned = c->any_learned ? 1 : 0; key.demag = c->any_demag ? 1 : 0; key.records = 1;
+        key.demag_buf = c->d_demag;
+        if (!c->pt_exec || !(key == c->pt_key)) {
+            drop_pt_graph(c);
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return set_err(c, ISING_ERR_HIP, "hipStreamBeginCapture");
+            rc = enqueue_period(c, s, (int)period, sd);
+            hipError_t ec = hipStreamEndCapture(c->stream, &g);
+            if (rc || ec != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : set_err(c, ISING_ERR_HIP, "capture");
+            }
+            hipGraphExec_t ge = nullptr;
+            if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return set_err(c, ISING_ERR_HIP, "hipGraphInstantiate");
+            }
+            c->pt_graph = g;
+            c->pt_exec = ge;
+            c->pt_key = key;
+        }
+        const int reps = s.n_sweeps / (int)period;
+        for (int k = 0; k < reps; ++k)
+            if (hipGraphLaunch(c->pt_exec, c->stream) != hipSuccess) return set_err(c, ISING_ERR_HIP, "hipGraphLaunch");
+        done = reps * (int)period;
+    }
+    // Remaining sweeps eagerly, period by period (records keep their order).
+    while (done < s.n_sweeps) {
+        const int cnt = (int)std::min<int64_t>(period, s.n_sweeps - done);
+        if ((rc = enqueue_period(c, s, cnt, sd))) return rc;
+        done += cnt;
+    }
+    hipError_t es = hipStreamSynchronize(c->stream);
+    if (es != hipSuccess) return set_err(c, ISING_ERR_HIP, "pt_run: %s", hipGetErrorString(es));
+    advance_host(c, (uint32_t)s.n_sweeps);
+    if (out) {
+        hipError_t e1 = hipSuccess;
+        if (out->energy && n_meas)
+            e1 = hipMemcpy(out->energy, desc.e, n_meas * R * sizeof(double), hipMemcpyDeviceToHost);
+        if (e1 == hipSuccess && out->magnetisation && n_meas)
+            e1 = hipMemcpy(out->magnetisation, desc.m, n_meas * R * 3 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e1 == hipSuccess && out->winding && n_q)
+            e1 = hipMemcpy(out->winding, desc.qi, n_q * R * Z * sizeof(int64_t), hipMemcpyDeviceToHost);
+        if (e1 == hipSuccess && out->winding_raw && n_q)
+            e1 = hipMemcpy(out->winding_raw, desc.qr, n_q * R * Z * sizeof(double), hipMemcpyDeviceToHost);
+        if (e1 != hipSuccess) return set_err(c, ISING_ERR_HIP, "series copy-out: %s", hipGetErrorString(e1));
+    }
+    return ISING_OK;
+}
+
 } // namespace
 
 // ===========================================================================
@@ -3257,9 +3467,12 @@ void ising_destroy(ising_ctx* c)
                      c->d_mlp_m2, c->d_mlp_grad, c->d_wave_grad, c->d_grad_part, c->d_recs, c->d_swap_att, c->d_swap_acc, c->d_sites, c->d_props, c->d_de,
                      c->ck_spins, c->ck_sot, c->ck_bslot, c->ck_btemp, c->ck_ctr,
                      c->d_fblock_rep, c->d_acc_wave_f, c->d_acc_base, c->d_wave_part_f,
-                     c->d_demag, c->d_dmesh, c->d_dk1, c->d_dk2, c->d_tw, c->d_dreps, c->d_cic };
+                     c->d_demag, c->d_dmesh, c->d_dk1, c->d_dk2, c->d_tw, c->d_dreps, c->d_cic,
+                     c->ser_e, c->ser_m, c->ser_qr, c->ser_qi, c->d_series };
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->pt_exec) (void)hipGraphExecDestroy(c->pt_exec);
+    if (c->pt_graph) (void)hipGraphDestroy(c->pt_graph);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -3393,6 +3606,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     c->learned.assign(n_ladders, 0);
     if ((rc = dalloc(c, &c->d_swap_att, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_swap_acc, R))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_series, 1))) return fail(rc);
     if ((rc = fused_setup(c))) return fail(rc);
 
     std::vector<float> beta_slot(R);
@@ -3720,92 +3934,7 @@ int ising_delta_energy(ising_ctx* c, int replica, const int32_t* sites, const do
 
 int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* out)
 {
-    if (!c || !sched) return ISING_ERR_INVALID_ARG;
-    if (c->any_demag)
-        return set_err(c, ISING_ERR_UNSUPPORTED, "parallel tempering with the demagnetisation term (the reference's "
-                       "PT runs without it)");
-    const ising_schedule s = *sched;
-    if (s.n_sweeps < 0 || s.swap_every < 0 || s.measure_every < 0 || s.winding_every < 0)
-        return set_err(c, ISING_ERR_INVALID_ARG, "negative schedule entry");
-    if (s.winding_layer < 0 || s.winding_layer > c->zmax)
-        return set_err(c, ISING_ERR_INVALID_ARG, "winding_layer %d outside 0..%d", s.winding_layer, c->zmax);
-    HIP_TRY(c, hipSetDevice(c->device));
-    int rc;
-    SeriesDev sd;
-    sd.n_meas = s.measure_every ? (uint32_t)(s.n_sweeps / s.measure_every) : 0u;
-    sd.n_q = s.winding_every ? (uint32_t)(s.n_sweeps / s.winding_every) : 0u;
-    const size_t R = (size_t)c->R, Z = (size_t)c->zmax;
-    auto cleanup = [&]() {
-        if (sd.e) (void)hipFree(sd.e);
-        if (sd.m) (void)hipFree(sd.m);
-        if (sd.qr) (void)hipFree(sd.qr);
-        if (sd.qi) (void)hipFree(sd.qi);
-    };
-    if (sd.n_meas) {
-        if ((rc = dalloc(c, &sd.e, sd.n_meas * R)) || (rc = dalloc(c, &sd.m, sd.n_meas * R * 3))) { cleanup(); return rc; }
-    }
-    if (sd.n_q) {
-        if ((rc = dalloc(c, &sd.qr, sd.n_q * R * Z)) || (rc = dalloc(c, &sd.qi, sd.n_q * R * Z))) { cleanup(); return rc; }
-    }
-    // Record counters restart at zero for this run (plain kernel, no memset).
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, c->d_ctr + 1, 2, 0u);
-    if (hipGetLastError() != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "counter reset"); }
-
-    int64_t period = 1;
-    for (int v : { s.swap_every, s.measure_every, s.winding_every })
-        if (v > 0) period = period / gcd64(period, v) * v;
-    const bool graph = s.use_graph && period <= 512 && s.n_sweeps >= 2 * period;
-    int done = 0;
-    if (graph) {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            cleanup();
-            return set_err(c, ISING_ERR_HIP, "hipStreamBeginCapture");
-        }
-        rc = enqueue_period(c, s, (int)period, sd);
-        hipError_t ec = hipStreamEndCapture(c->stream, &g);
-        if (rc || ec != hipSuccess) { if (g) (void)hipGraphDestroy(g); cleanup(); return rc ? rc : set_err(c, ISING_ERR_HIP, "capture"); }
-        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
-            (void)hipGraphDestroy(g); cleanup();
-            return set_err(c, ISING_ERR_HIP, "hipGraphInstantiate");
-        }
-        const int reps = s.n_sweeps / (int)period;
-        for (int k = 0; k < reps; ++k) {
-            if (hipGraphLaunch(ge, c->stream) != hipSuccess) {
-                (void)hipGraphExecDestroy(ge); (void)hipGraphDestroy(g); cleanup();
-                return set_err(c, ISING_ERR_HIP, "hipGraphLaunch");
-            }
-        }
-        done = reps * (int)period;
-        hipError_t es = hipStreamSynchronize(c->stream);
-        (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
-        if (es != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "graph replay: %s", hipGetErrorString(es)); }
-    }
-    // Remaining sweeps eagerly, period by period (records keep their order).
-    while (done < s.n_sweeps) {
-        const int cnt = (int)std::min<int64_t>(period, s.n_sweeps - done);
-        if ((rc = enqueue_period(c, s, cnt, sd))) { cleanup(); return rc; }
-        done += cnt;
-    }
-    hipError_t es = hipStreamSynchronize(c->stream);
-    if (es != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "pt_run: %s", hipGetErrorString(es)); }
-    advance_host(c, (uint32_t)s.n_sweeps);
-    if (out) {
-        hipError_t e1 = hipSuccess;
-        if (out->energy && sd.n_meas)
-            e1 = hipMemcpy(out->energy, sd.e, sd.n_meas * R * sizeof(double), hipMemcpyDeviceToHost);
-        if (e1 == hipSuccess && out->magnetisation && sd.n_meas)
-            e1 = hipMemcpy(out->magnetisation, sd.m, sd.n_meas * R * 3 * sizeof(double), hipMemcpyDeviceToHost);
-        if (e1 == hipSuccess && out->winding && sd.n_q)
-            e1 = hipMemcpy(out->winding, sd.qi, sd.n_q * R * Z * sizeof(int64_t), hipMemcpyDeviceToHost);
-        if (e1 == hipSuccess && out->winding_raw && sd.n_q)
-            e1 = hipMemcpy(out->winding_raw, sd.qr, sd.n_q * R * Z * sizeof(double), hipMemcpyDeviceToHost);
-        if (e1 != hipSuccess) { cleanup(); return set_err(c, ISING_ERR_HIP, "series copy-out"); }
-    }
-    cleanup();
-    return ISING_OK;
+    return pt_run_impl(c, sched, out, nullptr);
 }
 
 int ising_save_state(ising_ctx* c)

@@ -92,6 +92,9 @@ int64_t orc_half_sweep(const orc_lattice* L, const orc_ham* h, double* s, double
                        uint64_t seed, uint32_t slot, uint32_t sweep, int colour);
 int64_t orc_sweeps(const orc_lattice* L, const orc_ham* h, double* s, double beta, double radius,
                    uint64_t seed, uint32_t slot, uint32_t sweep0, int n_sweeps);
+int64_t orc_sweeps_learned(const orc_lattice* L, const orc_ham* h, double* s, double beta, const double* params,
+                           int quantize, double log_rmin, double log_rmax, uint64_t seed, uint32_t slot,
+                           uint32_t sweep0, int n_sweeps);       /* mlp_oracle.c */
 void    orc_init_random(const orc_lattice* L, double* s, uint64_t seed, uint32_t slot);
 void    orc_init_ferro(const orc_lattice* L, double* s, uint64_t seed, uint32_t slot, double frac);
 

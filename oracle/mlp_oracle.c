@@ -39,6 +39,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include "ising_oracle.h"
+
 #define NIN 21
 #define NH 32
 #define NPARAM (NH * NIN + NH + NH * NH + NH + NH + 1)
@@ -202,3 +204,59 @@ void orc_mlp_unit_ball(uint64_t seed, uint32_t rec, uint32_t step, double u[3])
 }
 
 int orc_mlp_nparam(void) { return NPARAM; }
+
+/* Sweeps with the learned radius (build form; the reference has none): each
+ * colour of the checkerboard in turn, every site of that colour one proposal
+ * with radius r = exp(clamp(f(x), log_rmin, log_rmax)), x = the six
+ * neighbour spins (+x, -x, +y, -y, +z, -z; zero across an open face) and
+ * beta*h (h the local field of orc_local_field: exchange, DMI and Zeeman,
+ * no anisotropy), then the Metropolis test of orc_sweeps with the same
+ * Philox counters. The radius never reads the site's own spin, so the
+ * proposal is symmetric. Simple-cubic lattices only (-1 otherwise).
+ *
+ * Pinned (tests/test_oracle_mlp.py) by: a network with zero weights and
+ * b3 = ln r repeats orc_sweeps with radius exp(b3) bit for bit; with an
+ * arbitrary network a paramagnet keeps the Langevin law <Sz> =
+ * coth(beta B) - 1/(beta B) (any symmetric proposal does; one that read the
+ * site's own spin would not). */
+int64_t orc_sweeps_learned(const orc_lattice* L, const orc_ham* h, double* s, double beta, const double* params,
+                           int quantize, double log_rmin, double log_rmax, uint64_t seed, uint32_t slot,
+                           uint32_t sweep0, int n_sweeps)
+{
+    if (L->cell != ORC_CELL_SC) return -1;
+    const int64_t N = orc_nsites(L);
+    net_t nt = unpack(params);
+    int64_t acc = 0;
+    for (int k = 0; k < n_sweeps; ++k) {
+        for (int c = 0; c < 2; ++c) {
+            #pragma omp parallel for reduction(+:acc) schedule(static)
+            for (int64_t i = 0; i < N; ++i) {
+                if (orc_site_colour(L, i) != c) continue;
+                double x[NIN];
+                int f = 0;
+                for (int axis = 0; axis < 3; ++axis)
+                    for (int d = 1; d >= -1; d -= 2) {
+                        const int64_t j = orc_neighbour(L, i, axis, d);
+                        for (int m = 0; m < 3; ++m) x[f + m] = j >= 0 ? s[m * N + j] : 0.0;
+                        f += 3;
+                    }
+                double hf[3];
+                orc_local_field(L, h, s, i, hf);
+                for (int m = 0; m < 3; ++m) x[18 + m] = beta * hf[m];
+                fwd_t o;
+                forward(&nt, x, quantize, &o);
+                const double lr = o.f < log_rmin ? log_rmin : (o.f > log_rmax ? log_rmax : o.f);
+                uint32_t rnd[4];
+                orc_metropolis_random(seed, slot, (uint32_t)i, sweep0 + (uint32_t)k, rnd);
+                double so[3] = { s[i], s[N + i], s[2 * N + i] }, sn[3], u;
+                if (!orc_propose(so, exp(lr), rnd, sn, &u)) continue;
+                const double de = orc_delta_energy(L, h, s, i, sn);
+                if (u < exp(-beta * de)) {
+                    s[i] = sn[0]; s[N + i] = sn[1]; s[2 * N + i] = sn[2];
+                    acc++;
+                }
+            }
+        }
+    }
+    return acc;
+}

@@ -96,6 +96,8 @@ def lib() -> C.CDLL:
             "orc_mlp_loss_grad": (C.c_double, [dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
             "orc_mlp_unit_ball": (None, [C.c_uint64, C.c_uint32, C.c_uint32, dp]),
             "orc_mlp_nparam": (C.c_int, []),
+            "orc_sweeps_learned": (C.c_int64, [lp, hp, dp, C.c_double, dp, C.c_int, C.c_double, C.c_double,
+                                               C.c_uint64, C.c_uint32, C.c_uint32, C.c_int]),
         }
 
         class ref_rng(C.Structure):
@@ -356,6 +358,19 @@ def mlp_forward(params, x, quantize=True) -> float:
     p = np.ascontiguousarray(params, dtype=np.float64)
     xx = np.ascontiguousarray(x, dtype=np.float64)
     return lib().orc_mlp_forward(_dp(p), _dp(xx), 1 if quantize else 0)
+
+
+def sweeps_learned(L, h, s, beta, params, seed, slot, sweep0, n, quantize=True, lo=None, hi=None) -> int:
+    """n sweeps with the network's radius at every site (orc_sweeps_learned)."""
+    assert s.flags.c_contiguous and s.dtype == np.float64
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    lo = MLP_LOG_RMIN if lo is None else lo
+    hi = MLP_LOG_RMAX if hi is None else hi
+    r = lib().orc_sweeps_learned(C.byref(L), C.byref(h), _dp(s), beta, _dp(p), 1 if quantize else 0, lo, hi,
+                                 seed, slot, sweep0, n)
+    if r < 0:
+        raise ValueError("orc_sweeps_learned: simple-cubic lattices only")
+    return r
 
 
 def mlp_unit_ball(seed, rec, step) -> np.ndarray:

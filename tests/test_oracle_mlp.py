@@ -118,3 +118,48 @@ def test_quantized_close_to_exact(orc):
     l1, g1 = orc.mlp_loss_grad(p, rec, u, quantize=True)
     assert abs(l0 - l1) < 0.02 * abs(l0)
     assert np.linalg.norm(g1 - g0) < 0.05 * np.linalg.norm(g0)
+
+
+def test_learned_sweep_with_constant_network_is_the_fixed_sweep(orc):
+    """Zero weights and b3 = ln r: every site gets radius exp(b3), so the
+    learned sweep repeats orc_sweeps with that radius bit for bit."""
+    lat = orc.lattice(6, 6, 4, (True, True, False))
+    h = orc.ham(1.0, 0.5, "bulk", 0.2, 0.05)
+    p = np.zeros(NP); p[SL["b3"]] = np.log(0.37)
+    a = orc.init_random(lat, 9, 2)
+    b = a.copy()
+    na = orc.sweeps_learned(lat, h, a, 1.3, p, 9, 2, 5, 3)
+    nb = orc.sweeps(lat, h, b, 1.3, float(np.exp(p[SL["b3"]][0])), 9, 2, 5, 3)
+    assert na == nb and np.array_equal(a, b)
+
+
+def test_learned_sweep_keeps_the_langevin_law(orc):
+    """A paramagnet in a field under an arbitrary network: every single-spin
+    stationary law is exact for a symmetric proposal, <Sz> = coth(bB) - 1/(bB).
+    The network's inputs are the neighbour spins, which keep changing, so the
+    radius varies from step to step; a radius that read the site's own spin
+    would make the proposal asymmetric and bias <Sz>."""
+    rng = np.random.default_rng(12)
+    p = random_params(rng, 0.8)
+    p[SL["w3"]] *= 3.0
+    lat = orc.lattice(8, 8, 8)
+    B, beta = 1.0, 1.5
+    h = orc.ham(0.0, 0.0, "none", B, 0.0)
+    s = orc.init_random(lat, 3, 0)
+    orc.sweeps_learned(lat, h, s, beta, p, 3, 0, 0, 50)
+    mz = []
+    for k in range(1500):
+        orc.sweeps_learned(lat, h, s, beta, p, 3, 0, 50 + k, 1)
+        mz.append(s[2].mean())
+    mz = np.array(mz)
+    exact = 1.0 / np.tanh(beta * B) - 1.0 / (beta * B)
+    from ising_amd.analysis import mean_error
+    m, err, _ = mean_error(mz)
+    assert abs(m - exact) < 4 * err + 1e-3, (m, exact, err)
+
+
+def test_learned_sweep_refuses_other_cells(orc):
+    lat = orc.lattice(4, 4, 4, cell="bcc")
+    s = orc.init_random(lat, 1, 0)
+    with pytest.raises(ValueError):
+        orc.sweeps_learned(lat, orc.ham(1.0), s, 1.0, np.zeros(NP), 1, 0, 0, 1)
