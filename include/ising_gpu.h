@@ -225,6 +225,20 @@ int  ising_set_demag_interval(ising_ctx* ctx, int sweeps);
  * with demag = 1 are refused (the reference's PT runs without the term,
  * src/PT.f90:229-230,251-254). */
 int  ising_pt_run(ising_ctx* ctx, const ising_schedule* sched, const ising_series* out);
+/* Allocation-free variant of ising_pt_run. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under graph
+ * capture, serving loops, harnesses that forbid hipMalloc). The same chains,
+ * records and results as ising_pt_run, but the series are staged in a
+ * statically allocated device array of ISING_STATIC_SCRATCH_BYTES per GPU
+ * (held until the process ends) and every period is launched eagerly, so the
+ * call makes no hipMalloc, hipFree or graph instantiation. The staging needs
+ * 256 + (n_meas*R*4 + n_q*R*Zmax*2)*8 bytes plus alignment (n_meas, n_q, R,
+ * Zmax as in ising_series); larger schedules return ISING_ERR_INVALID_ARG.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (several host threads, or several contexts on one device used at
+ * once) race. */
+#define ISING_STATIC_SCRATCH_BYTES (16u << 20)
+int ising_pt_run_static(ising_ctx* ctx, const ising_schedule* sched, const ising_series* out);
 /* slot_of_temp[R]: storage slot currently at temperature position t
  * (the reference's TemperatureMeshArray, src/PT.f90:58,201,264-268). */
 int  ising_temperature_map(ising_ctx* ctx, int32_t* slot_of_temp);

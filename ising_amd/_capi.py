@@ -32,6 +32,7 @@ TUNE_NT_OWN = 2
 TUNE_FUSED_SHIFT = 8
 TUNE_MAX_CANDIDATES = 16
 MU0_REF = 1.256637006008532e-06
+STATIC_SCRATCH_BYTES = 16 << 20
 
 
 def tune_fused(planes: int) -> int:
@@ -101,6 +102,7 @@ SIGNATURES = {
     "ising_set_demag_field": (C.c_int, [_P, C.c_int, _dp, C.c_int64]),
     "ising_set_demag_interval": (C.c_int, [_P, C.c_int]),
     "ising_pt_run": (C.c_int, [_P, C.POINTER(ising_schedule), C.POINTER(ising_series)]),
+    "ising_pt_run_static": (C.c_int, [_P, C.POINTER(ising_schedule), C.POINTER(ising_series)]),
     "ising_temperature_map": (C.c_int, [_P, _i32p]),
     "ising_save_state": (C.c_int, [_P]),
     "ising_restore_state": (C.c_int, [_P]),

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <string>
 #include <vector>
+#include <mutex>
 #include <algorithm>
 #include <numeric>
 #include <type_traits>
@@ -2499,6 +2500,9 @@ struct ising_ctx {
     uint32_t* ck_ctr = nullptr; uint32_t ck_host_sweep = 0, ck_host_fused = 0; bool ck_valid = false;
 };
 
+// Series staging of ising_pt_run_static (see static_scratch_setup below).
+static __device__ __align__(256) unsigned char scratch[ISING_STATIC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_create_err;
@@ -3269,6 +3273,30 @@ int64_t schedule_period(const ising_schedule& s)
     return period;
 }
 
+// ---- statically allocated series staging (ising_pt_run_static) ------------
+// For callers that cannot allocate device memory: a file-scope device array,
+// one copy per GPU, each at its own address. That address is taken once per
+// device, with the device current, when the first context on it is created,
+// and host code uses only that address (the array's name in host code is a
+// host-side stand-in, not a device pointer). Every call on a GPU shares its
+// copy, so calls that overlap on one GPU race.
+constexpr size_t kStaticScratchBytes = (size_t)ISING_STATIC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch_setup(ising_ctx* c)
+{
+    if (c->device >= kMaxDevices) return ISING_OK;      // ising_pt_run_static refuses such devices
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[c->device]) return ISING_OK;
+    void* p = nullptr;
+    HIP_TRY(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_err(c, ISING_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[c->device] = p;
+    return ISING_OK;
+}
+
 // Byte layout of one run's series in a staging area: the descriptor, then
 // energies, magnetisations, FP64 and integer charges (each 256-byte aligned).
 struct SeriesLayout {
@@ -3608,6 +3636,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
     if ((rc = dalloc(c, &c->d_swap_acc, R))) return fail(rc);
     if ((rc = dalloc(c, &c->d_series, 1))) return fail(rc);
     if ((rc = fused_setup(c))) return fail(rc);
+    if ((rc = static_scratch_setup(c))) return fail(rc);
 
     std::vector<float> beta_slot(R);
     std::vector<double> beta_temp(R);
@@ -3935,6 +3964,23 @@ int ising_delta_energy(ising_ctx* c, int replica, const int32_t* sites, const do
 int ising_pt_run(ising_ctx* c, const ising_schedule* sched, const ising_series* out)
 {
     return pt_run_impl(c, sched, out, nullptr);
+}
+
+int ising_pt_run_static(ising_ctx* c, const ising_schedule* sched, const ising_series* out)
+{
+    if (!c || !sched) return ISING_ERR_INVALID_ARG;
+    if (c->device >= kMaxDevices || !g_scratch_ptr[c->device])
+        return set_err(c, ISING_ERR_STATE, "no static scratch address for device %d", c->device);
+    const ising_schedule& s = *sched;
+    if (s.n_sweeps < 0 || s.measure_every < 0 || s.winding_every < 0)
+        return set_err(c, ISING_ERR_INVALID_ARG, "negative schedule entry");
+    const size_t n_meas = s.measure_every ? (size_t)(s.n_sweeps / s.measure_every) : 0u;
+    const size_t n_q = s.winding_every ? (size_t)(s.n_sweeps / s.winding_every) : 0u;
+    const size_t need = series_layout(n_meas, n_q, (size_t)c->R, (size_t)c->zmax).total;
+    if (need > kStaticScratchBytes)
+        return set_err(c, ISING_ERR_INVALID_ARG, "series of %zu bytes exceed the static scratch of %zu bytes "
+                       "(ISING_STATIC_SCRATCH_BYTES)", need, kStaticScratchBytes);
+    return pt_run_impl(c, sched, out, (unsigned char*)g_scratch_ptr[c->device]);
 }
 
 int ising_save_state(ising_ctx* c)

@@ -289,11 +289,12 @@ class Engine:
     # -- parallel tempering -------------------------------------------------
     def pt_run(self, n_sweeps: int, swap_every: int = 10, measure_every: int = 1, winding_every: int = 0,
                use_graph: bool = True, want: Sequence[str] = ("energy", "magnetisation", "winding"),
-               winding_layer: int | None = None) -> dict:
+               winding_layer: int | None = None, static: bool = False) -> dict:
         """Run the PT schedule on device; returns series indexed by temperature
         position: energy [n_meas, R], magnetisation [n_meas, R, 3],
         winding / winding_raw [n_q, R, Zmax] (with winding_layer = z only that
-        layer is computed; the other layers read 0)."""
+        layer is computed; the other layers read 0). static=True calls
+        ising_pt_run_static (no device allocation, eager periods)."""
         R, Z = self.n_replicas, self.zmax
         n_meas = n_sweeps // measure_every if measure_every else 0
         n_q = n_sweeps // winding_every if winding_every else 0
@@ -312,7 +313,8 @@ class Engine:
             ser.winding_raw = out["winding_raw"].ctypes.data_as(C.POINTER(C.c_double))
         sch = _capi.ising_schedule(n_sweeps, swap_every, measure_every, winding_every, 1 if use_graph else 0,
                                    0 if winding_layer is None else int(winding_layer) + 1)
-        self._check(self._lib.ising_pt_run(self._h, C.byref(sch), C.byref(ser)))
+        fn = self._lib.ising_pt_run_static if static else self._lib.ising_pt_run
+        self._check(fn(self._h, C.byref(sch), C.byref(ser)))
         return out
 
     def save_state(self) -> None:

@@ -231,6 +231,13 @@ module ising_gpu_c
                         type(ising_series), intent(in) :: series
                         integer(c_int) :: rc
                 end function
+                function ising_pt_run_static(ctx, sched, series) bind(C, name="ising_pt_run_static") result(rc)
+                        import :: c_ptr, c_int, ising_schedule, ising_series
+                        type(c_ptr), value :: ctx
+                        type(ising_schedule), intent(in) :: sched
+                        type(ising_series), intent(in) :: series
+                        integer(c_int) :: rc
+                end function
                 function ising_save_state(ctx) bind(C, name="ising_save_state") result(rc)
                         import :: c_ptr, c_int
                         type(c_ptr), value :: ctx
