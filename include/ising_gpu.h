@@ -186,6 +186,46 @@ int  ising_winding_all(ising_ctx* ctx, int64_t* q, double* q_raw);
  * plaquette with corner (i, j) -- the column-major order of
  * density_matrix(numCellsX, numCellsY); it sums to the layer's q_raw. */
 int  ising_winding_density(ising_ctx* ctx, int replica, int z, double* density);
+/* ---- skyrmion count -------------------------------------------------- */
+/* Replaces calculate_winding_density_interpolated(chainMesh, z, density)
+ * (src/reciprocalSpaceProcesses.f90:365-431): the charge density of layer z
+ * (0-based) on a grid scale_x times finer in x and scale_y in y (the
+ * reference's sclx, scly, 10 by default, src/chainMesh.f90:764-765):
+ * band-limited interpolation of the corner spins (each component; an even
+ * axis splits its Nyquist term between +-N/2), then per fine plaquette
+ * (sigma(s1,s2,s3) + sigma(s1,s3,s4)) / (4 pi), s2 = +x, s3 = +x+y, s4 = +y,
+ * periodic; density[J*NX + I], NX = scale_x lx. Needs periodic x and y
+ * (ISING_ERR_UNSUPPORTED) and at most 2^26 fine points. */
+int  ising_winding_density_interpolated(ising_ctx* ctx, int replica, int z, int scale_x, int scale_y,
+                                        double* density);
+
+typedef struct {
+    int32_t scale_x, scale_y;  /* fine grid per cell, 10 x 10 in the reference           */
+    int32_t n_particles;       /* N: counts of regions of charge 1..N                     */
+    int32_t num_thresholds;    /* mask thresholds, 60 in chainMesh_statistics (<= 254)    */
+    int32_t num_sigmas;        /* Gaussian widths, 20 in the reference                    */
+    int32_t reserved;
+    double  min_threshold;     /* 1e-5 in chainMesh_statistics                            */
+    double  max_threshold;     /* 0.95                                                    */
+} ising_skyrmion_params;
+
+/* Replaces compute_skyrmion_distribution(chainMesh, N, winding_array,
+ * min_threshold, max_threshold, num_thresholds, Z_index)
+ * (src/reciprocalSpaceProcesses.f90:930-983) and the calculate_skyrmion_number
+ * calls it makes (:627-812), with the intended semantics: for each
+ * threshold q_t = max - (t/T)(max - min) (t = 1..T) and width
+ * sigma_k = 1 - (k/S)(1 - 1e-5) (k = 1..S) the interpolated density is
+ * filtered by a centred 5x5 Gaussian, pixels with |f| > q_t max|f| are
+ * masked, every masked pixel joins the local maximum its steepest ascent over
+ * the 8 neighbours reaches, and each region whose summed density lies within
+ * 0.05 of a particle number j adds its sign once to count j. counts[N]
+ * receives the first (t, k) entry (t outer) whose sum_j j count_j lies within
+ * 0.1 of the layer's winding number, else the first whose nearest integer came
+ * closest; table (may be NULL) receives every entry, [T][S][N]. Deviations
+ * from the reference's arithmetic are listed with the kernels. */
+int  ising_skyrmion_distribution(ising_ctx* ctx, int replica, int z, const ising_skyrmion_params* params,
+                                 int32_t* counts, int32_t* table);
+
 /* Test hook standing in for calculateHeisenbergEnergy
  * (src/energyMin.f90:86-171): FP32 device dE = E(S') - E(S) of single-site
  * proposals props[n][3] at sites[n] (reference site order) of one replica. */

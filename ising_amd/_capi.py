@@ -66,6 +66,12 @@ class ising_series(C.Structure):
                 ("winding", C.POINTER(C.c_int64)), ("winding_raw", C.POINTER(C.c_double))]
 
 
+class ising_skyrmion_params(C.Structure):
+    _fields_ = [("scale_x", C.c_int32), ("scale_y", C.c_int32), ("n_particles", C.c_int32),
+                ("num_thresholds", C.c_int32), ("num_sigmas", C.c_int32), ("reserved", C.c_int32),
+                ("min_threshold", C.c_double), ("max_threshold", C.c_double)]
+
+
 _P = C.c_void_p
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
@@ -97,6 +103,8 @@ SIGNATURES = {
     "ising_winding_all": (C.c_int, [_P, _i64p, _dp]),
     "ising_winding_density": (C.c_int, [_P, C.c_int, C.c_int, _dp]),
     "ising_delta_energy": (C.c_int, [_P, C.c_int, _i32p, _dp, C.c_int, _dp]),
+    "ising_winding_density_interpolated": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    "ising_skyrmion_distribution": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(ising_skyrmion_params), _i32p, _i32p]),
     "ising_demag_field": (C.c_int, [_P, C.c_int, _dp, C.c_int64]),
     "ising_get_demag_field": (C.c_int, [_P, C.c_int, _dp, C.c_int64]),
     "ising_set_demag_field": (C.c_int, [_P, C.c_int, _dp, C.c_int64]),

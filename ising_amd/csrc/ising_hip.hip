@@ -1433,6 +1433,224 @@ __global__ __launch_bounds__(kBlock) void k_winding_density(
     out[pl] = d;
 }
 
+// ---------------------------------------------------------------------------
+// Skyrmion count of one layer (compute_skyrmion_distribution ->
+// calculate_skyrmion_number -> calculate_winding_density_interpolated,
+// src/reciprocalSpaceProcesses.f90:930-983, 627-812, 365-431), with the
+// intended semantics of each step:
+//   k_sk_coarse      corner spins of layer z (map_spins_to_std_grid), FP64;
+//   k_sk_interp_x/_y band-limited interpolation onto the (sx nx) x (sy ny)
+//                    grid, separable, kernel tables D_N(r/s + m) from the
+//                    host, sums in ascending m without contraction;
+//   k_sk_density     per fine plaquette (sigma(s1,s2,s3) + sigma(s1,s3,s4))
+//                    / (4 pi), periodic, renormalised spins;
+//   k_sk_filter      centred 5x5 Gaussian, periodic;
+//   k_sk_absmax      max |f| (order-free: the bit patterns of non-negative
+//                    doubles order like their values);
+//   k_sk_root        each pixel's steepest-ascent root over its 8
+//                    neighbours (|f|, ties to the earlier pixel of the
+//                    reference's x-major scan);
+//   k_sk_level       first threshold index that masks the pixel;
+//   k_sk_accum       region charges in fixed point, threshold by threshold
+//                    (64-bit integer atomics: exact and order-free);
+//   k_sk_count       regions whose charge is within 0.05 of a particle
+//                    number add its sign once.
+constexpr double kSkFixed = 1099511627776.0;    // 2^40
+constexpr double kSkFixedInv = 1.0 / 1099511627776.0;
+
+__global__ __launch_bounds__(kBlock) void k_sk_coarse(const float* __restrict__ spins, const Rep* __restrict__ reps,
+                                                      int slot, int z, double* __restrict__ S)
+{
+    const Rep R = reps[slot];
+    const uint32_t nx = (uint32_t)R.lx, ny = (uint32_t)R.ly;
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= nx * ny) return;
+    const uint32_t j = p / nx, i = p - j * nx;
+    const f3 v = spin_at(spins, R, i, j, (uint32_t)z);
+    S[p] = v.x; S[nx * ny + p] = v.y; S[2 * nx * ny + p] = v.z;
+}
+
+// T[c][j][I] = sum_m Kx[r][m] S[c][j][(n0 - m) mod nx], I = n0 sx + r.
+__global__ __launch_bounds__(kBlock) void k_sk_interp_x(const double* __restrict__ S, int nx, int ny, int sx,
+                                                        const double* __restrict__ Kx, double* __restrict__ T)
+{
+#pragma clang fp contract(off)
+    const int NX = nx * sx;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= (int64_t)3 * ny * NX) return;
+    const int I = (int)(e % NX);
+    const int64_t row = e / NX;                 // c*ny + j
+    const int n0 = I / sx, r = I - n0 * sx;
+    const double* k = Kx + (int64_t)r * nx;
+    const double* in = S + row * nx;
+    double acc = 0.0;
+    int src = n0;                               // (n0 - m) mod nx
+    for (int m = 0; m < nx; ++m) {
+        acc += k[m] * in[src];
+        src = src == 0 ? nx - 1 : src - 1;
+    }
+    T[e] = acc;
+}
+
+// F[c][J][I] = sum_m Ky[r][m] T[c][(m0 - m) mod ny][I], J = m0 sy + r.
+__global__ __launch_bounds__(kBlock) void k_sk_interp_y(const double* __restrict__ T, int ny, int NX, int sy,
+                                                        const double* __restrict__ Ky, double* __restrict__ F)
+{
+#pragma clang fp contract(off)
+    const int NY = ny * sy;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= (int64_t)3 * NY * NX) return;
+    const int I = (int)(e % NX);
+    const int64_t cj = e / NX;
+    const int J = (int)(cj % NY), c = (int)(cj / NY);
+    const int m0 = J / sy, r = J - m0 * sy;
+    const double* k = Ky + (int64_t)r * ny;
+    const double* in = T + (int64_t)c * ny * NX + I;
+    double acc = 0.0;
+    int src = m0;
+    for (int m = 0; m < ny; ++m) {
+        acc += k[m] * in[(int64_t)src * NX];
+        src = src == 0 ? ny - 1 : src - 1;
+    }
+    F[e] = acc;
+}
+
+__device__ __forceinline__ bool sk_unit(const double* __restrict__ F, int64_t plane, int64_t idx, double o[3])
+{
+#pragma clang fp contract(off)
+    const double x = F[idx], y = F[plane + idx], z = F[2 * plane + idx];
+    const double nn = sqrt(x * x + y * y + z * z);
+    if (!(nn > 0.0)) return false;
+    o[0] = x / nn; o[1] = y / nn; o[2] = z / nn;
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sk_density(const double* __restrict__ F, int NX, int NY,
+                                                       double* __restrict__ d)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t plane = (int64_t)NX * NY;
+    if (p >= plane) return;
+    const int J = (int)(p / NX), I = (int)(p - (int64_t)J * NX);
+    const int I2 = I + 1 == NX ? 0 : I + 1, J2 = J + 1 == NY ? 0 : J + 1;
+    double s1[3], s2[3], s3[3], s4[3];
+    double v = 0.0;
+    if (sk_unit(F, plane, p, s1) && sk_unit(F, plane, (int64_t)J * NX + I2, s2) &&
+        sk_unit(F, plane, (int64_t)J2 * NX + I2, s3) && sk_unit(F, plane, (int64_t)J2 * NX + I, s4))
+        v = (triangle_sigma(s1, s2, s3) + triangle_sigma(s1, s3, s4)) / (4.0 * M_PI);
+    d[p] = v;
+}
+
+// f(I, J) = sum_{a,b = -2..2} d(I - a, J - b) G[a+2][b+2], a outer.
+__global__ __launch_bounds__(kBlock) void k_sk_filter(const double* __restrict__ d, int NX, int NY,
+                                                      const double* __restrict__ G, double* __restrict__ f)
+{
+#pragma clang fp contract(off)
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= (int64_t)NX * NY) return;
+    const int J = (int)(p / NX), I = (int)(p - (int64_t)J * NX);
+    double acc = 0.0;
+    for (int a = -2; a <= 2; ++a) {
+        int x = I - a;
+        x = x < 0 ? x + NX : (x >= NX ? x - NX : x);
+        for (int b = -2; b <= 2; ++b) {
+            int y = J - b;
+            y = y < 0 ? y + NY : (y >= NY ? y - NY : y);
+            acc += d[(int64_t)y * NX + x] * G[(a + 2) * 5 + (b + 2)];
+        }
+    }
+    f[p] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sk_absmax(const double* __restrict__ f, int64_t n,
+                                                      unsigned long long* __restrict__ mx)
+{
+    unsigned long long v = 0ull;
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < n; p += (int64_t)gridDim.x * kBlock)
+        v = max(v, (unsigned long long)__double_as_longlong(fabs(f[p])));
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, (unsigned long long)__shfl_xor((long long)v, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicMax(mx, v);
+}
+
+// Pixel key (|f|, earlier scan rank wins ties); scan rank = I*NY + J.
+__device__ __forceinline__ bool sk_above(double v, int64_t r, double bv, int64_t br)
+{
+    return v > bv || (v == bv && r < br);
+}
+
+// Root of every pixel the lowest threshold masks (lvl <= T; the others get
+// -1): repeated steps to the highest of the 8 neighbours while one is above
+// the current pixel. The key order is strict, so every path ends, and it
+// never leaves the mask (|f| does not decrease along it).
+__global__ __launch_bounds__(kBlock) void k_sk_root(const double* __restrict__ f, int NX, int NY,
+                                                    const uint8_t* __restrict__ lvl, int T, int32_t* __restrict__ root)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= (int64_t)NX * NY) return;
+    if (lvl[p] > T) { root[p] = -1; return; }
+    int J = (int)(p / NX), I = (int)(p - (int64_t)J * NX);
+    double v = fabs(f[p]);
+    for (;;) {
+        double bv = v;
+        int64_t br = (int64_t)I * NY + J;
+        int bi = I, bj = J;
+        for (int a = -1; a <= 1; ++a) {
+            const int x = I + a < 0 ? NX - 1 : (I + a >= NX ? 0 : I + a);
+            for (int b = -1; b <= 1; ++b) {
+                if (a == 0 && b == 0) continue;
+                const int y = J + b < 0 ? NY - 1 : (J + b >= NY ? 0 : J + b);
+                const double w = fabs(f[(int64_t)y * NX + x]);
+                const int64_t r = (int64_t)x * NY + y;
+                if (sk_above(w, r, bv, br)) { bv = w; br = r; bi = x; bj = y; }
+            }
+        }
+        if (bi == I && bj == J) break;
+        I = bi; J = bj; v = bv;
+    }
+    root[p] = (int32_t)((int64_t)J * NX + I);
+}
+
+// First threshold index t (1-based) with |f| > q_t max|f|, q_t = max_t -
+// (t/T)(max_t - min_t); T + 1 when none masks the pixel.
+__global__ __launch_bounds__(kBlock) void k_sk_level(const double* __restrict__ f, int64_t n,
+                                                     const unsigned long long* __restrict__ mxbits, int T,
+                                                     double min_t, double max_t, uint8_t* __restrict__ lvl)
+{
+#pragma clang fp contract(off)
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const double mx = __longlong_as_double((long long)*mxbits);
+    const double v = fabs(f[p]);
+    int t = 1;
+    for (; t <= T; ++t) {
+        const double q = max_t - ((double)t / (double)T) * (max_t - min_t);
+        if (v > q * mx) break;
+    }
+    lvl[p] = (uint8_t)t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sk_accum(const double* __restrict__ f, int64_t n,
+                                                     const int32_t* __restrict__ root, const uint8_t* __restrict__ lvl,
+                                                     int t, unsigned long long* __restrict__ acc)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n || lvl[p] != t) return;
+    atomicAdd(acc + root[p], (unsigned long long)(long long)llrint(f[p] * kSkFixed));
+}
+
+__global__ __launch_bounds__(kBlock) void k_sk_count(int64_t n, const int32_t* __restrict__ root,
+                                                     const uint8_t* __restrict__ lvl,
+                                                     const unsigned long long* __restrict__ acc, int t, int n_part,
+                                                     int32_t* __restrict__ counts)
+{
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n || root[p] != (int32_t)p || lvl[p] > t) return;
+    const double ch = (double)(long long)acc[p] * kSkFixedInv;
+    for (int j = 1; j <= n_part; ++j)
+        if (fabs(fabs(ch) - (double)j) < 0.05) atomicAdd(counts + (j - 1), ch >= 0.0 ? 1 : -1);
+}
+
 // One workgroup per (slot, layer): fixed-order sum, Q = sum / (4 pi). In
 // one-layer mode (layer >= 0) the other layers read 0.
 __device__ __forceinline__ void winding_finalize_block(
@@ -3912,6 +4130,199 @@ int ising_winding_density(ising_ctx* c, int replica, int z, double* density)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return set_err(c, ISING_ERR_HIP, "winding density: %s", hipGetErrorString(e));
+    return ISING_OK;
+}
+
+// ---- skyrmion count ----------------------------------------------------------
+
+// D_N(u): the band-limited interpolation kernel of N periodic samples, the
+// Nyquist term of an even N split between +-N/2, in long double.
+static double sk_interp_kernel(int n, long double u)
+{
+    const long double pi = 3.14159265358979323846264338327950288L;
+    long double acc = 1.0L;
+    const int kmax = (n % 2 == 0) ? n / 2 - 1 : (n - 1) / 2;
+    for (int k = 1; k <= kmax; ++k) acc += 2.0L * cosl(2.0L * pi * (long double)k * u / (long double)n);
+    if (n % 2 == 0) acc += cosl(pi * u);
+    return (double)(acc / (long double)n);
+}
+
+// K[r][m] = D_N(r/s + m): the weight of the sample m cells before the
+// fine point r/s past a sample.
+static std::vector<double> sk_kernel_table(int n, int s)
+{
+    std::vector<double> K((size_t)s * n);
+    for (int r = 0; r < s; ++r)
+        for (int m = 0; m < n; ++m) K[(size_t)r * n + m] = sk_interp_kernel(n, (long double)r / (long double)s + m);
+    return K;
+}
+
+// Normalised 5x5 Gaussian weights G[a+2][b+2] (Gaussian_filter_2d,
+// src/reciprocalSpaceProcesses.f90:814-857, centred).
+static void sk_gauss_weights(double sigma, double (&G)[25])
+{
+    double acc = 0.0;
+    for (int a = -2; a <= 2; ++a)
+        for (int b = -2; b <= 2; ++b) {
+            const double w = std::exp(-(double)(a * a + b * b) / (2.0 * sigma * sigma));
+            G[(a + 2) * 5 + (b + 2)] = w;
+            acc += w;
+        }
+    for (double& g : G) g /= acc;
+}
+
+// Device buffers of one layer's statistic, released by the destructor.
+struct SkBufs {
+    double *S = nullptr, *T = nullptr, *F = nullptr, *d = nullptr, *f = nullptr, *Kx = nullptr, *Ky = nullptr,
+           *G = nullptr;
+    int32_t *root = nullptr, *counts = nullptr;
+    uint8_t* lvl = nullptr;
+    unsigned long long *acc = nullptr, *mx = nullptr;
+    ~SkBufs()
+    {
+        void* ptrs[] = { S, T, F, d, f, Kx, Ky, G, root, counts, lvl, acc, mx };
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+static dim3 grid_of(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+// The fine density of layer z of `replica` into b.d (NX*NY doubles).
+static int sk_fine_density(ising_ctx* c, int replica, int z, int sx, int sy, SkBufs& b)
+{
+    const Rep& r = c->reps[replica];
+    const int nx = r.lx, ny = r.ly, NX = nx * sx, NY = ny * sy;
+    const int64_t n = (int64_t)NX * NY;
+    int rc;
+    if ((rc = dalloc(c, &b.S, 3 * (size_t)nx * ny)) || (rc = dalloc(c, &b.T, 3 * (size_t)ny * NX)) ||
+        (rc = dalloc(c, &b.F, 3 * (size_t)n)) || (rc = dalloc(c, &b.d, (size_t)n)) ||
+        (rc = dalloc(c, &b.Kx, (size_t)sx * nx)) || (rc = dalloc(c, &b.Ky, (size_t)sy * ny)))
+        return rc;
+    const std::vector<double> kx = sk_kernel_table(nx, sx), ky = sk_kernel_table(ny, sy);
+    HIP_TRY(c, hipMemcpyAsync(b.Kx, kx.data(), kx.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(b.Ky, ky.data(), ky.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_sk_coarse, grid_of((int64_t)nx * ny), dim3(kBlock), 0, c->stream, (const float*)cur_spins(c),
+                       c->d_reps, replica, z, b.S);
+    hipLaunchKernelGGL(k_sk_interp_x, grid_of((int64_t)3 * ny * NX), dim3(kBlock), 0, c->stream, (const double*)b.S,
+                       nx, ny, sx, (const double*)b.Kx, b.T);
+    hipLaunchKernelGGL(k_sk_interp_y, grid_of(3 * n), dim3(kBlock), 0, c->stream, (const double*)b.T, ny, NX, sy,
+                       (const double*)b.Ky, b.F);
+    hipLaunchKernelGGL(k_sk_density, grid_of(n), dim3(kBlock), 0, c->stream, (const double*)b.F, NX, NY, b.d);
+    LAUNCH_CHECK(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));     // the host tables may go now
+    return ISING_OK;
+}
+
+static int sk_check(ising_ctx* c, int replica, int z, int sx, int sy)
+{
+    int rc = check_replica(c, replica);
+    if (rc) return rc;
+    const Rep& r = c->reps[replica];
+    if (z < 0 || z >= r.lz) return set_err(c, ISING_ERR_INVALID_ARG, "layer %d out of range", z);
+    if ((r.per & 3) != 3)
+        return set_err(c, ISING_ERR_UNSUPPORTED, "the interpolated charge density needs periodic x and y");
+    if (sx < 1 || sy < 1 || (int64_t)r.lx * sx * r.ly * sy > ((int64_t)1 << 26))
+        return set_err(c, ISING_ERR_INVALID_ARG, "interpolation scales %d x %d (fine grid up to 2^26 points)", sx, sy);
+    return ISING_OK;
+}
+
+int ising_winding_density_interpolated(ising_ctx* c, int replica, int z, int scale_x, int scale_y, double* density)
+{
+    int rc = sk_check(c, replica, z, scale_x, scale_y);
+    if (rc) return rc;
+    if (!density) return set_err(c, ISING_ERR_INVALID_ARG, "null density");
+    HIP_TRY(c, hipSetDevice(c->device));
+    SkBufs b;
+    if ((rc = sk_fine_density(c, replica, z, scale_x, scale_y, b))) return rc;
+    const Rep& r = c->reps[replica];
+    const size_t n = (size_t)r.lx * scale_x * r.ly * scale_y;
+    HIP_TRY(c, hipMemcpyAsync(density, b.d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ISING_OK;
+}
+
+int ising_skyrmion_distribution(ising_ctx* c, int replica, int z, const ising_skyrmion_params* prm, int32_t* counts,
+                                int32_t* table)
+{
+    if (!c || !prm || !counts) return ISING_ERR_INVALID_ARG;
+    const ising_skyrmion_params P = *prm;
+    int rc = sk_check(c, replica, z, P.scale_x, P.scale_y);
+    if (rc) return rc;
+    if (P.n_particles < 1 || P.num_thresholds < 1 || P.num_thresholds > 254 || P.num_sigmas < 1 ||
+        !(P.min_threshold >= 0.0) || !(P.max_threshold >= P.min_threshold) || !(P.max_threshold <= 1.0))
+        return set_err(c, ISING_ERR_INVALID_ARG, "skyrmion parameters (particles %d, thresholds %d, widths %d, "
+                       "threshold range %g..%g)", P.n_particles, P.num_thresholds, P.num_sigmas, P.min_threshold,
+                       P.max_threshold);
+    HIP_TRY(c, hipSetDevice(c->device));
+    SkBufs b;
+    if ((rc = sk_fine_density(c, replica, z, P.scale_x, P.scale_y, b))) return rc;
+    const Rep& r = c->reps[replica];
+    const int NX = r.lx * P.scale_x, NY = r.ly * P.scale_y;
+    const int64_t n = (int64_t)NX * NY;
+    const int T = P.num_thresholds, S = P.num_sigmas, NP = P.n_particles;
+    const size_t ntab = (size_t)T * S * NP;
+    if ((rc = dalloc(c, &b.f, (size_t)n)) || (rc = dalloc(c, &b.root, (size_t)n)) || (rc = dalloc(c, &b.lvl, (size_t)n)) ||
+        (rc = dalloc(c, &b.acc, (size_t)n)) || (rc = dalloc(c, &b.mx, 1)) || (rc = dalloc(c, &b.G, 25)) ||
+        (rc = dalloc(c, &b.counts, ntab)))
+        return rc;
+    HIP_TRY(c, hipMemsetAsync(b.counts, 0, ntab * sizeof(int32_t), c->stream));
+    std::vector<double> Gs((size_t)S * 25);
+    for (int k = 1; k <= S; ++k) {
+        const double sigma = 1.0 - ((double)k / (double)S) * (1.0 - 0.00001);
+        double G[25];
+        sk_gauss_weights(sigma, G);
+        std::copy(G, G + 25, Gs.begin() + (size_t)(k - 1) * 25);
+    }
+    for (int k = 0; k < S; ++k) {
+        HIP_TRY(c, hipMemcpyAsync(b.G, Gs.data() + (size_t)k * 25, 25 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_sk_filter, grid_of(n), dim3(kBlock), 0, c->stream, (const double*)b.d, NX, NY,
+                           (const double*)b.G, b.f);
+        HIP_TRY(c, hipMemsetAsync(b.mx, 0, sizeof(unsigned long long), c->stream));
+        hipLaunchKernelGGL(k_sk_absmax, dim3(1024), dim3(kBlock), 0, c->stream, (const double*)b.f, n, b.mx);
+        hipLaunchKernelGGL(k_sk_level, grid_of(n), dim3(kBlock), 0, c->stream, (const double*)b.f, n,
+                           (const unsigned long long*)b.mx, T, P.min_threshold, P.max_threshold, b.lvl);
+        hipLaunchKernelGGL(k_sk_root, grid_of(n), dim3(kBlock), 0, c->stream, (const double*)b.f, NX, NY,
+                           (const uint8_t*)b.lvl, T, b.root);
+        HIP_TRY(c, hipMemsetAsync(b.acc, 0, (size_t)n * sizeof(unsigned long long), c->stream));
+        for (int t = 1; t <= T; ++t) {
+            hipLaunchKernelGGL(k_sk_accum, grid_of(n), dim3(kBlock), 0, c->stream, (const double*)b.f, n,
+                               (const int32_t*)b.root, (const uint8_t*)b.lvl, t, b.acc);
+            hipLaunchKernelGGL(k_sk_count, grid_of(n), dim3(kBlock), 0, c->stream, n, (const int32_t*)b.root,
+                               (const uint8_t*)b.lvl, (const unsigned long long*)b.acc, t, NP,
+                               b.counts + ((size_t)(t - 1) * S + k) * NP);
+        }
+        LAUNCH_CHECK(c);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));   // b.G is refilled for the next width
+    }
+    std::vector<int32_t> tab(ntab);
+    HIP_TRY(c, hipMemcpyAsync(tab.data(), b.counts, ntab * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (table) std::copy(tab.begin(), tab.end(), table);
+    // The layer's winding number and the search of compute_skyrmion_distribution
+    // (src/reciprocalSpaceProcesses.f90:955-981): thresholds outer, widths inner;
+    // the first entry within 0.1 of Q, else the first whose nint came closest.
+    int64_t qi = 0;
+    double q = 0.0;
+    if ((rc = ising_winding(c, replica, z, &qi, &q))) return rc;
+    const long long nq = llround(q);
+    std::vector<int32_t> closest(NP, 0);
+    for (int t = 0; t < T; ++t)
+        for (int k = 0; k < S; ++k) {
+            const int32_t* arr = tab.data() + ((size_t)t * S + k) * NP;
+            double winding = 0.0;
+            long long current = 0;
+            for (int j = 1; j <= NP; ++j) {
+                winding += (double)j * arr[j - 1];
+                current += (long long)j * closest[j - 1];
+            }
+            if (std::llabs(llround(winding) - nq) < std::llabs(current - nq)) std::copy(arr, arr + NP, closest.begin());
+            if (std::fabs(winding - q) < 0.1) {
+                std::copy(arr, arr + NP, counts);
+                return ISING_OK;
+            }
+        }
+    std::copy(closest.begin(), closest.end(), counts);
     return ISING_OK;
 }
 

@@ -249,6 +249,29 @@ class Engine:
         self._check(self._lib.ising_winding_density(self._h, replica, z, d.ctypes.data_as(C.POINTER(C.c_double))))
         return d.reshape(lat.ly, lat.lx)
 
+    def winding_density_interpolated(self, replica: int, z: int, scale_x: int = 10, scale_y: int = 10) -> np.ndarray:
+        """Charge density of layer z on the interpolated grid, [sy*ly, sx*lx]."""
+        lat = self.replica_lattice[replica]
+        d = np.empty(lat.lx * scale_x * lat.ly * scale_y)
+        self._check(self._lib.ising_winding_density_interpolated(self._h, replica, z, scale_x, scale_y,
+                                                                 d.ctypes.data_as(C.POINTER(C.c_double))))
+        return d.reshape(lat.ly * scale_y, lat.lx * scale_x)
+
+    def skyrmion_distribution(self, replica: int, z: int, n_particles: int = 1, scale: tuple[int, int] = (10, 10),
+                              num_thresholds: int = 60, num_sigmas: int = 20, min_threshold: float = 1e-5,
+                              max_threshold: float = 0.95) -> tuple[np.ndarray, np.ndarray]:
+        """(counts[n_particles], table[num_thresholds, num_sigmas, n_particles]):
+        the skyrmion count of layer z (compute_skyrmion_distribution) and every
+        (threshold, width) entry it searched."""
+        prm = _capi.ising_skyrmion_params(scale[0], scale[1], n_particles, num_thresholds, num_sigmas, 0,
+                                          min_threshold, max_threshold)
+        counts = np.zeros(n_particles, dtype=np.int32)
+        table = np.zeros((num_thresholds, num_sigmas, n_particles), dtype=np.int32)
+        self._check(self._lib.ising_skyrmion_distribution(self._h, replica, z, C.byref(prm),
+                                                          counts.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                          table.ctypes.data_as(C.POINTER(C.c_int32))))
+        return counts, table
+
     def delta_energy(self, replica: int, sites: np.ndarray, props: np.ndarray) -> np.ndarray:
         s = np.ascontiguousarray(sites, dtype=np.int32)
         p = np.ascontiguousarray(props, dtype=np.float64).reshape(-1, 3)

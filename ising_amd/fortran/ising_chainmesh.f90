@@ -21,6 +21,8 @@ module ChainMesh
                 integer :: cell = 0           ! ISING_CELL_SC, ISING_CELL_BCC or ISING_CELL_HYPERKAGOME
                 integer :: atomsPerCell = 1
                 real(kind=8) :: a = 1.0_8     ! lattice constant (positions in output files)
+                ! upsampling of the interpolated charge density (src/chainMesh.f90:44,764-773)
+                integer :: sclx = 10, scly = 10
                 real(kind=8), allocatable :: atomSpins(:,:)
                 ! the last demagnetising field computed (src/chainMesh.f90:38),
                 ! filled by the demag = .true. paths of EnergyMin
@@ -47,11 +49,12 @@ contains
         ! are provided. `a` is the cell edge used for positions in output files;
         ! by default the edge its unit-cell routine sets (bcc 2, hyper-kagome 6,
         ! src/unit_cells.f90:21,47), 1 for simple cubic.
-        function makeChainMesh(numCellsX, numCellsY, numCellsZ, periodic, cell, a) result(mesh)
+        function makeChainMesh(numCellsX, numCellsY, numCellsZ, periodic, cell, a, sclx, scly) result(mesh)
                 integer, intent(in) :: numCellsX, numCellsY, numCellsZ
                 logical, intent(in), optional :: periodic(3)
                 integer, intent(in), optional :: cell
                 real(kind=8), intent(in), optional :: a
+                integer, intent(in), optional :: sclx, scly
                 type(ChainMesh_t) :: mesh
                 integer :: stat
                 mesh%numCellsX = numCellsX
@@ -70,6 +73,10 @@ contains
                         mesh%a = 1.0_8
                 end select
                 if (present(a)) mesh%a = a
+                if (present(sclx)) mesh%sclx = sclx
+                if (present(scly)) mesh%scly = scly
+                if (mesh%sclx < 2) error stop "x interpolation scale must be greater than or equal to 2"
+                if (mesh%scly < 2) error stop "y interpolation scale must be greater than or equal to 2"
                 mesh%numAtoms = numCellsX * numCellsY * numCellsZ * mesh%atomsPerCell
                 if (present(periodic)) mesh%periodic = periodic
                 allocate(mesh%atomSpins(mesh%numAtoms, 3), stat=stat)

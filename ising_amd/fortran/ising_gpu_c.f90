@@ -69,6 +69,16 @@ module ising_gpu_c
                 type(c_ptr) :: winding_raw = c_null_ptr
         end type ising_series
 
+        type, bind(C) :: ising_skyrmion_params
+                integer(c_int32_t) :: scale_x = 10, scale_y = 10
+                integer(c_int32_t) :: n_particles = 1
+                integer(c_int32_t) :: num_thresholds = 60
+                integer(c_int32_t) :: num_sigmas = 20
+                integer(c_int32_t) :: reserved = 0
+                real(c_double) :: min_threshold = 1.0e-5_c_double
+                real(c_double) :: max_threshold = 0.95_c_double
+        end type ising_skyrmion_params
+
         interface
                 function ising_version() bind(C, name="ising_version") result(p)
                         import :: c_ptr
@@ -205,6 +215,24 @@ module ising_gpu_c
                         type(c_ptr), value :: ctx
                         integer(c_int), value :: replica, z
                         real(c_double), intent(out) :: density(*)
+                        integer(c_int) :: rc
+                end function
+                function ising_winding_density_interpolated(ctx, replica, z, scale_x, scale_y, density) &
+                                bind(C, name="ising_winding_density_interpolated") result(rc)
+                        import :: c_ptr, c_int, c_double
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: replica, z, scale_x, scale_y
+                        real(c_double), intent(out) :: density(*)
+                        integer(c_int) :: rc
+                end function
+                function ising_skyrmion_distribution(ctx, replica, z, params, counts, table) &
+                                bind(C, name="ising_skyrmion_distribution") result(rc)
+                        import :: c_ptr, c_int, c_int32_t, ising_skyrmion_params
+                        type(c_ptr), value :: ctx
+                        integer(c_int), value :: replica, z
+                        type(ising_skyrmion_params), intent(in) :: params
+                        integer(c_int32_t), intent(out) :: counts(*)
+                        type(c_ptr), value :: table
                         integer(c_int) :: rc
                 end function
                 function ising_winding_all(ctx, q, q_raw) bind(C, name="ising_winding_all") result(rc)

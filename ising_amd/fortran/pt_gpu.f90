@@ -38,7 +38,7 @@ program pt_gpu
         integer :: numIterations, numMCSSweepsPerSwap, numRelax
         real(dp) :: JMin, JMax, DMin, DMax, BMin, BMax
         integer :: NJ, ND, NB, NumParams, i, j, k, JIndex, DIndex, BIndex, slot, mid
-        character(len=400) :: outputPath, outputPathSpins, outputPathDensity
+        character(len=400) :: outputPath, outputPathSpins, outputPathDensity, outputPathDensityInterpolated
         character(len=500) :: string_buff
         character(len=:), allocatable :: output_string, filename_string
         real(dp), allocatable :: ParamArray(:,:)
@@ -49,7 +49,10 @@ program pt_gpu
         integer(c_int) :: rc
         integer(c_int32_t), allocatable :: slot_of_temp(:)
         integer(c_int64_t), allocatable :: q(:)
-        real(c_double), allocatable :: qr(:), mag(:), spins(:), density(:)
+        real(c_double), allocatable :: qr(:), mag(:), spins(:), density(:), density_fine(:)
+        type(ising_skyrmion_params) :: sk_params
+        integer(c_int32_t) :: sk_counts(1)
+        integer :: scl, jx, ky
         type(ising_schedule) :: sched
         type(ising_series) :: series
         integer :: fileHandle, nsites, ix, iy, iz, site, atom
@@ -68,6 +71,9 @@ program pt_gpu
         numIterations = env_int("ISING_PT_ITERATIONS", 40)
         numMCSSweepsPerSwap = env_int("ISING_PT_SWEEPS_PER_SWAP", 100)
         numRelax = env_int("ISING_PT_RELAX", 1000)
+        ! upsampling of the interpolated density (makeChainMesh's sclx = scly = 10,
+        ! src/chainMesh.f90:764-765)
+        scl = env_int("ISING_PT_INTERP_SCALE", 10)
 
         NumParams = NJ * ND * NB
         if (NumParams < 1) error stop "ERROR: Number of Parameter Slots must be at least one"
@@ -126,10 +132,21 @@ program pt_gpu
         call ising_check(rc, ctx, "pt_gpu: magnetisation")
         outputPathSpins = trim(adjustl(outputPath)) // "_spins"
         outputPathDensity = trim(adjustl(outputPath)) // "_density"
+        outputPathDensityInterpolated = trim(adjustl(outputPath)) // "_density_interpolated"
         call execute_command_line("mkdir -p " // trim(adjustl(outputPath)))
         call execute_command_line("mkdir -p " // trim(adjustl(outputPathSpins)))
         call execute_command_line("mkdir -p " // trim(adjustl(outputPathDensity)))
-        allocate(density(numCellsX * numCellsY))
+        call execute_command_line("mkdir -p " // trim(adjustl(outputPathDensityInterpolated)))
+        allocate(density(numCellsX * numCellsY), density_fine(scl * numCellsX * scl * numCellsY))
+        ! skyrmion count of the middle layer as chainMesh_statistics takes it
+        ! (src/PT-Utils.f90:54-56): N = 1, thresholds 1e-5..0.95 in 60 steps
+        sk_params%scale_x = scl
+        sk_params%scale_y = scl
+        sk_params%n_particles = 1
+        sk_params%num_thresholds = 60
+        sk_params%num_sigmas = 20
+        sk_params%min_threshold = 1e-5_dp
+        sk_params%max_threshold = 0.95_dp
         mid = max(1, numCellsZ / 2)
         do i = 1, NumParams
                 do j = 1, numTemps
@@ -137,6 +154,9 @@ program pt_gpu
                         k = slot * numCellsZ
                         winding_middle = qr(k + mid)
                         winding_spread = abs(maxval(qr(k + 1:k + numCellsZ)) - minval(qr(k + 1:k + numCellsZ)))
+                        rc = ising_skyrmion_distribution(ctx, int(slot, c_int), int(mid - 1, c_int), sk_params, &
+                                                         sk_counts, c_null_ptr)
+                        call ising_check(rc, ctx, "pt_gpu: skyrmion count")
                         output_string = "J,D,B,T,winding_number_middle,&
                                 &skyrmion_number_middle,winding_number_spread,sx,&
                                 &sy,sz" // new_line('a')
@@ -146,7 +166,7 @@ program pt_gpu
                         output_string = output_string // trim(adjustl(string_buff))
                         string_buff = " "
                         write(string_buff, '((F0.4,",",I0.4,",",F0.2,",",F0.2,",",F0.2,",",F0.2))') winding_middle, &
-                                int(abs(q(k + mid))), winding_spread, mag(3 * slot + 1), mag(3 * slot + 2), mag(3 * slot + 3)
+                                int(sk_counts(1)), winding_spread, mag(3 * slot + 1), mag(3 * slot + 2), mag(3 * slot + 3)
                         output_string = output_string // string_buff // new_line('a')
                         string_buff = " "
                         write(string_buff, '((F0.4,"_",F0.4,"_",F0.4,"_",e0.4))') ParamArray(i, 1), ParamArray(i, 2), &
@@ -190,6 +210,24 @@ program pt_gpu
                                         do iy = 1, numCellsY
                                                 call write_density_line(fileHandle, a_bravais * ix, a_bravais * iy, &
                                                         a_bravais * iz, density(ix + numCellsX * (iy - 1)))
+                                        end do
+                                end do
+                        end do
+                        close(fileHandle)
+                        ! Interpolated charge density, every layer (write_winding_number_density_interpolated,
+                        ! src/reciprocalSpaceProcesses.f90:502-540; fine indices times the cell vectors).
+                        filename_string = trim(adjustl(outputPathDensityInterpolated)) // "/density_interp_" // &
+                                trim(adjustl(string_buff)) // ".csv"
+                        open(newunit=fileHandle, status="replace", action="write", file=filename_string)
+                        write(fileHandle, '(A)') "x,y,z,Winding_Density"
+                        do iz = 1, numCellsZ
+                                rc = ising_winding_density_interpolated(ctx, int(slot, c_int), int(iz - 1, c_int), &
+                                                                        int(scl, c_int), int(scl, c_int), density_fine)
+                                call ising_check(rc, ctx, "pt_gpu: interpolated density")
+                                do jx = 1, scl * numCellsX
+                                        do ky = 1, scl * numCellsY
+                                                call write_density_line(fileHandle, a_bravais * jx, a_bravais * ky, &
+                                                        a_bravais * iz, density_fine(jx + scl * numCellsX * (ky - 1)))
                                         end do
                                 end do
                         end do

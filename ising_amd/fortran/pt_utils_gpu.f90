@@ -62,6 +62,88 @@ contains
                 close(unit)
         end subroutine write_winding_number_density
 
+        ! Replaces calculate_winding_density_interpolated (src/reciprocalSpaceProcesses.f90:365-431):
+        ! the charge density of layer z_index (1-based) on the grid sclx times finer
+        ! in x and scly in y, density(numX, numY), from band-limited interpolation of
+        ! the corner spins (computed on the device).
+        subroutine calculate_winding_density_interpolated(chainMesh, z_index, density)
+                type(ChainMesh_t), intent(inout) :: chainMesh
+                integer, intent(in) :: z_index
+                real(kind=8), dimension(:,:), allocatable, intent(inout) :: density
+                integer(c_int) :: rc
+                integer :: numX, numY
+                if (z_index < 1 .or. z_index > chainMesh%numCellsZ) error stop "Z_index out of bounds"
+                numX = chainMesh%sclx * chainMesh%numCellsX
+                numY = chainMesh%scly * chainMesh%numCellsY
+                if (allocated(density)) then
+                        if (any(shape(density) /= [numX, numY])) deallocate(density)
+                end if
+                if (.not. allocated(density)) allocate(density(numX, numY))
+                call chainmesh_push_spins(chainMesh)
+                rc = ising_winding_density_interpolated(chainMesh%gpu, 0_c_int, int(z_index - 1, c_int), &
+                                                        int(chainMesh%sclx, c_int), int(chainMesh%scly, c_int), density)
+                call ising_check(rc, chainMesh%gpu, "calculate_winding_density_interpolated")
+        end subroutine calculate_winding_density_interpolated
+
+        ! Replaces write_winding_number_density_interpolated
+        ! (src/reciprocalSpaceProcesses.f90:502-540): header "x,y,z,Winding_Density",
+        ! then for every layer i, fine point j along x and k along y the position
+        ! (j, k, i) * a -- the reference writes fine indices times the cell
+        ! vectors -- and the density, each F0.8.
+        subroutine write_winding_number_density_interpolated(chainMesh, filepath)
+                type(ChainMesh_t), intent(inout) :: chainMesh
+                character(len=*), intent(in) :: filepath
+                real(kind=8), dimension(:,:), allocatable :: density_matrix
+                integer :: i, j, k, unit, stat
+                open(newunit=unit, file=filepath, status="replace", action="write", iostat=stat)
+                if (stat /= 0) error stop "Error opening file to write density information"
+                write(unit, '(A)') "x,y,z,Winding_Density"
+                do i = 1, chainMesh%numCellsZ
+                        call calculate_winding_density_interpolated(chainMesh, i, density_matrix)
+                        do j = 1, size(density_matrix, 1)
+                                do k = 1, size(density_matrix, 2)
+                                        call write_density_line(unit, chainMesh%a * dble(j), chainMesh%a * dble(k), &
+                                                                chainMesh%a * dble(i), density_matrix(j, k))
+                                end do
+                        end do
+                end do
+                close(unit)
+        end subroutine write_winding_number_density_interpolated
+
+        ! Replaces compute_skyrmion_distribution (src/reciprocalSpaceProcesses.f90:930-983):
+        ! winding_array(j) = number of regions of charge j (signed) of layer
+        ! Z_index (1-based), searched over num_thresholds mask thresholds from
+        ! max_threshold down to min_threshold and 20 filter widths, on the device.
+        subroutine compute_skyrmion_distribution(chainMesh, N, winding_array, min_threshold, max_threshold, &
+                                                 num_thresholds, Z_index)
+                type(ChainMesh_t), intent(inout) :: chainMesh
+                integer, intent(in) :: N
+                integer, dimension(:), allocatable, intent(inout) :: winding_array
+                real(kind=8), intent(in) :: min_threshold, max_threshold
+                integer, intent(in) :: num_thresholds, Z_index
+                type(ising_skyrmion_params) :: prm
+                integer(c_int32_t), allocatable :: counts(:)
+                integer(c_int) :: rc
+                if (max_threshold < min_threshold) error stop "max_threshold must be greater than min_threshold"
+                if (Z_index < 1 .or. Z_index > chainMesh%numCellsZ) error stop "Z_index out of bounds"
+                if (allocated(winding_array)) then
+                        if (size(winding_array) /= N) deallocate(winding_array)
+                end if
+                if (.not. allocated(winding_array)) allocate(winding_array(N))
+                allocate(counts(N))
+                prm%scale_x = chainMesh%sclx
+                prm%scale_y = chainMesh%scly
+                prm%n_particles = N
+                prm%num_thresholds = num_thresholds
+                prm%num_sigmas = 20
+                prm%min_threshold = min_threshold
+                prm%max_threshold = max_threshold
+                call chainmesh_push_spins(chainMesh)
+                rc = ising_skyrmion_distribution(chainMesh%gpu, 0_c_int, int(Z_index - 1, c_int), prm, counts, c_null_ptr)
+                call ising_check(rc, chainMesh%gpu, "compute_skyrmion_distribution")
+                winding_array = counts
+        end subroutine compute_skyrmion_distribution
+
         ! Replaces calculate_demagnetisation_field (src/reciprocalSpaceProcesses.f90:202-281):
         ! the demagnetising field of the current spins, computed on the device
         ! (cloud-in-cell deposit, discrete Fourier transform, -(k.M) k / k^2,
@@ -111,10 +193,9 @@ contains
         end function SlotFromIndices
 
         ! Replaces chainMesh_statistics (src/PT-Utils.f90:33-66): winding number
-        ! of every layer (one device pass), spread = max - min, the middle
-        ! layer numCellsZ/2, and the mean spin vector. The flood-fill
-        ! skyrmion count of the reference (compute_skyrmion_distribution) is
-        ! not computed here: skyrmion_number_middle returns nint(|Q_middle|).
+        ! of every layer (one device pass), spread = max - min, the skyrmion
+        ! count of the middle layer numCellsZ/2 (compute_skyrmion_distribution
+        ! with N = 1, thresholds 1e-5..0.95 in 60 steps), and the mean spin vector.
         subroutine chainMesh_statistics(chainMesh, skyrmion_number_middle, winding_number_middle, &
                                         winding_number_spread, magnetisation)
                 type(ChainMesh_t), intent(inout) :: chainMesh
@@ -126,6 +207,9 @@ contains
                 real(c_double) :: m(3)
                 integer(c_int) :: rc
                 integer :: mid
+                integer, allocatable :: skyrmion_array(:)
+                real(kind=8), parameter :: lower_bound = 1e-5_8, upper_bound = 0.95_8
+                integer, parameter :: num_thresholds = 60
                 call chainmesh_push_spins(chainMesh)
                 allocate(q(chainMesh%numCellsZ), qr(chainMesh%numCellsZ))
                 rc = ising_winding_all(chainMesh%gpu, q, qr)
@@ -133,7 +217,9 @@ contains
                 winding_number_spread = abs(maxval(qr) - minval(qr))
                 mid = max(1, chainMesh%numCellsZ / 2)
                 winding_number_middle = qr(mid)
-                skyrmion_number_middle = int(abs(q(mid)))
+                call compute_skyrmion_distribution(chainMesh, 1, skyrmion_array, lower_bound, upper_bound, &
+                                                   num_thresholds, mid)
+                skyrmion_number_middle = skyrmion_array(1)
                 rc = ising_magnetisation(chainMesh%gpu, m)
                 call ising_check(rc, chainMesh%gpu, "chainMesh_statistics: magnetisation")
                 magnetisation = m

@@ -68,7 +68,7 @@ program test_dropin
         print '(A,ES24.16)', "fm_winding ", winding_middle
         do ix = 0, 15
                 do iy = 0, 15
-                        th = pi * exp(-((ix - 7.7_8)**2 + (iy - 7.8_8)**2) / 9.0_8)
+                        th = pi * max(0.0_8, 1.0_8 - sqrt((ix - 7.7_8)**2 + (iy - 7.8_8)**2) / 5.5_8)
                         do iz = 0, 3
                                 k = (ix * 16 + iy) * 4 + iz + 1
                                 fm%atomSpins(k, :) = [sin(th) * cos(atan2(iy - 7.8_8, ix - 7.7_8)), &

@@ -114,6 +114,21 @@ int64_t orc_sweeps_demag(const orc_lattice* L, const orc_ham* h, double* s, doub
                          uint64_t seed, uint32_t slot, uint32_t sweep0, int n_sweeps, double mu0, int interval,
                          double* H);
 
+/* ---- skyrmion count of a layer (skyrmion_oracle.c) ----
+ * Coarse grid S[3][ny][nx]; fine grids and densities [NY][NX] with
+ * NX = sx nx, NY = sy ny; count tables [T][S][n_part]. */
+void    orc_sk_kernel_table(int n, int s, double* K);
+void    orc_sk_coarse(const orc_lattice* L, const double* s, int z, double* S);
+void    orc_sk_interpolate(int nx, int ny, int sx, int sy, const double* S, double* F);
+void    orc_sk_density(int NX, int NY, const double* F, double* d);
+void    orc_sk_gauss_weights(double sigma, double G[25]);
+void    orc_sk_filter(int NX, int NY, const double* d, double sigma, double* f);
+void    orc_sk_count_table(int NX, int NY, const double* d, int n_part, int T, int S, double min_t, double max_t,
+                           int32_t* counts);
+void    orc_sk_select(const int32_t* counts, int n_part, int T, int S, double q, int32_t* out);
+int     orc_sk_distribution(const orc_lattice* L, const double* s, int z, int sx, int sy, int n_part, double min_t,
+                            double max_t, int T, int S, int32_t* out, int32_t* table, double* d_out);
+
 /* ---- parallel tempering ---- */
 double  orc_swap_uniform(uint64_t seed, uint32_t ladder, uint32_t pair, uint32_t round);
 int     orc_pt_swap_round(int n_temps, const double* beta_of_temp, const double* e_of_slot,

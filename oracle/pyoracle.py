@@ -98,6 +98,18 @@ def lib() -> C.CDLL:
             "orc_mlp_nparam": (C.c_int, []),
             "orc_sweeps_learned": (C.c_int64, [lp, hp, dp, C.c_double, dp, C.c_int, C.c_double, C.c_double,
                                                C.c_uint64, C.c_uint32, C.c_uint32, C.c_int]),
+            "orc_sk_kernel_table": (None, [C.c_int, C.c_int, dp]),
+            "orc_sk_coarse": (None, [lp, dp, C.c_int, dp]),
+            "orc_sk_interpolate": (None, [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]),
+            "orc_sk_density": (None, [C.c_int, C.c_int, dp, dp]),
+            "orc_sk_gauss_weights": (None, [C.c_double, dp]),
+            "orc_sk_filter": (None, [C.c_int, C.c_int, dp, C.c_double, dp]),
+            "orc_sk_count_table": (None, [C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                          C.POINTER(C.c_int32)]),
+            "orc_sk_select": (None, [C.POINTER(C.c_int32), C.c_int, C.c_int, C.c_int, C.c_double,
+                                     C.POINTER(C.c_int32)]),
+            "orc_sk_distribution": (C.c_int, [lp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                              C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp]),
         }
 
         class ref_rng(C.Structure):
@@ -449,3 +461,69 @@ def sweeps_demag(L, h, s, beta, radius, seed, slot, sweep0, n, mu0=MU0_REF, inte
     if acc < 0:
         raise ValueError("demagnetisation needs a fully periodic lattice")
     return acc, H
+
+
+# ---- skyrmion count of a layer (skyrmion_oracle.c) ----------------------
+
+SK_THRESHOLDS, SK_SIGMAS = 60, 20
+SK_MIN_T, SK_MAX_T = 1e-5, 0.95
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def sk_coarse(L, s, z) -> np.ndarray:
+    """Corner spins of layer z, [3, ly, lx]."""
+    out = np.zeros((3, L.ly, L.lx))
+    lib().orc_sk_coarse(C.byref(L), _dp(np.ascontiguousarray(s, dtype=np.float64)), z, _dp(out))
+    return out
+
+
+def sk_interpolate(S, sx, sy) -> np.ndarray:
+    """Band-limited interpolation of a coarse grid [3, ny, nx] to [3, sy ny, sx nx]."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    _, ny, nx = S.shape
+    F = np.zeros((3, sy * ny, sx * nx))
+    lib().orc_sk_interpolate(nx, ny, sx, sy, _dp(S), _dp(F))
+    return F
+
+
+def sk_density(F) -> np.ndarray:
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    d = np.zeros(F.shape[1:])
+    lib().orc_sk_density(F.shape[2], F.shape[1], _dp(F), _dp(d))
+    return d
+
+
+def sk_filter(d, sigma) -> np.ndarray:
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    f = np.zeros_like(d)
+    lib().orc_sk_filter(d.shape[1], d.shape[0], _dp(d), sigma, _dp(f))
+    return f
+
+
+def sk_count_table(d, n_part=1, T=SK_THRESHOLDS, S=SK_SIGMAS, min_t=SK_MIN_T, max_t=SK_MAX_T) -> np.ndarray:
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    out = np.zeros((T, S, n_part), dtype=np.int32)
+    lib().orc_sk_count_table(d.shape[1], d.shape[0], _dp(d), n_part, T, S, min_t, max_t, _ip(out))
+    return out
+
+
+def sk_select(table, q) -> np.ndarray:
+    t = np.ascontiguousarray(table, dtype=np.int32)
+    out = np.zeros(t.shape[2], dtype=np.int32)
+    lib().orc_sk_select(_ip(t), t.shape[2], t.shape[0], t.shape[1], q, _ip(out))
+    return out
+
+
+def sk_distribution(L, s, z, sx=10, sy=10, n_part=1, T=SK_THRESHOLDS, S=SK_SIGMAS, min_t=SK_MIN_T, max_t=SK_MAX_T):
+    """(counts[n_part], table[T, S, n_part], fine density [sy ly, sx lx])."""
+    out = np.zeros(n_part, dtype=np.int32)
+    tab = np.zeros((T, S, n_part), dtype=np.int32)
+    d = np.zeros((sy * L.ly, sx * L.lx))
+    r = lib().orc_sk_distribution(C.byref(L), _dp(np.ascontiguousarray(s, dtype=np.float64)), z, sx, sy, n_part,
+                                  min_t, max_t, T, S, _ip(out), _ip(tab), _dp(d))
+    if r < 0:
+        raise ValueError("skyrmion count: x and y must be periodic")
+    return out, tab, d

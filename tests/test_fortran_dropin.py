@@ -80,7 +80,9 @@ def test_dropin_entry_points_on_gpu():
     assert v["max_norm_dev"][0] < 1e-6
     assert v["energy_after"][0] < v["spiral_energy"][0] + 200        # sweeps ran and stayed finite
     assert np.allclose(v["fm_mag"], [0, 0, 1]) and v["fm_winding"][0] == 0.0
-    assert abs(abs(v["sk_winding_middle"][0]) - 1) < 1e-9 and v["sk_number"][0] == 1
+    # skyrmion_number_middle: the signed count of compute_skyrmion_distribution
+    assert abs(abs(v["sk_winding_middle"][0]) - 1) < 1e-9
+    assert v["sk_number"][0] == round(v["sk_winding_middle"][0])
     assert abs(v["sk_layer1"][0] - v["sk_winding_middle"][0]) < 1e-12
     # calculate_winding_number_density of the same skyrmion: sums to the layer's
     # charge, peaks at the plaquette whose corner cell is next to the centre
@@ -129,7 +131,8 @@ def test_dropin_demagnetisation_on_gpu(tmp_path, orc):
 @needs_fortran
 def test_pt_driver_reference_cli_and_csv(tmp_path):
     out = tmp_path / "run"
-    env = {"ISING_PT_ITERATIONS": "4", "ISING_PT_SWEEPS_PER_SWAP": "10", "ISING_PT_RELAX": "20"}
+    env = {"ISING_PT_ITERATIONS": "4", "ISING_PT_SWEEPS_PER_SWAP": "10", "ISING_PT_RELAX": "20",
+           "ISING_PT_INTERP_SCALE": "2"}
     p = run([str(LIB / "pt_gpu"), "-1.0", "1.0", "2", "0.5", "0.5", "1", "0.1", "0.1", "1", str(out)], env=env)
     assert p.returncode == 0, p.stdout + p.stderr
     files = sorted(out.glob("data_*.csv"))
@@ -161,3 +164,18 @@ def test_pt_driver_reference_cli_and_csv(tmp_path):
         mid = d[np.isclose(d[:, 2], 3 * 2.0), 3].sum()
         row = [float(x) for x in list(csv.reader(f.open()))[1]]
         assert abs(mid - row[4]) < 1e-4 + 1e-6 * 900
+    # write_winding_number_density_interpolated: every layer on the 2x finer
+    # grid (fine indices times the cell edge); each layer's total is an
+    # integer (the charge of the interpolated field); skyrmion_number_middle
+    # is a count (an integer field of the CSV)
+    interp = sorted((tmp_path / "run_density_interpolated").glob("density_interp_*.csv"))
+    assert len(interp) == 20 and [d.name[15:] for d in interp] == [f.name[5:] for f in files]
+    for df, f in zip(interp[:3], files[:3]):
+        d = np.loadtxt(df, delimiter=",", skiprows=1)
+        assert d.shape == (60 * 60 * 6, 4)
+        assert d[:, 0].max() == pytest.approx(60 * 2.0) and d[:, 1].max() == pytest.approx(60 * 2.0)
+        for z in range(1, 7):
+            tot = d[np.isclose(d[:, 2], 2.0 * z), 3].sum()
+            assert abs(tot - round(tot)) < 1e-4 + 1e-8 * 3600, (z, tot)
+        row = list(csv.reader(f.open()))[1]
+        assert row[5].lstrip("-").isdigit()
