@@ -32,6 +32,10 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # a*b + c is fused only inside one source expression, so an inlined
+    # device function rounds the same way in every kernel that calls it (the
+    # fused sweep and the half sweeps give the same chains bit for bit)
+    "-ffp-contract=on",
     # matrix-core results land in ordinary VGPRs (no accumulator-register copies)
     "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]

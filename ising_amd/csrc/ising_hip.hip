@@ -787,29 +787,60 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
 }
 
 // ---------------------------------------------------------------------------
-// Fused sweep over LDS-staged planes: one workgroup per (replica, chunk of
-// lc z planes [z0, z1)) updates colour 0 and then colour 1 of its planes in
-// one pass. A z plane of one colour (P = ly*hx sites) is staged whole in LDS,
-// so x and y neighbours never leave it; the workgroup marches in z with two
-// rings of three planes:
-//   ring 0: colour-1 spins as they were before the sweep (planes s-1, s, s+1),
-//   ring 1: colour-0 spins after their update (planes s-2, s-1, s).
-// Step s updates colour 0 on plane s from ring 0, then (after a barrier)
-// colour 1 on plane s-1 from ring 1, and moves the colour-1 plane s+2, loaded
-// from HBM at the start of the step, into the ring-0 slot of plane s-1. The
-// halo planes z0-1 and z1 of colour 0 are recomputed from the same Philox
-// counters as their owners', so results equal the two half-sweep launches bit
-// for bit; every site of [z0, z1) is written to the destination buffer, and
-// halos of neighbouring chunks read only the source buffer.
-// Each thread keeps the same sites (k = j*blockDim + tid) on every plane, so
-// the ring-0 slot it reads its own colour-1 spin from in step s is the slot
-// it refills, at the same positions, in the same step.
+// Fused sweep: both colours of every site in one launch, marching along z.
+// One workgroup per (replica, chunk of lc z planes [z0, z1)). Thread t owns
+// the in-plane half indices j = t + m T (m < kFusedPasses) of every plane,
+// i.e. the pair of columns ix = 2h, 2h + 1 at row iy (j = iy hx + h): on each
+// plane one of the two sites is colour 0 and the other colour 1. Step s
+// (z0-1 <= s <= z1) updates colour 0 on plane s, then colour 1 on plane s-1:
+//   * the z neighbours of both updates are the other colour on the planes
+//     above and below in the same column, which the owning thread keeps in
+//     registers (colour 1 before the sweep on planes s-1 .. s+1, colour 0
+//     after its update on planes s-2 .. s), and so is one x neighbour (the
+//     other site of the column pair);
+//   * the other x neighbour and both y neighbours come from LDS: colour 1 of
+//     plane s before the sweep (for the colour-0 update) and colour 0 of plane
+//     s-1 after its update (for the colour-1 update), each double-buffered by
+//     plane parity, so a step has one barrier; a neighbour across an open face
+//     reads a slot that holds zeros;
+//   * the planes the next step needs are loaded one step ahead, through buffer
+//     descriptors whose scalar offsets select colour, component and plane (the
+//     per-thread vector offset never changes).
+// The register windows shift by copies: unrolling the z loop to rotate them
+// instead raised the register use past the 128 registers that 16 wavefronts
+// per CU allow, and the spills made the launch twice as slow.
+// Colour-0 planes z0-1 and z1 are halo: recomputed from the same Philox
+// counters as by their owners and not written. Reads come from the source
+// buffer and every owned site is written to the other one, so neighbouring
+// chunks never race. The neighbour sums, arithmetic and counters are those
+// of the half sweeps, so the chains are bit-identical.
 // kMeasure: per-thread FP64 sums of pair_energy and spins over the owned
-// colour-1 sites, then per wavefront into wave_part.
+// colour-1 sites, then per wavefront into wave_part; with kExact the
+// per-site energies are pair_energy_exact of the same six neighbours, in the
+// half sweeps' bond order (the energies a replica exchange takes).
 constexpr int kFusedMaxThreads = 1024;
-constexpr int kFusedPasses = 2;          // sites per colour and thread on a plane
+constexpr int kFusedPasses = 2;          // column pairs per thread on a plane
+constexpr int kBufferFormat = 0x00020000; // buffer descriptor word 3: 32-bit data
 
-template <bool kMeasure, bool kLearned>
+__device__ __forceinline__ f3 bld3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t s0, uint32_t sN)
+{
+    return f3{ __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)s0, 0)),
+               __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)(s0 + sN), 0)),
+               __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)(s0 + 2u * sN), 0)) };
+}
+__device__ __forceinline__ void bst3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t s0, uint32_t sN, const f3& v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.x), r, (int)voff, (int)s0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.y), r, (int)voff, (int)(s0 + sN), 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.z), r, (int)voff, (int)(s0 + 2u * sN), 0);
+}
+__device__ __forceinline__ f3 flip_sign(const f3& v, uint32_t mask)
+{
+    return f3{ __uint_as_float(__float_as_uint(v.x) ^ mask), __uint_as_float(__float_as_uint(v.y) ^ mask),
+               __uint_as_float(__float_as_uint(v.z) ^ mask) };
+}
+
+template <bool kMeasure, bool kLearned, bool kExact = false>
 __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     SpinBuf sb, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ fblock_rep,
@@ -817,7 +848,7 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count,
     double* __restrict__ wave_part, uint32_t tune, const MlpDev* __restrict__ mlp)
 {
-    extern __shared__ float lds[];    // [ring 2][slot 3][component 3][pstride]
+    extern __shared__ float lds[];    // [colour 2][parity 2][component 3][pstride], slot P of each is zero
     const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
     const int slot = fblock_rep[blk];
     const Rep R = reps[slot];
@@ -825,8 +856,6 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     const float beta = beta_slot[slot];
     const uint32_t sweep = *sweep_ctr + sweep_off;
     const uint32_t buf = *sb.ctr + sweep_off;       // every sweep of the period is fused
-    const float* src = spin_buffer(sb, buf);
-    float* dst = spin_buffer(sb, buf + 1u);
     const uint32_t P = (uint32_t)R.plane;
     const uint32_t T = blockDim.x, tid = threadIdx.x;
     const uint32_t hx = (uint32_t)R.hx, ly = (uint32_t)R.ly;
@@ -835,8 +864,14 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     const bool px = R.per & 1, py = (R.per >> 1) & 1;
     const int z0 = ((int)blk - R.f_first_block) * lc;
     const int z1 = min(z0 + lc, lz);
-    const Planes src0 = planes(src, R, 0), src1 = planes(src, R, 1);
-    const Planes dst0 = planes(dst, R, 0), dst1 = planes(dst, R, 1);
+    const bool small = R.n < (1 << 24);
+    // the replica's three component blocks in the source and destination
+    // buffers; byte offsets: component c at c*sN, colour 1 at sH, plane z at z*sP
+    const uint32_t sN = (uint32_t)R.n * 4u, sH = R.half * 4u, sP = P * 4u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(spin_buffer(sb, buf) + R.off), 0, (int)(3u * sN), kBufferFormat);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(spin_buffer(sb, buf + 1u) + R.off), 0, (int)(3u * sN), kBufferFormat);
 
     // Plane z of the lattice (periodic image), or -1 beyond an open face.
     auto wrapz = [&](int z) -> int {
@@ -844,135 +879,161 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
         if (z >= lz) return pz ? z - lz : -1;
         return z;
     };
-    auto ring = [&](int r, int z) -> float* { return lds + (size_t)(r * 3 + (z + 3) % 3) * 3 * pstride; };
+    auto lplane = [&](int c, int q) -> float* { return lds + (size_t)((c * 2 + q) * 3) * pstride; };
+    if (tid < 12u) lds[(size_t)tid * pstride + P] = 0.f;     // the zero slots
 
-    uint32_t kk[kFusedPasses], iyv[kFusedPasses], hv[kFusedPasses];
+    // Per owned column pair: its index, row and the LDS indices of its
+    // in-plane neighbours (the +x one of an odd site and the -x one of an even
+    // site, +-y; the zero slot across an open face).
+    uint32_t jj[kFusedPasses], iyv[kFusedPasses], xnp[kFusedPasses], xnm[kFusedPasses], ynp[kFusedPasses],
+        ynm[kFusedPasses], sb0[kFusedPasses], sb1[kFusedPasses];
     bool act[kFusedPasses];
 #pragma unroll
-    for (int j = 0; j < kFusedPasses; ++j) {
-        const uint32_t k = (uint32_t)j * T + tid;
-        act[j] = k < P;
-        kk[j] = act[j] ? k : P - 1u;
-        iyv[j] = fdiv(kk[j], R.div_hx);
-        hv[j] = fmodq(kk[j], iyv[j], R.div_hx);
+    for (int m = 0; m < kFusedPasses; ++m) {
+        const uint32_t k = (uint32_t)m * T + tid;
+        act[m] = k < P;
+        const uint32_t j = act[m] ? k : P - 1u;
+        jj[m] = j;
+        const uint32_t iy = fdiv(j, R.div_hx);
+        const uint32_t h = fmodq(j, iy, R.div_hx);
+        iyv[m] = iy;
+        const bool xpw = h == hx - 1u, xmw = h == 0u;
+        xnp[m] = xpw ? (px ? j + 1u - hx : P) : j + 1u;
+        xnm[m] = xmw ? (px ? j - 1u + hx : P) : j - 1u;
+        const bool ypw = iy + 1u == ly, ymw = iy == 0u;
+        ynp[m] = ypw ? (py ? j + hx - P : P) : j + hx;
+        ynm[m] = ymw ? (py ? j - hx + P : P) : j - hx;
+        // site number (ix*ly + iy)*lz of ix = 2h and 2h + 1, without the plane
+        sb0[m] = small ? __umul24(__umul24(2u * h, ly) + iy, (uint32_t)lz) : ((2u * h) * ly + iy) * (uint32_t)lz;
+        sb1[m] = small ? __umul24(__umul24(2u * h + 1u, ly) + iy, (uint32_t)lz) : ((2u * h + 1u) * ly + iy) * (uint32_t)lz;
     }
-    auto load_plane = [&](const Planes& S, int z, f3 (&v)[kFusedPasses]) {
+    auto load = [&](int colour, int z, int m) -> f3 {
         const int zz = wrapz(z);
-#pragma unroll
-        for (int j = 0; j < kFusedPasses; ++j)
-            v[j] = (zz >= 0 && act[j]) ? ld3(S, (uint32_t)zz * P + kk[j]) : f3{ 0.f, 0.f, 0.f };
+        return (zz >= 0 && act[m]) ? bld3(rs, jj[m] * 4u, (colour ? sH : 0u) + (uint32_t)zz * sP, sN)
+                                   : f3{ 0.f, 0.f, 0.f };
     };
-    auto put = [&](float* b, int j, const f3& v) {
-        b[kk[j]] = v.x; b[pstride + kk[j]] = v.y; b[2 * pstride + kk[j]] = v.z;
+    auto put = [&](float* b, int m, const f3& v) {
+        b[jj[m]] = v.x; b[pstride + jj[m]] = v.y; b[2 * pstride + jj[m]] = v.z;
     };
     auto get = [&](const float* b, uint32_t i) -> f3 { return f3{ b[i], b[pstride + i], b[2 * pstride + i] }; };
-    // Six neighbours of site j of a plane with parity p (ix & 1), from the
-    // other colour's ring slots of the planes below, at and above it (zero
-    // across open faces).
-    auto gather = [&](const float* bm, const float* b0, const float* bp, int j, uint32_t p, f3 (&v)[6]) {
-        const uint32_t k = kk[j], h = hv[j], iy = iyv[j];
-        const bool xp_w = (p != 0u) && (h == hx - 1u), xm_w = (p == 0u) && (h == 0u);
-        const bool yp_w = iy + 1u == ly, ym_w = iy == 0u;
-        v[0] = get(b0, k + p - (xp_w ? hx : 0u));
-        v[1] = get(b0, k + p - 1u + (xm_w ? hx : 0u));
-        v[2] = get(b0, k + hx - (yp_w ? P : 0u));
-        v[3] = get(b0, k - hx + (ym_w ? P : 0u));
-        v[4] = get(bp, k);
-        v[5] = get(bm, k);
-        if (!px || !py) {
-            if (!px && xp_w) v[0] = f3{ 0.f, 0.f, 0.f };
-            if (!px && xm_w) v[1] = f3{ 0.f, 0.f, 0.f };
-            if (!py && yp_w) v[2] = f3{ 0.f, 0.f, 0.f };
-            if (!py && ym_w) v[3] = f3{ 0.f, 0.f, 0.f };
-        }
-    };
 
-    f3 own0[kFusedPasses], pre0[kFusedPasses], pre1[kFusedPasses];
-    for (int z = z0 - 2; z <= z0; ++z) {
-        load_plane(src1, z, pre1);
+    // Register windows: colour 1 before the sweep on planes s-1, s, s+1;
+    // colour 0 after its update on planes s-2, s-1 and before it on plane s.
+    f3 am1[kFusedPasses], a0[kFusedPasses], ap1[kFusedPasses];
+    f3 bm2[kFusedPasses], bm1[kFusedPasses], b0[kFusedPasses];
 #pragma unroll
-        for (int j = 0; j < kFusedPasses; ++j)
-            if (act[j]) put(ring(0, z), j, pre1[j]);
+    for (int m = 0; m < kFusedPasses; ++m) {
+        am1[m] = load(1, z0 - 2, m);
+        a0[m] = load(1, z0 - 1, m);
+        ap1[m] = load(1, z0, m);
+        bm2[m] = f3{ 0.f, 0.f, 0.f };
+        bm1[m] = f3{ 0.f, 0.f, 0.f };
+        b0[m] = load(0, z0 - 1, m);
     }
-    load_plane(src0, z0 - 1, own0);
-    __syncthreads();
 
     double me = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
     uint32_t n_acc = 0;        // wavefront-uniform
-    const bool two = (uint32_t)T < P;    // second pass has sites (uniform)
     for (int s = z0 - 1; s <= z1; ++s) {
-        if (s <= z1 - 1) load_plane(src1, s + 2, pre1);
-        if (s + 1 <= z1) load_plane(src0, s + 1, pre0);
-        // colour 0 on plane s
         const int zs = wrapz(s);
-        float* b1s = ring(1, s);
-        if (zs >= 0) {
-            const bool owned = s >= z0 && s < z1;
-            const float *am = ring(0, s - 1), *a0 = ring(0, s), *ap = ring(0, s + 1);
+        const int q = s & 1;
+        float* l1 = lplane(1, q);          // colour 1 of plane s, before the sweep
+        float* l0 = lplane(0, q);          // colour 0 of plane s, after its update
+        const float* l0p = lplane(0, q ^ 1);    // colour 0 of plane s-1, after its update
 #pragma unroll
-            for (int j = 0; j < kFusedPasses; ++j) {
-                if (j > 0 && !two) break;
-                const uint32_t p = (iyv[j] + (uint32_t)zs) & 1u;
-                f3 nbr[6];
-                gather(am, a0, ap, j, p, nbr);
-                const f3 hf = field_from(L, nbr);
-                bool accepted;
-                const f3 fs = metropolis_site<kLearned>(L, beta, own0[j], nbr, hf,
-                                                        site_number(R, 2u * hv[j] + p, iyv[j], (uint32_t)zs), sweep,
-                                                        seed_lo, seed_hi, (uint32_t)slot, mlp, act[j], accepted);
-                if (act[j]) {
-                    put(b1s, j, fs);
-                    if (owned) {
-                        const uint32_t q = (uint32_t)zs * P + kk[j];
-                        stf((float*)dst0.x, q, fs.x); stf((float*)dst0.y, q, fs.y); stf((float*)dst0.z, q, fs.z);
-                    }
-                }
-                if (owned) n_acc += (uint32_t)__popcll(__ballot(accepted));
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < kFusedPasses; ++j)
-                if (act[j]) put(b1s, j, f3{ 0.f, 0.f, 0.f });
-        }
+        for (int m = 0; m < kFusedPasses; ++m)
+            if (act[m]) put(l1, m, a0[m]);
         __syncthreads();
-        // colour 1 on plane s-1
-        if (s - 1 >= z0) {
-            const uint32_t zb = (uint32_t)(s - 1);
-            const float* a0 = ring(0, s - 1);
-            const float *bm = ring(1, s - 2), *b0 = ring(1, s - 1);
+        // the planes step s+1 adds to the windows
+        f3 na[kFusedPasses], nb[kFusedPasses];
 #pragma unroll
-            for (int j = 0; j < kFusedPasses; ++j) {
-                if (j > 0 && !two) break;
-                const uint32_t p = (iyv[j] + zb + 1u) & 1u;
-                const f3 s1 = get(a0, kk[j]);
+        for (int m = 0; m < kFusedPasses; ++m) {
+            na[m] = s + 2 <= z1 + 1 ? load(1, s + 2, m) : f3{ 0.f, 0.f, 0.f };
+            nb[m] = s + 1 <= z1 ? load(0, s + 1, m) : f3{ 0.f, 0.f, 0.f };
+        }
+        // colour 0 on plane s
+        const bool owned0 = s >= z0 && s < z1;
+#pragma unroll
+        for (int m = 0; m < kFusedPasses; ++m) {
+            f3 fs = f3{ 0.f, 0.f, 0.f };
+            bool accepted = false;
+            if (zs >= 0) {             // uniform per workgroup
+                const uint32_t p = (iyv[m] + (uint32_t)zs) & 1u;
+                // +x and -x: the own pair's colour-1 site and the LDS one;
+                // their sum, and their difference with the sign of +x - -x
+                const f3 own = a0[m], oth = get(l1, p ? xnp[m] : xnm[m]);
+                const f3 yp = get(l1, ynp[m]), ym = get(l1, ynm[m]);
+                const f3 &zp = ap1[m], &zm = am1[m];
+                const f3 sum{ (own.x + oth.x) + (yp.x + ym.x) + (zp.x + zm.x),
+                              (own.y + oth.y) + (yp.y + ym.y) + (zp.y + zm.y),
+                              (own.z + oth.z) + (yp.z + ym.z) + (zp.z + zm.z) };
+                const f3 dx = flip_sign(f3{ own.x - oth.x, own.y - oth.y, own.z - oth.z }, p << 31);
+                const f3 dy{ yp.x - ym.x, yp.y - ym.y, yp.z - ym.z };
+                const f3 dz{ zp.x - zm.x, zp.y - zm.y, zp.z - zm.z };
+                const f3 hf = field_from_sums(L, sum, dx, dy, dz);
                 f3 nbr[6];
-                gather(bm, b0, b1s, j, p, nbr);
-                const f3 hf = field_from(L, nbr);
+                if constexpr (kLearned) {
+                    nbr[0] = p ? oth : own; nbr[1] = p ? own : oth;
+                    nbr[2] = yp; nbr[3] = ym; nbr[4] = zp; nbr[5] = zm;
+                }
+                fs = metropolis_site<kLearned>(L, beta, b0[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zs,
+                                               sweep, seed_lo, seed_hi, (uint32_t)slot, mlp, act[m], accepted);
+                if (act[m] && owned0) bst3(rd, jj[m] * 4u, (uint32_t)zs * sP, sN, fs);
+            }
+            if (owned0) n_acc += (uint32_t)__popcll(__ballot(accepted));
+            if (act[m]) put(l0, m, fs);
+            b0[m] = fs;                 // colour 0 of plane s, after its update
+        }
+        // colour 1 on plane s-1 (owned planes only; l0p was filled in step s-1)
+        const int zb = s - 1;
+        if (zb >= z0 && zb < z1) {     // uniform
+#pragma unroll
+            for (int m = 0; m < kFusedPasses; ++m) {
+                const uint32_t p = (iyv[m] + (uint32_t)zb + 1u) & 1u;
+                const f3 own = bm1[m], oth = get(l0p, p ? xnp[m] : xnm[m]);
+                const f3 yp = get(l0p, ynp[m]), ym = get(l0p, ynm[m]);
+                const f3 &zp = b0[m], &zm = bm2[m];
+                const f3 sum{ (own.x + oth.x) + (yp.x + ym.x) + (zp.x + zm.x),
+                              (own.y + oth.y) + (yp.y + ym.y) + (zp.y + zm.y),
+                              (own.z + oth.z) + (yp.z + ym.z) + (zp.z + zm.z) };
+                const f3 dx = flip_sign(f3{ own.x - oth.x, own.y - oth.y, own.z - oth.z }, p << 31);
+                const f3 dy{ yp.x - ym.x, yp.y - ym.y, yp.z - ym.z };
+                const f3 dz{ zp.x - zm.x, zp.y - zm.y, zp.z - zm.z };
+                const f3 hf = field_from_sums(L, sum, dx, dy, dz);
+                f3 nbr[6];
+                if constexpr (kLearned) {
+                    nbr[0] = p ? oth : own; nbr[1] = p ? own : oth;
+                    nbr[2] = yp; nbr[3] = ym; nbr[4] = zp; nbr[5] = zm;
+                }
                 bool accepted;
-                const f3 fs = metropolis_site<kLearned>(L, beta, s1, nbr, hf,
-                                                        site_number(R, 2u * hv[j] + p, iyv[j], zb), sweep,
-                                                        seed_lo, seed_hi, (uint32_t)slot, mlp, act[j], accepted);
-                if (act[j]) {
-                    const uint32_t q = zb * P + kk[j];
-                    stf((float*)dst1.x, q, fs.x); stf((float*)dst1.y, q, fs.y); stf((float*)dst1.z, q, fs.z);
+                const f3 fs = metropolis_site<kLearned>(L, beta, am1[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zb,
+                                                        sweep, seed_lo, seed_hi, (uint32_t)slot, mlp, act[m], accepted);
+                if (act[m]) {
+                    bst3(rd, jj[m] * 4u, sH + (uint32_t)zb * sP, sN, fs);
                     if constexpr (kMeasure) {
-                        const f3 c0 = get(b0, kk[j]);
-                        me += (double)pair_energy(L, fs, hf, c0);
-                        mx += (double)(fs.x + c0.x); my += (double)(fs.y + c0.y); mz += (double)(fs.z + c0.z);
+                        const f3 c0 = bm1[m];
+                        if constexpr (kExact) {
+                            const f3 v[6] = { p ? oth : own, p ? own : oth, yp, ym, zp, zm };
+                            double sum[3], A[3][3];
+                            sc_sums64(v, sum, A);
+                            me += pair_energy_exact(L, fs, sum, A, c0);
+                            mx += (double)fs.x + (double)c0.x;
+                            my += (double)fs.y + (double)c0.y;
+                            mz += (double)fs.z + (double)c0.z;
+                        } else {
+                            me += (double)pair_energy(L, fs, hf, c0);
+                            mx += (double)(fs.x + c0.x); my += (double)(fs.y + c0.y); mz += (double)(fs.z + c0.z);
+                        }
                     }
                 }
                 n_acc += (uint32_t)__popcll(__ballot(accepted));
             }
         }
-        if (s <= z1 - 1) {
-            float* a = ring(0, s + 2);     // the slot of plane s-1
+        // shift the windows to step s+1
 #pragma unroll
-            for (int j = 0; j < kFusedPasses; ++j)
-                if (act[j]) put(a, j, pre1[j]);
+        for (int m = 0; m < kFusedPasses; ++m) {
+            am1[m] = a0[m]; a0[m] = ap1[m]; ap1[m] = na[m];
+            bm2[m] = bm1[m]; bm1[m] = b0[m]; b0[m] = nb[m];
         }
-#pragma unroll
-        for (int j = 0; j < kFusedPasses; ++j) own0[j] = pre0[j];
-        __syncthreads();
     }
 
     const uint32_t wpb = T / kWave, wave = tid / kWave;
@@ -3009,11 +3070,11 @@ void launch_half_kernel(ising_ctx* c, uint32_t off, int colour, hipEvent_t t0 = 
                        (const double*)c->d_demag);
 }
 
-template <bool kMeasure, bool kLearned>
+template <bool kMeasure, bool kLearned, bool kExact = false>
 void launch_fused_kernel(ising_ctx* c, uint32_t off, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr)
 {
     if (t0) {
-        hipExtLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned>), dim3(c->n_fblocks), dim3(c->fused_threads),
+        hipExtLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned, kExact>), dim3(c->n_fblocks), dim3(c->fused_threads),
                               (uint32_t)c->fused_lds, c->stream, t0, t1, 0u, spin_buf(c), c->d_reps, c->d_lads,
                               (const float*)c->d_beta_slot, (const int32_t*)c->d_fblock_rep, (const uint32_t*)c->d_ctr,
                               off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
@@ -3021,7 +3082,7 @@ void launch_fused_kernel(ising_ctx* c, uint32_t off, hipEvent_t t0 = nullptr, hi
                               (const MlpDev*)c->d_mlp);
         return;
     }
-    hipLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned>), dim3(c->n_fblocks), dim3(c->fused_threads), c->fused_lds,
+    hipLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned, kExact>), dim3(c->n_fblocks), dim3(c->fused_threads), c->fused_lds,
                        c->stream, spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_fblock_rep,
                        (const uint32_t*)c->d_ctr, off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
                        c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu,
@@ -3065,7 +3126,7 @@ uint32_t buf_off(const ising_ctx* c, int done) { return fused_active(c) ? (uint3
 // period; with rec, also recorded in the series.
 int launch_measure_all(ising_ctx* c, int done = 0, const SeriesRec& rec = kNoRecord);
 
-// exact (half sweeps only): the colour-1 launch records FP64 per-site
+// exact: the colour-1 launch (or the fused sweep) records FP64 per-site
 // energies (pair_energy_exact) instead of FP32 ones. finalize = false leaves
 // the sums of a measuring sweep to launch_winding_all's combined finalize. With a hyper-kagome
 // ladder every sweep is three colour launches (colour 2 leaves two-colour
@@ -3082,7 +3143,10 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
     };
     int rc;
     if (fused_active(c)) {
-        if (measure) {
+        if (measure && exact) {
+            if (learned) launch_fused_kernel<true, true, true>(c, off);
+            else launch_fused_kernel<true, false, true>(c, off);
+        } else if (measure) {
             if (learned) launch_fused_kernel<true, true>(c, off);
             else launch_fused_kernel<true, false>(c, off);
         } else {
@@ -3388,10 +3452,10 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
         const SeriesRec rm = (do_meas && sd.desc) ? SeriesRec{ sd.desc, c->d_ctr + 1, nm } : kNoRecord;
         // Measurements ride on the sweep launches. Before a swap, whose
         // exponent takes FP64 per-site energies, the half sweep's colour-1
-        // launch computes them (the fused sweep and three-colour sweeps leave
+        // launch or the fused sweep computes them (three-colour sweeps leave
         // them to the measure kernel); the swap kernel then records E and M
         // at the new temperatures.
-        const bool exact = do_swap && !fused_active(c) && c->max_ncol == 2;
+        const bool exact = do_swap && c->max_ncol == 2;
         // a sweep that records E/M on its launches and also the charge finishes
         // both sums in one launch (before a swap E must be final first)
         const bool merge = do_meas && !do_swap && do_q && c->max_ncol == 2;
@@ -3412,9 +3476,10 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
     return launch_advance(c, (uint32_t)count, nm, nq);
 }
 
-// Fused-sweep resources: available when every replica's z plane of one
-// colour fits the two LDS rings (72 bytes per site) and two sites per thread
-// in one workgroup. Counter and partial arrays are sized for one plane per
+// Fused-sweep resources: available when every replica is simple cubic and
+// the LDS planes of the largest (two colours x two parities, 48 bytes per
+// site of one colour) fit, with two column pairs per thread in one
+// workgroup. Counter and partial arrays are sized for one plane per
 // workgroup, the finest chunking.
 int fused_setup(ising_ctx* c)
 {
@@ -3428,12 +3493,17 @@ int fused_setup(ising_ctx* c)
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, c->device);
     lds_max = std::max(lds_max, optin);
-    const size_t lds = (size_t)18 * pmax * sizeof(float);
+    // 2 colours x 2 parities x 3 components of (plane + one zero slot)
+    const size_t lds = (size_t)12 * (pmax + 1) * sizeof(float);
     int threads = (int)((((pmax + kFusedPasses - 1) / kFusedPasses) + kWave - 1) / kWave * kWave);
-    c->fused_ok = all_sc && pmax > 0 && threads <= kFusedMaxThreads && lds <= (size_t)lds_max;
+    // the kernel addresses a replica's spins through 32-bit buffer offsets
+    int64_t nmax = 0;
+    for (const Rep& r : c->reps) nmax = std::max(nmax, r.n);
+    c->fused_ok = all_sc && pmax > 0 && threads <= kFusedMaxThreads && lds <= (size_t)lds_max &&
+                  12 * nmax < ((int64_t)1 << 31);
     c->fused_threads = std::max(threads, kWave);
     c->fused_wpb = c->fused_threads / kWave;
-    c->fused_pstride = (uint32_t)pmax;
+    c->fused_pstride = (uint32_t)pmax + 1u;
     c->fused_lds = lds;
     c->max_fblocks = zsum;
     if (c->fused_ok) {
@@ -4589,7 +4659,7 @@ int ising_set_tuning(ising_ctx* c, uint32_t flags)
     if (lc > 0 && !c->fused_ok)
         return set_err(c, ISING_ERR_UNSUPPORTED,
                        "fused sweep unavailable: it needs simple-cubic replicas whose z plane of one colour "
-                       "(%u sites here) fits in LDS", c->fused_pstride);
+                       "(%u sites here) fits in LDS", c->fused_pstride - 1u);
     HIP_TRY(c, hipSetDevice(c->device));
     if (lc != fused_lc(c)) {
         int rc = fused_layout(c, lc);

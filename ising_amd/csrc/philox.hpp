@@ -24,6 +24,13 @@ constexpr uint32_t kStreamSwap = 2u;
 
 struct u32x4 { uint32_t x, y, z, w; };
 
+// a ^ b ^ c in one instruction (gfx950's three-input bitwise operation,
+// truth table 0x96); the compiler emits two XORs for the plain expression.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
 {
 #pragma unroll
@@ -33,7 +40,10 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        c = u32x4{ hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0 };
+        // round 0: counter words 1 and 3 are usually wavefront-uniform like
+        // the key, so their XOR folds into one scalar operation
+        if (r == 0) c = u32x4{ hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0 };
+        else c = u32x4{ xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0 };
     }
     return c;
 }
