@@ -17,9 +17,11 @@ r = 0.5, and learned cone.
 value  = site updates/s of the fixed-cone run (every replica's sites x sweeps
 / wall time); also reported per sampler: tau_int-corrected independent
 samples/s of E and of Q (middle layer) and their learned/fixed ratio, the
-half-sweep kernel's HBM roofline fraction from hipEvent pairs around its
-launches inside the same schedule (ising_time_pt_sweeps, 200 sweeps after
-each timed run), and the FP64 CPU oracle's rate on this host's cores.
+plain sweep kernel's HBM roofline fraction (the fused sweep when the
+autotuner keeps a fused flag set, else the half sweep; the learned sampler
+always runs half sweeps) from hipEvent pairs around its launches inside the
+same schedule (ising_time_pt_sweeps, 200 sweeps after each timed run), and
+the FP64 CPU oracle's rate on this host's cores.
 
 The tau_int half of the metric comes from a record window of its own,
 10,000 sweeps per sampler with the same schedule (timed: samples/s uses
@@ -247,7 +249,7 @@ def main() -> int:
         q_series = out["winding_raw"][:, :, L // 2]               # middle layer, [n_q, R]
         _, tau_e, conv_e = independent_samples(e_series)
         _, tau_q, conv_q = independent_samples(q_series)
-        # the plain half-sweep kernel timed inside this schedule (hipEvents)
+        # the plain sweep launches timed inside this schedule (hipEvents), per half sweep
         ms_launch = eng.time_pt_sweeps(LAUNCH_TIMING_SWEEPS, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY,
                                        winding_layer=L // 2)
         return {"wall": wall, "updates_per_s": n_sites * n_sweeps / wall, "ms_per_launch": ms_launch,
@@ -264,10 +266,16 @@ def main() -> int:
     progress(f"learned: {learned['wall'] * 1e3 / args.steps:.3f} ms/step")
     att, acc = eng.swap_stats()
 
-    bytes_launch = BYTES_PER_UPDATE * n_sites / 2
-    achieved = bytes_launch / (fixed["ms_per_launch"] * 1e-3) / 1e9
+    # The dominant kernel of the fixed-cone run: one fused launch per sweep
+    # when the autotuner kept a fused flag set, else the plain half sweep.
+    # time_pt_sweeps reports per half-sweep equivalent; a fused launch is two.
+    fused = (tune_flags >> 8) & 0xFF > 0
+    halves = 2 if fused else 1
+    ms_launch = fixed["ms_per_launch"] * halves
+    bytes_launch = BYTES_PER_UPDATE * n_sites / 2 * halves
+    achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
     counters = counters_from_profiles()
-    traffic = counters.get("hbm_bytes_per_launch")
+    traffic = counters.get("fused", {}).get("hbm_bytes_per_launch") if fused else counters.get("hbm_bytes_per_launch")
     copy_ms = eng.bandwidth_probe(256 << 20)
     mlp_tflops = MLP_FLOPS_PER_UPDATE * n_sites / 2 / (learned["ms_per_launch"] * 1e-3) / 1e12
 
@@ -277,7 +285,7 @@ def main() -> int:
 
     def summary(r: dict) -> dict:
         out = {"site_updates_per_s": r["updates_per_s"], "ms_per_step": r["wall"] * 1e3 / args.steps,
-               "ms_per_half_sweep_launch": r["ms_per_launch"], "tau_window_sweeps": TAU_SWEEPS,
+               "ms_per_half_sweep": r["ms_per_launch"], "tau_window_sweeps": TAU_SWEEPS,
                "tau_window_s": r["tau_wall"]}
         for o, key in (("E", "samples_per_s_E"), ("Q", "samples_per_s_Q_middle_layer")):
             conv, tau = r[f"conv_{o}"], r[f"tau_int_{o}_sweeps"]
@@ -326,8 +334,10 @@ def main() -> int:
                             "mean": float((acc[:-1] / np.maximum(att[:-1], 1)).mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "valu_busy": counters.get("valu_busy"), "learned_valu_busy": counters.get("learned_valu_busy"),
-                     "kernel": "k_half_sweep", "ms_per_launch": fixed["ms_per_launch"],
+                     "valu_busy": counters.get("fused", {}).get("valu_busy") if fused else counters.get("valu_busy"),
+                     "learned_valu_busy": counters.get("learned_valu_busy"),
+                     "kernel": "k_sweep_fused<false, false>" if fused else "k_half_sweep<false, false, false, false>",
+                     "ms_per_launch": ms_launch, "site_updates_per_launch": n_sites // 2 * halves,
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "copy_probe_GBs": 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9,
                      "tuning_flags": tune_flags,

@@ -821,6 +821,10 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
 constexpr int kFusedMaxThreads = 1024;
 constexpr int kFusedPasses = 2;          // column pairs per thread on a plane
 constexpr int kBufferFormat = 0x00020000; // buffer descriptor word 3: 32-bit data
+// A vector offset at or past a descriptor's byte range: loads through it
+// return zero and read no memory (every fused-sweep descriptor spans less
+// than 2^31 bytes).
+constexpr uint32_t kBufferOob = 0x80000000u;
 
 __device__ __forceinline__ f3 bld3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t s0, uint32_t sN)
 {
@@ -840,15 +844,15 @@ __device__ __forceinline__ f3 flip_sign(const f3& v, uint32_t mask)
                __uint_as_float(__float_as_uint(v.z) ^ mask) };
 }
 
-template <bool kMeasure, bool kLearned, bool kExact = false>
+template <bool kMeasure, bool kExact = false>
 __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     SpinBuf sb, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ fblock_rep,
     const uint32_t* __restrict__ sweep_ctr, uint32_t sweep_off, int lc, uint32_t pstride,
     uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count,
-    double* __restrict__ wave_part, uint32_t tune, const MlpDev* __restrict__ mlp)
+    double* __restrict__ wave_part, uint32_t tune)
 {
-    extern __shared__ float lds[];    // [colour 2][parity 2][component 3][pstride], slot P of each is zero
+    extern __shared__ float4 lds[];   // [colour 2][parity 2][pstride] of (x, y, z, unused); slot P of each is zero
     const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
     const int slot = fblock_rep[blk];
     const Rep R = reps[slot];
@@ -879,8 +883,8 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
         if (z >= lz) return pz ? z - lz : -1;
         return z;
     };
-    auto lplane = [&](int c, int q) -> float* { return lds + (size_t)((c * 2 + q) * 3) * pstride; };
-    if (tid < 12u) lds[(size_t)tid * pstride + P] = 0.f;     // the zero slots
+    auto lplane = [&](int c, int q) -> float4* { return lds + (size_t)(c * 2 + q) * pstride; };
+    if (tid < 4u) lds[(size_t)tid * pstride + P] = float4{ 0.f, 0.f, 0.f, 0.f };     // the zero slots
 
     // Per owned column pair: its index, row and the LDS indices of its
     // in-plane neighbours (the +x one of an odd site and the -x one of an even
@@ -907,48 +911,60 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
         sb0[m] = small ? __umul24(__umul24(2u * h, ly) + iy, (uint32_t)lz) : ((2u * h) * ly + iy) * (uint32_t)lz;
         sb1[m] = small ? __umul24(__umul24(2u * h + 1u, ly) + iy, (uint32_t)lz) : ((2u * h + 1u) * ly + iy) * (uint32_t)lz;
     }
+    // Colour `colour` of plane z in the source buffer. Every lane loads (an
+    // idle lane's index is clamped into the plane); a plane beyond an open
+    // face, or one the chunk never reads, is read through an offset past the
+    // descriptor's range, which returns zeros without a branch.
     auto load = [&](int colour, int z, int m) -> f3 {
-        const int zz = wrapz(z);
-        return (zz >= 0 && act[m]) ? bld3(rs, jj[m] * 4u, (colour ? sH : 0u) + (uint32_t)zz * sP, sN)
-                                   : f3{ 0.f, 0.f, 0.f };
+        const int zz = z > z1 + 1 ? -1 : wrapz(z);      // uniform
+        return bld3(rs, jj[m] * 4u + (zz >= 0 ? 0u : kBufferOob),
+                    zz >= 0 ? (colour ? sH : 0u) + (uint32_t)zz * sP : 0u, sN);
     };
-    auto put = [&](float* b, int m, const f3& v) {
-        b[jj[m]] = v.x; b[pstride + jj[m]] = v.y; b[2 * pstride + jj[m]] = v.z;
+    // one 16-byte LDS access per site: a single address per neighbour
+    auto put = [&](float4* b, int m, const f3& v) { b[jj[m]] = float4{ v.x, v.y, v.z, 0.f }; };
+    auto get = [&](const float4* b, uint32_t i) -> f3 {
+        const float4 t = b[i];
+        return f3{ t.x, t.y, t.z };
     };
-    auto get = [&](const float* b, uint32_t i) -> f3 { return f3{ b[i], b[pstride + i], b[2 * pstride + i] }; };
 
-    // Register windows: colour 1 before the sweep on planes s-1, s, s+1;
-    // colour 0 after its update on planes s-2, s-1 and before it on plane s.
-    f3 am1[kFusedPasses], a0[kFusedPasses], ap1[kFusedPasses];
-    f3 bm2[kFusedPasses], bm1[kFusedPasses], b0[kFusedPasses];
+    // Register windows: colour 1 before the sweep on planes s-1, s, s+1 and
+    // the plane s+2 loaded during step s; colour 0 after its update on planes
+    // s-2, s-1, before it (then after it) on plane s, and plane s+1 loaded
+    // during step s. Four slots each. The plain sweep rotates them by
+    // renaming (a loop body of four steps whose window arguments shift by one
+    // slot, so no step copies a window); the measuring variants copy them
+    // after every step, which keeps their register use under the 128 that 16
+    // wavefronts per CU allow (renamed, they spill).
+    f3 A[4][kFusedPasses], B[4][kFusedPasses];
 #pragma unroll
     for (int m = 0; m < kFusedPasses; ++m) {
-        am1[m] = load(1, z0 - 2, m);
-        a0[m] = load(1, z0 - 1, m);
-        ap1[m] = load(1, z0, m);
-        bm2[m] = f3{ 0.f, 0.f, 0.f };
-        bm1[m] = f3{ 0.f, 0.f, 0.f };
-        b0[m] = load(0, z0 - 1, m);
+        A[0][m] = load(1, z0 - 2, m);
+        A[1][m] = load(1, z0 - 1, m);
+        A[2][m] = load(1, z0, m);
+        B[0][m] = f3{ 0.f, 0.f, 0.f };
+        B[1][m] = f3{ 0.f, 0.f, 0.f };
+        B[2][m] = load(0, z0 - 1, m);
     }
 
     double me = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
     uint32_t n_acc = 0;        // wavefront-uniform
-    for (int s = z0 - 1; s <= z1; ++s) {
+    using Win = f3[kFusedPasses];
+    auto step = [&](int s, Win& am1, Win& a0, Win& ap1, Win& na, Win& bm2, Win& bm1, Win& b0,
+                    Win& nb) __attribute__((always_inline)) {
         const int zs = wrapz(s);
         const int q = s & 1;
-        float* l1 = lplane(1, q);          // colour 1 of plane s, before the sweep
-        float* l0 = lplane(0, q);          // colour 0 of plane s, after its update
-        const float* l0p = lplane(0, q ^ 1);    // colour 0 of plane s-1, after its update
+        float4* l1 = lplane(1, q);         // colour 1 of plane s, before the sweep
+        float4* l0 = lplane(0, q);         // colour 0 of plane s, after its update
+        const float4* l0p = lplane(0, q ^ 1);   // colour 0 of plane s-1, after its update
 #pragma unroll
         for (int m = 0; m < kFusedPasses; ++m)
             if (act[m]) put(l1, m, a0[m]);
         __syncthreads();
         // the planes step s+1 adds to the windows
-        f3 na[kFusedPasses], nb[kFusedPasses];
 #pragma unroll
         for (int m = 0; m < kFusedPasses; ++m) {
-            na[m] = s + 2 <= z1 + 1 ? load(1, s + 2, m) : f3{ 0.f, 0.f, 0.f };
-            nb[m] = s + 1 <= z1 ? load(0, s + 1, m) : f3{ 0.f, 0.f, 0.f };
+            na[m] = load(1, s + 2, m);
+            nb[m] = load(0, s + 1, m);
         }
         // colour 0 on plane s
         const bool owned0 = s >= z0 && s < z1;
@@ -970,13 +986,9 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                 const f3 dy{ yp.x - ym.x, yp.y - ym.y, yp.z - ym.z };
                 const f3 dz{ zp.x - zm.x, zp.y - zm.y, zp.z - zm.z };
                 const f3 hf = field_from_sums(L, sum, dx, dy, dz);
-                f3 nbr[6];
-                if constexpr (kLearned) {
-                    nbr[0] = p ? oth : own; nbr[1] = p ? own : oth;
-                    nbr[2] = yp; nbr[3] = ym; nbr[4] = zp; nbr[5] = zm;
-                }
-                fs = metropolis_site<kLearned>(L, beta, b0[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zs,
-                                               sweep, seed_lo, seed_hi, (uint32_t)slot, mlp, act[m], accepted);
+                const f3 nbr[6] = {};      // (the learned proposal's input; never fused)
+                fs = metropolis_site<false>(L, beta, b0[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zs,
+                                            sweep, seed_lo, seed_hi, (uint32_t)slot, nullptr, act[m], accepted);
                 if (act[m] && owned0) bst3(rd, jj[m] * 4u, (uint32_t)zs * sP, sN, fs);
             }
             if (owned0) n_acc += (uint32_t)__popcll(__ballot(accepted));
@@ -999,14 +1011,10 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                 const f3 dy{ yp.x - ym.x, yp.y - ym.y, yp.z - ym.z };
                 const f3 dz{ zp.x - zm.x, zp.y - zm.y, zp.z - zm.z };
                 const f3 hf = field_from_sums(L, sum, dx, dy, dz);
-                f3 nbr[6];
-                if constexpr (kLearned) {
-                    nbr[0] = p ? oth : own; nbr[1] = p ? own : oth;
-                    nbr[2] = yp; nbr[3] = ym; nbr[4] = zp; nbr[5] = zm;
-                }
+                const f3 nbr[6] = {};
                 bool accepted;
-                const f3 fs = metropolis_site<kLearned>(L, beta, am1[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zb,
-                                                        sweep, seed_lo, seed_hi, (uint32_t)slot, mlp, act[m], accepted);
+                const f3 fs = metropolis_site<false>(L, beta, am1[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zb,
+                                                     sweep, seed_lo, seed_hi, (uint32_t)slot, nullptr, act[m], accepted);
                 if (act[m]) {
                     bst3(rd, jj[m] * 4u, sH + (uint32_t)zb * sP, sN, fs);
                     if constexpr (kMeasure) {
@@ -1028,11 +1036,25 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                 n_acc += (uint32_t)__popcll(__ballot(accepted));
             }
         }
-        // shift the windows to step s+1
+    };
+    if constexpr (!kMeasure) {
+        for (int s = z0 - 1; s <= z1; s += 4) {
+            step(s, A[0], A[1], A[2], A[3], B[0], B[1], B[2], B[3]);
+            if (s + 1 > z1) break;
+            step(s + 1, A[1], A[2], A[3], A[0], B[1], B[2], B[3], B[0]);
+            if (s + 2 > z1) break;
+            step(s + 2, A[2], A[3], A[0], A[1], B[2], B[3], B[0], B[1]);
+            if (s + 3 > z1) break;
+            step(s + 3, A[3], A[0], A[1], A[2], B[3], B[0], B[1], B[2]);
+        }
+    } else {
+        for (int s = z0 - 1; s <= z1; ++s) {
+            step(s, A[0], A[1], A[2], A[3], B[0], B[1], B[2], B[3]);
 #pragma unroll
-        for (int m = 0; m < kFusedPasses; ++m) {
-            am1[m] = a0[m]; a0[m] = ap1[m]; ap1[m] = na[m];
-            bm2[m] = bm1[m]; bm1[m] = b0[m]; b0[m] = nb[m];
+            for (int m = 0; m < kFusedPasses; ++m) {
+                A[0][m] = A[1][m]; A[1][m] = A[2][m]; A[2][m] = A[3][m];
+                B[0][m] = B[1][m]; B[1][m] = B[2][m]; B[2][m] = B[3][m];
+            }
         }
     }
 
@@ -3045,9 +3067,12 @@ float* cur_spins(const ising_ctx* c) { return c->d_spins + (int64_t)(c->host_fus
 int fused_lc(const ising_ctx* c) { return (int)((c->tune >> 8) & 0xFFu); }
 
 // Sweeps run as fused launches (which swap the spin buffers): the fused flag
-// is set and no ladder has the demagnetisation term, which only the
-// in-place half sweeps carry.
-bool fused_active(const ising_ctx* c) { return fused_lc(c) > 0 && !c->any_demag; }
+// is set, no ladder has the demagnetisation term, which only the in-place
+// half sweeps carry, and no ladder takes its radius from the network (the
+// network's registers do not fit beside the fused sweep's windows under the
+// 128 per thread of a 1024-thread workgroup, so the learned sampler keeps
+// the half sweeps under every tuning).
+bool fused_active(const ising_ctx* c) { return fused_lc(c) > 0 && !c->any_demag && !c->any_learned; }
 
 // t0/t1: start/stop events in the dispatch packet (timed launches only;
 // graph-captured launches take the plain path).
@@ -3070,23 +3095,21 @@ void launch_half_kernel(ising_ctx* c, uint32_t off, int colour, hipEvent_t t0 = 
                        (const double*)c->d_demag);
 }
 
-template <bool kMeasure, bool kLearned, bool kExact = false>
+template <bool kMeasure, bool kExact = false>
 void launch_fused_kernel(ising_ctx* c, uint32_t off, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr)
 {
     if (t0) {
-        hipExtLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned, kExact>), dim3(c->n_fblocks), dim3(c->fused_threads),
+        hipExtLaunchKernelGGL((k_sweep_fused<kMeasure, kExact>), dim3(c->n_fblocks), dim3(c->fused_threads),
                               (uint32_t)c->fused_lds, c->stream, t0, t1, 0u, spin_buf(c), c->d_reps, c->d_lads,
                               (const float*)c->d_beta_slot, (const int32_t*)c->d_fblock_rep, (const uint32_t*)c->d_ctr,
                               off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
-                              c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu,
-                              (const MlpDev*)c->d_mlp);
+                              c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu);
         return;
     }
-    hipLaunchKernelGGL((k_sweep_fused<kMeasure, kLearned, kExact>), dim3(c->n_fblocks), dim3(c->fused_threads), c->fused_lds,
+    hipLaunchKernelGGL((k_sweep_fused<kMeasure, kExact>), dim3(c->n_fblocks), dim3(c->fused_threads), c->fused_lds,
                        c->stream, spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_fblock_rep,
                        (const uint32_t*)c->d_ctr, off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
-                       c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu,
-                       (const MlpDev*)c->d_mlp);
+                       c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu);
 }
 
 // Sweep number `off` of a period (relative to the device sweep counter):
@@ -3144,16 +3167,13 @@ int launch_sweep(ising_ctx* c, uint32_t off, bool measure = false, const SeriesR
     int rc;
     if (fused_active(c)) {
         if (measure && exact) {
-            if (learned) launch_fused_kernel<true, true, true>(c, off);
-            else launch_fused_kernel<true, false, true>(c, off);
+            launch_fused_kernel<true, true>(c, off);
         } else if (measure) {
-            if (learned) launch_fused_kernel<true, true>(c, off);
-            else launch_fused_kernel<true, false>(c, off);
+            launch_fused_kernel<true>(c, off);
         } else {
             if (timer) timer->launches_per_pair = 2;
             if ((rc = pair())) return rc;
-            if (learned) launch_fused_kernel<false, true>(c, off, t0, t1);
-            else launch_fused_kernel<false, false>(c, off, t0, t1);
+            launch_fused_kernel<false>(c, off, t0, t1);
         }
         LAUNCH_CHECK(c);
         if (measure && finalize) {
@@ -3477,7 +3497,7 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
 }
 
 // Fused-sweep resources: available when every replica is simple cubic and
-// the LDS planes of the largest (two colours x two parities, 48 bytes per
+// the LDS planes of the largest (two colours x two parities, 64 bytes per
 // site of one colour) fit, with two column pairs per thread in one
 // workgroup. Counter and partial arrays are sized for one plane per
 // workgroup, the finest chunking.
@@ -3493,8 +3513,8 @@ int fused_setup(ising_ctx* c)
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, c->device);
     lds_max = std::max(lds_max, optin);
-    // 2 colours x 2 parities x 3 components of (plane + one zero slot)
-    const size_t lds = (size_t)12 * (pmax + 1) * sizeof(float);
+    // 2 colours x 2 parities of (plane + one zero slot), 16 bytes a site
+    const size_t lds = (size_t)4 * (pmax + 1) * sizeof(float4);
     int threads = (int)((((pmax + kFusedPasses - 1) / kFusedPasses) + kWave - 1) / kWave * kWave);
     // the kernel addresses a replica's spins through 32-bit buffer offsets
     int64_t nmax = 0;
@@ -3507,8 +3527,8 @@ int fused_setup(ising_ctx* c)
     c->fused_lds = lds;
     c->max_fblocks = zsum;
     if (c->fused_ok) {
-        const void* fns[] = { (const void*)k_sweep_fused<false, false>, (const void*)k_sweep_fused<true, false>,
-                              (const void*)k_sweep_fused<false, true>, (const void*)k_sweep_fused<true, true> };
+        const void* fns[] = { (const void*)k_sweep_fused<false>, (const void*)k_sweep_fused<true>,
+                              (const void*)k_sweep_fused<true, true> };
         for (const void* f : fns)
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
                 c->fused_ok = false;

@@ -73,6 +73,8 @@ def test_fused_uneven_ladders():
 
 
 def test_fused_learned_equals_half_sweeps():
+    """A fused tuning with the learned proposal on: the learned sampler keeps
+    the half sweeps, so the chains equal the untuned ones."""
     rng = np.random.default_rng(4)
     p = initial_params(scale=0.3, seed=4)
     p[1760:1792] = rng.normal(scale=0.1, size=32)
@@ -82,6 +84,34 @@ def test_fused_learned_equals_half_sweeps():
     for r in range(len(BETAS)):
         assert np.array_equal(ref[0][r], got[0][r]), r
     assert np.array_equal(ref[1], got[1])
+
+
+def test_switch_between_fused_and_learned_sweeps():
+    """An odd count of fused sweeps leaves the state in the second spin
+    buffer; learned sweeps (half sweeps) continue from it, eager and in
+    replayed PT periods, and fused sweeps take over again afterwards: the
+    chains equal those of half sweeps throughout."""
+    p = initial_params(scale=0.3, seed=5)
+    lat = Lattice(16, 16, 8)
+    outs = []
+    for flags in (0, tune_fused(3)):
+        with engine(lat, HAM, BETAS, seed=31) as e:
+            e.set_tuning(flags)
+            e.mlp_set_params(p)
+            e.init_random()
+            e.sweep(3)
+            e.mlp_enable(True)
+            e.sweep(2)
+            e.pt_run(n_sweeps=6, swap_every=3, measure_every=1, winding_every=0)
+            e.mlp_enable(False)
+            o = e.pt_run(n_sweeps=9, swap_every=3, measure_every=1, winding_every=0)
+            outs.append(([e.get_spins(r) for r in range(len(BETAS))], e.accept_counts(), e.temperature_map(),
+                         o["energy"]))
+    for a, b in zip(outs[0][0], outs[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][2], outs[1][2])
+    assert np.allclose(outs[0][3], outs[1][3], rtol=1e-12, atol=1e-9)
 
 
 def test_fused_graph_odd_period():
