@@ -913,10 +913,11 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     }
     // Colour `colour` of plane z in the source buffer. Every lane loads (an
     // idle lane's index is clamped into the plane); a plane beyond an open
-    // face, or one the chunk never reads, is read through an offset past the
-    // descriptor's range, which returns zeros without a branch.
+    // face, or one the chunk never reads (colour 1 above z1 + 1, colour 0
+    // above z1), is read through an offset past the descriptor's range, which
+    // returns zeros without a branch or a memory access.
     auto load = [&](int colour, int z, int m) -> f3 {
-        const int zz = z > z1 + 1 ? -1 : wrapz(z);      // uniform
+        const int zz = z > z1 + colour ? -1 : wrapz(z);      // uniform
         return bld3(rs, jj[m] * 4u + (zz >= 0 ? 0u : kBufferOob),
                     zz >= 0 ? (colour ? sH : 0u) + (uint32_t)zz * sP : 0u, sN);
     };
@@ -3123,19 +3124,32 @@ constexpr SeriesRec kNoRecord{ nullptr, nullptr, 0u };
 // (hipExtLaunchKernel): the kernels' own execution time inside the workload
 // they are part of, without the barrier packets of separate event records
 // (ising_time_pt_sweeps).
+// Events that only time device work: without the system-scope release an
+// event normally carries, recording one does not write back and invalidate
+// the caches, which would add several microseconds to the launch it closes.
+hipError_t timing_event(hipEvent_t* e) { return hipEventCreateWithFlags(e, hipEventDisableSystemFence); }
+
 struct LaunchTimer {
     std::vector<hipEvent_t> ev;
     size_t used = 0;
     int launches_per_pair = 1;   // half sweeps covered by one timed launch (2 for a fused sweep)
     ~LaunchTimer() { for (hipEvent_t e : ev) (void)hipEventDestroy(e); }
-    hipError_t next_pair(hipEvent_t& a, hipEvent_t& b)
+    // Events are created before the timed launches are enqueued: creating
+    // them between launches lets the device idle while the host catches up.
+    hipError_t reserve(size_t n)
     {
-        while (ev.size() < used + 2) {
+        while (ev.size() < n) {
             hipEvent_t e;
-            hipError_t r = hipEventCreate(&e);
+            hipError_t r = timing_event(&e);
             if (r != hipSuccess) return r;
             ev.push_back(e);
         }
+        return hipSuccess;
+    }
+    hipError_t next_pair(hipEvent_t& a, hipEvent_t& b)
+    {
+        hipError_t r = reserve(used + 2);
+        if (r != hipSuccess) return r;
         a = ev[used]; b = ev[used + 1];
         used += 2;
         return hipSuccess;
@@ -3306,11 +3320,15 @@ int demag_setup(ising_ctx* c)
             c->dft_lds_zp = std::max(c->dft_lds_zp, (size_t)(3 * r.lz * (d.tz + 1) + r.lz) * sizeof(double2));
         }
     }
+    // A set-up that failed part way keeps what it allocated (the sizes depend
+    // only on the context's lattices), so a retry neither leaks nor
+    // allocates twice; the context frees them at destruction.
+    auto once = [&](auto** p, size_t n) { return *p ? ISING_OK : dalloc(c, p, n); };
     int rc;
-    if ((rc = dalloc(c, &c->d_demag, (size_t)c->total_floats)) || (rc = dalloc(c, &c->d_dmesh, (size_t)moff)) ||
-        (rc = dalloc(c, &c->d_dk1, (size_t)koff)) || (rc = dalloc(c, &c->d_dk2, (size_t)koff)) ||
-        (rc = dalloc(c, &c->d_tw, tw.size())) || (rc = dalloc(c, &c->d_dreps, c->dreps.size())) ||
-        (rc = dalloc(c, &c->d_cic, cic.size())))
+    if ((rc = once(&c->d_demag, (size_t)c->total_floats)) || (rc = once(&c->d_dmesh, (size_t)moff)) ||
+        (rc = once(&c->d_dk1, (size_t)koff)) || (rc = once(&c->d_dk2, (size_t)koff)) ||
+        (rc = once(&c->d_tw, tw.size())) || (rc = once(&c->d_dreps, c->dreps.size())) ||
+        (rc = once(&c->d_cic, cic.size())))
         return rc;
     HIP_TRY(c, hipMemcpyAsync(c->d_cic, cic.data(), cic.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     // LDS beyond the default 64 KB per workgroup (axes longer than 512)
@@ -4547,8 +4565,8 @@ int ising_time_sweeps(ising_ctx* c, int n_sweeps, double* ms_per_launch)
     if (!c || !ms_per_launch || n_sweeps < 1) return ISING_ERR_INVALID_ARG;
     HIP_TRY(c, hipSetDevice(c->device));
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(c, hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
+    HIP_TRY(c, timing_event(&e0));
+    if (timing_event(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
     int rc = ISING_OK;
     if (hipEventRecord(e0, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
     for (int k = 0; k < n_sweeps && !rc; ++k) {
@@ -4575,8 +4593,8 @@ int ising_time_demag(ising_ctx* c, int n_fields, double* ms_per_field)
     int rc = demag_setup(c);
     if (rc) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(c, hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
+    HIP_TRY(c, timing_event(&e0));
+    if (timing_event(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
     if (hipEventRecord(e0, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
     for (int k = 0; k < n_fields && !rc; ++k) rc = launch_demag(c, kDemagLadders);
     if (!rc && hipEventRecord(e1, c->stream) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventRecord");
@@ -4603,6 +4621,9 @@ int ising_time_pt_sweeps(ising_ctx* c, const ising_schedule* sched, int n_sweeps
     for (int v : { s.swap_every, s.measure_every, s.winding_every })
         if (v > 0) period = period / gcd64(period, v) * v;
     LaunchTimer timer;
+    // at most two plain launches per sweep, two events each (three-colour
+    // contexts take more and create the rest as they go)
+    if (timer.reserve(4 * (size_t)n_sweeps) != hipSuccess) return set_err(c, ISING_ERR_HIP, "hipEventCreate");
     SeriesDev none;                      // records are computed, not stored
     int rc = ISING_OK, done = 0;
     // The record counters restart as in ising_pt_run (they only index series).
@@ -4647,7 +4668,7 @@ int ising_bandwidth_probe(ising_ctx* c, int64_t bytes, double* ms)
     if ((rc = dalloc(c, &b, n))) { (void)hipFree(a); return rc; }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     rc = ISING_OK;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventCreate");
+    if (timing_event(&e0) != hipSuccess || timing_event(&e1) != hipSuccess) rc = set_err(c, ISING_ERR_HIP, "hipEventCreate");
     // one 16-byte element per thread: on MI355X this reaches ~6.1 TB/s over
     // buffers beyond the Infinity Cache, where grid-stride loops stay near 5
     const unsigned grid = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, (size_t)1 << 30);
@@ -4769,8 +4790,8 @@ int ising_autotune_schedule(ising_ctx* c, ising_schedule* sched, const int32_t* 
         return set_err(c, ISING_ERR_INVALID_ARG, "bad schedule");
     HIP_TRY(c, hipSetDevice(c->device));
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(c, hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
+    HIP_TRY(c, timing_event(&e0));
+    if (timing_event(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return set_err(c, ISING_ERR_HIP, "hipEventCreate"); }
     std::vector<double> t(n_cand, 1e300);
     SeriesDev none;
     int rc = ISING_OK;
