@@ -260,7 +260,7 @@ def main() -> int:
         eng.synchronize()
         wall = time.perf_counter() - t0
         e_series = out["energy"]                                  # [n_meas, R]
-        q_series = out["winding_raw"][:, :, L // 2]               # middle layer, [n_q, R]
+        q_series = out["winding_raw"]                             # middle layer, [n_q, R]
         _, tau_e, conv_e = independent_samples(e_series)
         _, tau_q, conv_q = independent_samples(q_series)
         # the plain sweep launches timed inside this schedule (hipEvents), per

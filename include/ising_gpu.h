@@ -119,13 +119,15 @@ typedef struct {
     int32_t measure_every;   /* E and M recorded every k sweeps (0=off)        */
     int32_t winding_every;   /* per-layer Q recorded every k sweeps (0=off)    */
     int32_t use_graph;       /* 1: replay one captured hipGraph per period     */
-    int32_t winding_layer;   /* 0: Q of every layer; z+1: Q of layer z only     */
+    int32_t winding_layer;   /* 0: Q of every layer; z+1: Q of layer z only ([n_q][R] series) */
 } ising_schedule;
 
 /* Measurement series of the last ising_pt_run, indexed by temperature
  * position t in the ladder concatenation (not by storage slot). Any pointer
  * may be NULL. Sizes: n_meas = n_sweeps/measure_every,
- * n_q = n_sweeps/winding_every, R = total replicas, Zmax = max lz. */
+ * n_q = n_sweeps/winding_every, R = total replicas, Zmax = max lz. With a
+ * schedule that names one layer (winding_layer = z+1) the charge series hold
+ * that layer alone: [n_q][R]. */
 typedef struct {
     double*  energy;         /* [n_meas][R]                                   */
     double*  magnetisation;  /* [n_meas][R][3] mean spin vector               */

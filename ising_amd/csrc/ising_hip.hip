@@ -1162,6 +1162,7 @@ struct SeriesDesc {
     double* qr;
     long long* qi;
     uint32_t cap_e, cap_q;
+    uint32_t zrec;      // charge entries per (record, temperature): Zmax, or 1 for one layer
 };
 
 // One record of a series, written by the kernel that finishes the value:
@@ -1737,16 +1738,20 @@ __global__ __launch_bounds__(kBlock) void k_sk_count(int64_t n, const int32_t* _
 
 // One workgroup per (slot, layer): fixed-order sum, Q = sum / (4 pi). In
 // one-layer mode (layer >= 0) the other layers read 0.
+// Block b finishes the charge of one (slot, layer): every layer of every
+// slot (layer < 0, zmax blocks per slot) or the one layer (one block per
+// slot).
 __device__ __forceinline__ void winding_finalize_block(
     int b, double* lds, const Rep* __restrict__ reps, const double* __restrict__ qpartial, int zmax, int layer,
     double* __restrict__ qraw, long long* __restrict__ qint, const int32_t* __restrict__ slot_of_temp, int nrep,
     const SeriesRec& rec)
 {
-    const int slot = b / zmax;
-    const int z = b - slot * zmax;
+    const int zl = layer < 0 ? zmax : 1;
+    const int slot = b / zl;
+    const int z = layer < 0 ? b - slot * zl : layer;
     const Rep R = reps[slot];
     double q = 0.0;
-    if (z < R.lz && (layer < 0 || z == layer)) {      // uniform
+    if (z < R.lz) {      // uniform
         double v[4] = { 0.0, 0.0, 0.0, 0.0 };
         const int base = layer < 0 ? R.q_first_block + z * R.q_blocks_layer : R.l_first_block;
         for (int j = threadIdx.x; j < R.q_blocks_layer; j += kBlock) v[0] += qpartial[base + j];
@@ -1754,15 +1759,15 @@ __device__ __forceinline__ void winding_finalize_block(
         q = v[0] / (4.0 * 3.14159265358979323846);
     }
     if (threadIdx.x == 0) {
-        qraw[b] = q;
-        qint[b] = llrint(q);
+        qraw[slot * zmax + z] = q;
+        qint[slot * zmax + z] = llrint(q);
     }
     if (rec.desc) {       // uniform
         const int t = temp_of_slot(slot_of_temp, nrep, slot);
         const uint32_t m = *rec.ctr + rec.local;
         const SeriesDesc& D = *rec.desc;
         if (threadIdx.x == 0 && m < D.cap_q && t >= 0) {
-            const int64_t e = ((int64_t)m * nrep + t) * zmax + z;
+            const int64_t e = ((int64_t)m * nrep + t) * D.zrec + (D.zrec == 1u ? 0 : z);
             D.qr[e] = q;
             D.qi[e] = llrint(q);
         }
@@ -1787,7 +1792,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize_q_and_e(
     SeriesRec rm, const int32_t* __restrict__ slot_of_temp, int nrep)
 {
     __shared__ double lds[4 * (kBlock / kWave)];
-    const int nq = nrep * zmax;
+    const int nq = nrep * (layer < 0 ? zmax : 1);
     if ((int)blockIdx.x < nq)
         winding_finalize_block(blockIdx.x, lds, reps, qpartial, zmax, layer, qraw, qint, slot_of_temp, nrep, rq);
     else
@@ -3421,6 +3426,7 @@ int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0, const SeriesR
                        bool with_e = false, const SeriesRec& rec_e = kNoRecord)
 {
     const uint32_t off = buf_off(c, done);
+    const int zl = layer < 0 ? c->zmax : 1;       // finalize blocks per slot
     if (layer < 0) {
         hipLaunchKernelGGL(k_winding, dim3(c->n_qblocks), dim3(kBlock), 0, c->stream,
                            spin_buf(c), off, c->d_reps, (const int32_t*)c->d_qblock_rep, -1, c->d_qpartial);
@@ -3431,12 +3437,12 @@ int launch_winding_all(ising_ctx* c, int layer = -1, int done = 0, const SeriesR
     LAUNCH_CHECK(c);
     if (with_e) {
         const bool fz = fused_active(c);
-        hipLaunchKernelGGL(k_finalize_q_and_e, dim3(c->R * c->zmax + c->R), dim3(kBlock), 0, c->stream,
+        hipLaunchKernelGGL(k_finalize_q_and_e, dim3(c->R * zl + c->R), dim3(kBlock), 0, c->stream,
                            c->d_reps, (const double*)c->d_qpartial, c->zmax, layer, c->d_qraw, c->d_qint, rec,
                            (const double*)(fz ? c->d_wave_part_f : c->d_wave_part), fz ? c->fused_wpb : 0,
                            c->d_energy, c->d_mag, rec_e, (const int32_t*)c->d_slot_of_temp, c->R);
     } else {
-        hipLaunchKernelGGL(k_winding_finalize, dim3(c->R * c->zmax), dim3(kBlock), 0, c->stream,
+        hipLaunchKernelGGL(k_winding_finalize, dim3(c->R * zl), dim3(kBlock), 0, c->stream,
                            c->d_reps, c->d_qpartial, c->zmax, layer, c->d_qraw, c->d_qint,
                            (const int32_t*)c->d_slot_of_temp, c->R, rec);
     }
@@ -3644,11 +3650,11 @@ SeriesLayout series_layout(size_t n_meas, size_t n_q, size_t R, size_t Z)
 }
 
 // Series buffers of the context, grown to hold n_meas E/M and n_q charge
-// records (they are never shrunk; a reallocation needs no new graph, the
+// records of Z entries per temperature (they are never shrunk; a reallocation needs no new graph, the
 // captured launches read the buffers through the descriptor).
-int series_reserve(ising_ctx* c, size_t n_meas, size_t n_q)
+int series_reserve(ising_ctx* c, size_t n_meas, size_t n_q, size_t Z)
 {
-    const size_t R = (size_t)c->R, Z = (size_t)c->zmax;
+    const size_t R = (size_t)c->R;
     int rc;
     if (n_meas * R > c->ser_cap_e) {
         if (c->ser_e) (void)hipFree(c->ser_e);
@@ -3696,7 +3702,9 @@ int pt_run_impl(ising_ctx* c, const ising_schedule* sched, const ising_series* o
     int rc;
     const size_t n_meas = s.measure_every ? (size_t)(s.n_sweeps / s.measure_every) : 0u;
     const size_t n_q = s.winding_every ? (size_t)(s.n_sweeps / s.winding_every) : 0u;
-    const size_t R = (size_t)c->R, Z = (size_t)c->zmax;
+    const size_t R = (size_t)c->R;
+    // one layer named: its charges alone, [n_q][R]; else [n_q][R][Zmax]
+    const size_t Z = s.winding_layer > 0 ? 1u : (size_t)c->zmax;
     SeriesDesc desc{};
     SeriesDesc* d_desc = nullptr;
     if (staging) {
@@ -3707,12 +3715,13 @@ int pt_run_impl(ising_ctx* c, const ising_schedule* sched, const ising_series* o
         desc.qr = (double*)(staging + lay.qr);
         desc.qi = (long long*)(staging + lay.qi);
     } else {
-        if ((rc = series_reserve(c, n_meas, n_q))) return rc;
+        if ((rc = series_reserve(c, n_meas, n_q, Z))) return rc;
         d_desc = c->d_series;
         desc.e = c->ser_e; desc.m = c->ser_m; desc.qr = c->ser_qr; desc.qi = c->ser_qi;
     }
     desc.cap_e = (uint32_t)n_meas;
     desc.cap_q = (uint32_t)n_q;
+    desc.zrec = (uint32_t)Z;
     SeriesDev sd;
     sd.desc = d_desc;
     // The descriptor and the record counters of this run (plain kernels: no
@@ -4541,7 +4550,7 @@ int ising_pt_run_static(ising_ctx* c, const ising_schedule* sched, const ising_s
         return set_err(c, ISING_ERR_INVALID_ARG, "negative schedule entry");
     const size_t n_meas = s.measure_every ? (size_t)(s.n_sweeps / s.measure_every) : 0u;
     const size_t n_q = s.winding_every ? (size_t)(s.n_sweeps / s.winding_every) : 0u;
-    const size_t need = series_layout(n_meas, n_q, (size_t)c->R, (size_t)c->zmax).total;
+    const size_t need = series_layout(n_meas, n_q, (size_t)c->R, s.winding_layer > 0 ? 1u : (size_t)c->zmax).total;
     if (need > kStaticScratchBytes)
         return set_err(c, ISING_ERR_INVALID_ARG, "series of %zu bytes exceed the static scratch of %zu bytes "
                        "(ISING_STATIC_SCRATCH_BYTES)", need, kStaticScratchBytes);

@@ -315,10 +315,11 @@ class Engine:
                winding_layer: int | None = None, static: bool = False) -> dict:
         """Run the PT schedule on device; returns series indexed by temperature
         position: energy [n_meas, R], magnetisation [n_meas, R, 3],
-        winding / winding_raw [n_q, R, Zmax] (with winding_layer = z only that
-        layer is computed; the other layers read 0). static=True calls
-        ising_pt_run_static (no device allocation, eager periods)."""
-        R, Z = self.n_replicas, self.zmax
+        winding / winding_raw [n_q, R, Zmax], or [n_q, R] with winding_layer = z
+        (only that layer is computed). static=True calls ising_pt_run_static
+        (no device allocation, eager periods)."""
+        R = self.n_replicas
+        Z = (self.zmax,) if winding_layer is None else ()
         n_meas = n_sweeps // measure_every if measure_every else 0
         n_q = n_sweeps // winding_every if winding_every else 0
         out = {}
@@ -330,8 +331,8 @@ class Engine:
             out["magnetisation"] = np.zeros((n_meas, R, 3))
             ser.magnetisation = out["magnetisation"].ctypes.data_as(C.POINTER(C.c_double))
         if n_q and "winding" in want:
-            out["winding"] = np.zeros((n_q, R, Z), dtype=np.int64)
-            out["winding_raw"] = np.zeros((n_q, R, Z))
+            out["winding"] = np.zeros((n_q, R) + Z, dtype=np.int64)
+            out["winding_raw"] = np.zeros((n_q, R) + Z)
             ser.winding = out["winding"].ctypes.data_as(C.POINTER(C.c_int64))
             ser.winding_raw = out["winding_raw"].ctypes.data_as(C.POINTER(C.c_double))
         sch = _capi.ising_schedule(n_sweeps, swap_every, measure_every, winding_every, 1 if use_graph else 0,

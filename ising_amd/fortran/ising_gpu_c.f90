@@ -59,7 +59,7 @@ module ising_gpu_c
 
         type, bind(C) :: ising_schedule
                 integer(c_int32_t) :: n_sweeps, swap_every, measure_every, winding_every, use_graph
-                integer(c_int32_t) :: winding_layer   ! 0: every layer; z+1: layer z only
+                integer(c_int32_t) :: winding_layer   ! 0: every layer; z+1: layer z only, series [n_q][R]
         end type ising_schedule
 
         type, bind(C) :: ising_series

@@ -173,8 +173,8 @@ def test_learned_matches_fixed_observables():
         ma = mean_error(np.linalg.norm(res[False]["magnetisation"][:, k], axis=1))
         mb = mean_error(np.linalg.norm(res[True]["magnetisation"][:, k], axis=1))
         assert abs(ma[0] - mb[0]) < 3 * np.hypot(ma[1], mb[1]) + 1e-9, (k, ma, mb)
-        qa = mean_error(res[False]["winding"][:, k, 6].astype(float))
-        qb = mean_error(res[True]["winding"][:, k, 6].astype(float))
+        qa = mean_error(res[False]["winding"][:, k].astype(float))
+        qb = mean_error(res[True]["winding"][:, k].astype(float))
         assert abs(qa[0] - qb[0]) < 3 * np.hypot(qa[1], qb[1]) + 1e-9, (k, qa, qb)
         ea, eb = res[False]["energy"][:, k], res[True]["energy"][:, k]
         ca, cea = specific_heat(ea, betas[k], lat.sites), specific_heat_error(ea, betas[k], lat.sites)

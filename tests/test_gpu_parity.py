@@ -415,6 +415,29 @@ def test_time_pt_sweeps_advances_like_pt_run():
         assert np.array_equal(x, y)
 
 
+def test_single_layer_series_are_that_layer_of_the_full_series():
+    """A schedule that names one layer records that layer's charges alone
+    ([n_q, R]), equal to its column of the every-layer series from the same
+    checkpoint, through the allocating and the allocation-free entry points."""
+    lat = Lattice(12, 12, 8)
+    betas = 1.0 / geometric_temperatures(0.4, 2.0, 4)
+    with engine(lat, HAMS["bulk_K"], betas) as e:
+        e.init_random()
+        e.pt_run(n_sweeps=20, swap_every=5, measure_every=0)
+        e.save_state()
+        full = e.pt_run(n_sweeps=40, swap_every=10, measure_every=2, winding_every=4)
+        e.restore_state()
+        one = e.pt_run(n_sweeps=40, swap_every=10, measure_every=2, winding_every=4, winding_layer=3)
+        e.restore_state()
+        st = e.pt_run(n_sweeps=40, swap_every=10, measure_every=2, winding_every=4, winding_layer=3, static=True)
+    assert full["winding"].shape == (10, 4, 8) and one["winding"].shape == (10, 4)
+    assert np.array_equal(one["winding"], full["winding"][:, :, 3])
+    assert np.array_equal(one["winding_raw"], full["winding_raw"][:, :, 3])
+    for k in ("winding", "winding_raw", "energy", "magnetisation"):
+        assert np.array_equal(st[k], one[k]), k
+    assert np.array_equal(one["energy"], full["energy"])
+
+
 def test_checkpoint_restore_replays_exactly():
     """ising_save_state / ising_restore_state: a restored run repeats the
     same random numbers, swaps and records bit for bit."""

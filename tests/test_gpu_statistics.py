@@ -45,7 +45,7 @@ def gpu_series(lat, ham, temps, n_meas, seed=5):
     out = []
     for t in range(len(temps)):
         out.append((o["energy"][:, t], np.linalg.norm(o["magnetisation"][:, t], axis=1),
-                    o["winding"][:, t, zmid].astype(float)))
+                    o["winding"][:, t].astype(float)))
     return out
 
 
