@@ -274,8 +274,9 @@ int  ising_pt_run(ising_ctx* ctx, const ising_schedule* sched, const ising_serie
  * statically allocated device array of ISING_STATIC_SCRATCH_BYTES per GPU
  * (held until the process ends) and every period is launched eagerly, so the
  * call makes no hipMalloc, hipFree or graph instantiation. The staging needs
- * 256 + (n_meas*R*4 + n_q*R*Zmax*2)*8 bytes plus alignment (n_meas, n_q, R,
- * Zmax as in ising_series); larger schedules return ISING_ERR_INVALID_ARG.
+ * 256 + (n_meas*R*4 + n_q*R*Zq*2)*8 bytes plus alignment (n_meas, n_q, R as
+ * in ising_series; Zq = Zmax, or 1 when the schedule names one layer);
+ * larger schedules return ISING_ERR_INVALID_ARG.
  * Every call on one GPU shares that GPU's array: calls that overlap on one
  * GPU (several host threads, or several contexts on one device used at
  * once) race. */
