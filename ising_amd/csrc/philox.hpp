@@ -40,9 +40,11 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        // round 0: counter words 1 and 3 are usually wavefront-uniform like
-        // the key, so their XOR folds into one scalar operation
-        if (r == 0) c = u32x4{ hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0 };
+        // rounds 0 and 1: with counter words 1..3 wavefront-uniform (sweep,
+        // stream, seed) one operand of each XOR besides the key is uniform or
+        // zero, and the plain form lets them fold into scalar operations (the
+        // three-input form would need a vector copy of one of two scalars)
+        if (r < 2) c = u32x4{ hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0 };
         else c = u32x4{ xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0 };
     }
     return c;
