@@ -4,7 +4,8 @@ C5, the headline: 64^3 periodic simple cubic, 64 replicas on a geometric
 ladder T = 0.2..2.0, J = 1, bulk D = 0.5, Bz = 0.2, cone r = 0.5; a fixed-
 radius segment and a learned-radius segment (a random network), each with
 swaps every 10 sweeps, E/M every 2 and the middle layer's Q every 2, as the
-benchmark runs them. Checked against the FP64 oracle: the last records' E
+benchmark runs them, with half-sweep launches and with the fused sweep the
+benchmark uses at this shape. Checked against the FP64 oracle: the last records' E
 and M of every temperature against orc.total_energy / orc.magnetisation of
 the final spins (E to 1e-9 N, M to 1e-12), the middle layer's Q as an
 integer against orc.winding_layer, one sweep of four replicas replayed on
@@ -26,6 +27,7 @@ import pytest
 
 from gpu_util import olat
 from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
+from ising_amd._capi import TUNE_XCD_REMAP, tune_fused
 from oracle import pyoracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -112,12 +114,16 @@ def replay_swap_rounds(e: Engine, rounds_done: int, n_rounds: int = 6) -> None:
     assert np.array_equal(att1 - att0, ra) and np.array_equal(acc1 - acc0, rc)
 
 
-@pytest.fixture(scope="module")
-def c5():
+@pytest.fixture(scope="module", params=["half_sweeps", "fused16"])
+def c5(request):
     """The C5 context and the device counters the replays need: sweeps and
-    swap rounds run so far."""
+    swap rounds run so far; once with two half-sweep launches per sweep and
+    once with the fused sweep the benchmark's autotuner keeps at this shape
+    (16 z planes per workgroup, die-contiguous blocks)."""
     T = geometric_temperatures(0.2, 2.0, R5)
     with Engine([Ladder(Lattice(L5, L5, L5), HAM5, list(1.0 / T), 0.5)], seed=SEED5) as e:
+        if request.param == "fused16":
+            e.set_tuning(tune_fused(16) | TUNE_XCD_REMAP)
         e.init_random()
         yield e, {"sweeps": 0, "rounds": 0}
 

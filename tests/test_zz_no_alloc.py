@@ -2,7 +2,8 @@
 allocated) against ising_pt_run on the same chains: two calls in a row, the
 second with a smaller schedule, each pair started from one device
 checkpoint, so records, final spins, temperature maps and swap counters must
-agree bit for bit. Also the size limit of the static staging."""
+agree bit for bit, with half sweeps and with the fused sweep. Also the size
+limit of the static staging."""
 import numpy as np
 import pytest
 
@@ -26,13 +27,15 @@ def _same(a: dict, b: dict) -> None:
         assert np.array_equal(a[k], b[k]), k
 
 
-def test_static_matches_allocating_entry_point():
+@pytest.mark.parametrize("tuning", [0, _capi.tune_fused(4)], ids=["half_sweeps", "fused4"])
+def test_static_matches_allocating_entry_point(tuning):
     lat = Lattice(16, 16, 8)
     ham = Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2)
     T = geometric_temperatures(0.5, 2.0, 6)
     runs = [dict(n_sweeps=60, swap_every=5, measure_every=2, winding_every=3),
             dict(n_sweeps=24, swap_every=4, measure_every=3, winding_every=4, winding_layer=3)]
     with Engine([Ladder(lat, ham, list(1.0 / T), 0.5)], seed=31) as e:
+        e.set_tuning(tuning)
         e.init_random()
         e.pt_run(20, 5, 0, 0)
         for k, kw in enumerate(runs):
