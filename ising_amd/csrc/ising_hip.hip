@@ -798,17 +798,18 @@ __global__ __launch_bounds__(kBlock) void k_half_sweep(
 //     registers (colour 1 before the sweep on planes s-1 .. s+1, colour 0
 //     after its update on planes s-2 .. s), and so is one x neighbour (the
 //     other site of the column pair);
-//   * the other x neighbour and both y neighbours come from LDS: colour 1 of
-//     plane s before the sweep (for the colour-0 update) and colour 0 of plane
-//     s-1 after its update (for the colour-1 update), each double-buffered by
-//     plane parity, so a step has one barrier; a neighbour across an open face
-//     reads a slot that holds zeros;
+//   * the other x neighbour and both y neighbours come from LDS (16 bytes a
+//     site, one access each): colour 1 of plane s before the sweep (for the
+//     colour-0 update) and colour 0 of plane s-1 after its update (for the
+//     colour-1 update), each double-buffered by plane parity, so a step has
+//     one barrier; a neighbour across an open face reads a slot that holds
+//     zeros;
 //   * the planes the next step needs are loaded one step ahead, through buffer
 //     descriptors whose scalar offsets select colour, component and plane (the
 //     per-thread vector offset never changes).
-// The register windows shift by copies: unrolling the z loop to rotate them
-// instead raised the register use past the 128 registers that 16 wavefronts
-// per CU allow, and the spills made the launch twice as slow.
+// The plain variant renames its register windows (the z loop is unrolled by
+// four); the measuring ones copy them after each step, which keeps them
+// within the 128 registers that 16 wavefronts per CU allow.
 // Colour-0 planes z0-1 and z1 are halo: recomputed from the same Philox
 // counters as by their owners and not written. Reads come from the source
 // buffer and every owned site is written to the other one, so neighbouring
