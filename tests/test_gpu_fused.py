@@ -27,6 +27,8 @@ LATS = {
     "open_z_odd_8x6x5": Lattice(8, 6, 5, (True, True, False)),
     "open_xy_10x7x8": Lattice(10, 7, 8, (False, False, True)),
     "film_32x32x4": Lattice(32, 32, 4, (True, True, False)),
+    "monolayer_16x12x1": Lattice(16, 12, 1, (True, True, False)),
+    "bilayer_10x8x2": Lattice(10, 8, 2, (True, True, False)),
 }
 BETAS = [2.0, 0.9, 0.4]
 
