@@ -694,8 +694,13 @@ __device__ __forceinline__ void bcc_sums64(const Rep& R, const Planes& oth, uint
 // summed in FP64 per wavefront into wave_part[wave][4] = (E, Mx, My, Mz);
 // with kExact the per-site energies are FP64 (pair_energy_exact), the exact
 // energies the replica exchange takes.
+// The learned measuring variant is held to 96 registers (five wavefronts per
+// SIMD, as the plain learned variant): left alone it takes 98, which the
+// 8-register allocation rounds to 104, four wavefronts and a 4% slower launch;
+// the FP64 (kExact) one would spill at 96.
 template <bool kMeasure, bool kLearned, bool kExact = false, bool kThree = false>
-__global__ __launch_bounds__(kBlock) void k_half_sweep(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kLearned && kMeasure && !kExact ? 5 : 1)))
+void k_half_sweep(
     SpinBuf sb, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ block_rep,
     const uint32_t* __restrict__ sweep_ctr, uint32_t sweep_off, int colour,
