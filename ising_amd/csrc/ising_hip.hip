@@ -3730,6 +3730,35 @@ int pt_run_impl(ising_ctx* c, const ising_schedule* sched, const ising_series* o
     desc.zrec = (uint32_t)Z;
     SeriesDev sd;
     sd.desc = d_desc;
+    // Requested series of up to kStageMax bytes go to a pinned staging block
+    // of the context by copies queued behind the last period, so they cost
+    // one synchronisation and a host copy instead of a blocking pageable copy
+    // each; larger ones, whose time is bandwidth, and the allocation-free
+    // entry point copy directly after the run. The block is (re)allocated
+    // before any work is queued, so a failure leaves the context unchanged.
+    struct Part { void* dst; const void* src; size_t bytes; };
+    Part parts[4] = {};
+    int n_parts = 0;
+    if (out) {
+        if (out->energy && n_meas) parts[n_parts++] = { out->energy, desc.e, n_meas * R * sizeof(double) };
+        if (out->magnetisation && n_meas) parts[n_parts++] = { out->magnetisation, desc.m, n_meas * R * 3 * sizeof(double) };
+        if (out->winding && n_q) parts[n_parts++] = { out->winding, desc.qi, n_q * R * Z * sizeof(int64_t) };
+        if (out->winding_raw && n_q) parts[n_parts++] = { out->winding_raw, desc.qr, n_q * R * Z * sizeof(double) };
+    }
+    size_t stage_bytes = 0;
+    for (int k = 0; k < n_parts; ++k) stage_bytes += (parts[k].bytes + 255u) & ~(size_t)255u;
+    constexpr size_t kStageMax = (size_t)64 << 20;
+    const bool staged = !staging && n_parts > 0 && stage_bytes <= kStageMax;
+    if (staged && stage_bytes > c->h_stage_cap) {
+        if (c->h_stage) (void)hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        c->h_stage_cap = 0;
+        if (hipHostMalloc((void**)&c->h_stage, stage_bytes, hipHostMallocDefault) != hipSuccess) {
+            c->h_stage = nullptr;
+            return set_err(c, ISING_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu bytes) for the series copy-out", stage_bytes);
+        }
+        c->h_stage_cap = stage_bytes;
+    }
     // The descriptor and the record counters of this run (plain kernels: no
     // staging copy, no memset).
     hipLaunchKernelGGL(k_set_series_desc, dim3(1), dim3(64), 0, c->stream, d_desc, desc);
@@ -3776,44 +3805,18 @@ int pt_run_impl(ising_ctx* c, const ising_schedule* sched, const ising_series* o
         if ((rc = enqueue_period(c, s, cnt, sd))) return rc;
         done += cnt;
     }
-    // Requested series of up to kStageMax bytes go to a pinned staging block
-    // of the context by copies queued behind the last period, so they cost
-    // one synchronisation and a host copy instead of a blocking pageable copy
-    // each; larger ones, whose time is bandwidth, and the allocation-free
-    // entry point copy directly after the run.
-    struct Part { void* dst; const void* src; size_t bytes; };
-    Part parts[4] = {};
-    int n_parts = 0;
-    if (out) {
-        if (out->energy && n_meas) parts[n_parts++] = { out->energy, desc.e, n_meas * R * sizeof(double) };
-        if (out->magnetisation && n_meas) parts[n_parts++] = { out->magnetisation, desc.m, n_meas * R * 3 * sizeof(double) };
-        if (out->winding && n_q) parts[n_parts++] = { out->winding, desc.qi, n_q * R * Z * sizeof(int64_t) };
-        if (out->winding_raw && n_q) parts[n_parts++] = { out->winding_raw, desc.qr, n_q * R * Z * sizeof(double) };
-    }
-    size_t stage_bytes = 0;
-    for (int k = 0; k < n_parts; ++k) stage_bytes += (parts[k].bytes + 255u) & ~(size_t)255u;
-    constexpr size_t kStageMax = (size_t)64 << 20;
-    const bool staged = !staging && n_parts > 0 && stage_bytes <= kStageMax;
-    if (staged && stage_bytes > c->h_stage_cap) {
-        if (c->h_stage) (void)hipHostFree(c->h_stage);
-        c->h_stage = nullptr;
-        c->h_stage_cap = 0;
-        if (hipHostMalloc((void**)&c->h_stage, stage_bytes, hipHostMallocDefault) != hipSuccess) {
-            c->h_stage = nullptr;
-            return set_err(c, ISING_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu bytes) for the series copy-out", stage_bytes);
-        }
-        c->h_stage_cap = stage_bytes;
-    }
+    hipError_t ecopy = hipSuccess;
     if (staged) {
         size_t off = 0;
-        for (int k = 0; k < n_parts; ++k) {
-            HIP_TRY(c, hipMemcpyAsync(c->h_stage + off, parts[k].src, parts[k].bytes, hipMemcpyDeviceToHost, c->stream));
+        for (int k = 0; k < n_parts && ecopy == hipSuccess; ++k) {
+            ecopy = hipMemcpyAsync(c->h_stage + off, parts[k].src, parts[k].bytes, hipMemcpyDeviceToHost, c->stream);
             off += (parts[k].bytes + 255u) & ~(size_t)255u;
         }
     }
     hipError_t es = hipStreamSynchronize(c->stream);
     if (es != hipSuccess) return set_err(c, ISING_ERR_HIP, "pt_run: %s", hipGetErrorString(es));
     advance_host(c, (uint32_t)s.n_sweeps);
+    if (ecopy != hipSuccess) return set_err(c, ISING_ERR_HIP, "series copy-out: %s", hipGetErrorString(ecopy));
     if (staged) {
         size_t off = 0;
         for (int k = 0; k < n_parts; ++k) {
