@@ -327,10 +327,10 @@ int  ising_bandwidth_probe(ising_ctx* ctx, int64_t bytes, double* ms);
 /* One fused launch per sweep instead of two half-sweep launches: each
  * workgroup stages whole z planes in LDS and updates both colours of lc
  * consecutive planes (1..255); simple-cubic contexts only;
- * ISING_ERR_UNSUPPORTED when a plane of the largest replica does not fit in
- * LDS or a replica is not simple cubic. While a ladder takes its cone radius
- * from the network (ising_mlp_enable) the sweeps run as half sweeps under
- * the same flags. */
+ * ISING_ERR_UNSUPPORTED when a plane of the largest replica has more than
+ * 2048 sites of one colour (one workgroup) or a replica is not simple
+ * cubic. While a ladder takes its cone radius from the network
+ * (ising_mlp_enable) the sweeps run as half sweeps under the same flags. */
 #define ISING_TUNE_FUSED_SHIFT 8
 #define ISING_TUNE_FUSED(lc) ((uint32_t)(lc) << ISING_TUNE_FUSED_SHIFT)
 #define ISING_TUNE_MAX_CANDIDATES 16

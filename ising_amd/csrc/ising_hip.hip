@@ -2727,7 +2727,7 @@ struct ising_ctx {
     uint32_t host_sweep = 0;
     uint32_t host_fused = 0;     // fused sweeps enqueued: parity selects the current spin buffer
     uint32_t tune = 0;           // sweep tuning flags (kTune*, fused chunk in bits 8..15)
-    // fused sweep (k_sweep_fused): available when every plane fits in LDS
+    // fused sweep (k_sweep_fused): available when every plane fits one workgroup and LDS
     bool fused_ok = false;
     int fused_threads = 0, fused_wpb = 0, n_fblocks = 0, max_fblocks = 0;
     uint32_t fused_pstride = 0;
@@ -4764,7 +4764,8 @@ int ising_set_tuning(ising_ctx* c, uint32_t flags)
     if (lc > 0 && !c->fused_ok)
         return set_err(c, ISING_ERR_UNSUPPORTED,
                        "fused sweep unavailable: it needs simple-cubic replicas whose z plane of one colour "
-                       "(%u sites here) fits in LDS", c->fused_pstride - 1u);
+                       "(%u sites here) fits one workgroup (%d sites) and LDS", c->fused_pstride - 1u,
+                       kFusedMaxThreads * kFusedPasses);
     HIP_TRY(c, hipSetDevice(c->device));
     if (lc != fused_lc(c)) {
         int rc = fused_layout(c, lc);
@@ -4783,7 +4784,7 @@ int ising_get_tuning(const ising_ctx* c, uint32_t* flags)
 
 // Flag sets the autotuner times: the half-sweep pair with and without
 // die-contiguous blocks and non-temporal own-colour access, then (when the
-// planes fit in LDS) the fused sweep with 8, 16, 32 or all z planes per
+// planes fit one workgroup) the fused sweep with 8, 16, 32 or all z planes per
 // workgroup, with and without die-contiguous blocks.
 static std::vector<uint32_t> tuning_candidates(const ising_ctx* c)
 {

@@ -170,9 +170,15 @@ def test_accept_counts_survive_layout_changes():
 def test_tuning_flags_are_checked():
     with engine(Lattice(128, 128, 4, (True, True, False)), HAM, [1.0]) as e:
         with pytest.raises(IsingError) as ei:
-            e.set_tuning(tune_fused(2))        # a 128x64 plane per colour does not fit in LDS
+            e.set_tuning(tune_fused(2))        # 8192 sites of one colour per plane: more than a workgroup holds
         assert "UNSUPPORTED" in str(ei.value)
         assert all((f >> 8) == 0 for f in e.tuning_candidates())
+    # the limit: 2048 sites of one colour per plane (1024 threads x 2 column pairs)
+    with engine(Lattice(64, 66, 3, (True, True, False)), HAM, [1.0]) as e:
+        with pytest.raises(IsingError):
+            e.set_tuning(tune_fused(2))
+    with engine(Lattice(64, 64, 3, (True, True, False)), HAM, [1.0]) as e:
+        e.set_tuning(tune_fused(2))
     with engine(Lattice(8, 8, 8), HAM, [1.0]) as e:
         with pytest.raises(IsingError):
             e.set_tuning(1 << 20)
