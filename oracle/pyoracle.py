@@ -1,7 +1,8 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/libising_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg. The product never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check its result and by bench.py's cpu_baseline leg. The product never
+imports it.
 """
 from __future__ import annotations
 

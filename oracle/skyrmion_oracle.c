@@ -1,7 +1,7 @@
 /*
  * skyrmion_oracle.c -- FP64 CPU oracle of the skyrmion count of one layer
  * (the statistic chainMesh_statistics reports as skyrmion_number_middle).
- * TEST INFRASTRUCTURE ONLY: tests/ use it as the checker; the product's
+ * TEST INFRASTRUCTURE ONLY: tests/ compare the device against it; the product's
  * kernels live in ising_amd/csrc/ising_hip.hip.
  *
  * Reference: compute_skyrmion_distribution (src/reciprocalSpaceProcesses.f90:
