@@ -2,8 +2,7 @@
  * ising_oracle.h -- FP64 CPU restatement of the reference's Metropolis /
  * parallel-tempering hot path (SamuelJosephs/ISING), used ONLY as test
  * infrastructure: by tests/, by __graft_entry__.smoke() to check its result,
- * and
- * by bench.py's cpu_baseline leg. The product (libising_hip.so and the
+ * and by bench.py's cpu_baseline leg. The product (libising_hip.so and the
  * ising_amd package) never links or calls it.
  *
  * Two families of functions:
