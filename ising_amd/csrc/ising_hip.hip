@@ -584,19 +584,12 @@ __device__ __forceinline__ f3 metropolis_site(const Lad& L, float beta, const f3
     const float rr = __builtin_amdgcn_exp2f(fmaf(__builtin_amdgcn_logf((float)(rnd.z >> 8)), 1.f / 3.f, lr8));
     const f3 t{ s.x + rr * st * cp, s.y + rr * st * sp, s.z + rr * ct };
     const float n2 = t.x * t.x + t.y * t.y + t.z * t.z;
-    accepted = false;
-    f3 fs = s;
-    if (active && n2 > 1e-30f) {
-        const float inv = __builtin_amdgcn_rsqf(n2);
-        const f3 sn{ t.x * inv, t.y * inv, t.z * inv };
-        const float de = delta_e(s, sn, hf, L.Kz);
-        const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
-        if (uacc < w) {
-            accepted = true;
-            fs = sn;
-        }
-    }
-    return fs;
+    const float inv = __builtin_amdgcn_rsqf(n2);
+    const f3 sn{ t.x * inv, t.y * inv, t.z * inv };
+    const float de = delta_e(s, sn, hf, L.Kz);
+    const float w = __builtin_amdgcn_exp2f(-beta * de * 1.4426950408889634f);
+    accepted = active && n2 > 1e-30f && uacc < w;
+    return accepted ? sn : s;
 }
 
 // Demagnetising field of the site at colour-major index t (colour*half + q),
@@ -894,7 +887,11 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
 
     // Per owned column pair: its index, row and the LDS indices of its
     // in-plane neighbours (the +x one of an odd site and the -x one of an even
-    // site, +-y; the zero slot across an open face).
+    // site, +-y; the zero slot across an open face). A lane past the end of
+    // the plane takes the last column pair: it reads, updates and writes that
+    // pair exactly as its owner does (same inputs, same counters), so its
+    // writes repeat the owner's values and the kernel needs no per-lane
+    // branches; only the acceptance counts and the sums skip it.
     uint32_t jj[kFusedPasses], iyv[kFusedPasses], xnp[kFusedPasses], xnm[kFusedPasses], ynp[kFusedPasses],
         ynm[kFusedPasses], sb0[kFusedPasses], sb1[kFusedPasses];
     bool act[kFusedPasses];
@@ -917,8 +914,7 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
         sb0[m] = small ? __umul24(__umul24(2u * h, ly) + iy, (uint32_t)lz) : ((2u * h) * ly + iy) * (uint32_t)lz;
         sb1[m] = small ? __umul24(__umul24(2u * h + 1u, ly) + iy, (uint32_t)lz) : ((2u * h + 1u) * ly + iy) * (uint32_t)lz;
     }
-    // Colour `colour` of plane z in the source buffer. Every lane loads (an
-    // idle lane's index is clamped into the plane); a plane beyond an open
+    // Colour `colour` of plane z in the source buffer. A plane beyond an open
     // face, or one the chunk never reads (colour 1 above z1 + 1, colour 0
     // above z1), is read through an offset past the descriptor's range, which
     // returns zeros without a branch or a memory access.
@@ -965,7 +961,7 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
         const float4* l0p = lplane(0, q ^ 1);   // colour 0 of plane s-1, after its update
 #pragma unroll
         for (int m = 0; m < kFusedPasses; ++m)
-            if (act[m]) put(l1, m, a0[m]);
+            put(l1, m, a0[m]);
         __syncthreads();
         // the planes step s+1 adds to the windows
 #pragma unroll
@@ -995,11 +991,11 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                 const f3 hf = field_from_sums(L, sum, dx, dy, dz);
                 const f3 nbr[6] = {};      // (the learned proposal's input; never fused)
                 fs = metropolis_site<false>(L, beta, b0[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zs,
-                                            sweep, seed_lo, seed_hi, (uint32_t)slot, nullptr, act[m], accepted);
-                if (act[m] && owned0) bst3(rd, jj[m] * 4u, (uint32_t)zs * sP, sN, fs);
+                                            sweep, seed_lo, seed_hi, (uint32_t)slot, nullptr, true, accepted);
+                if (owned0) bst3(rd, jj[m] * 4u, (uint32_t)zs * sP, sN, fs);
             }
-            if (owned0) n_acc += (uint32_t)__popcll(__ballot(accepted));
-            if (act[m]) put(l0, m, fs);
+            if (owned0) n_acc += (uint32_t)__popcll(__ballot(accepted && act[m]));
+            put(l0, m, fs);
             b0[m] = fs;                 // colour 0 of plane s, after its update
         }
         // colour 1 on plane s-1 (owned planes only; l0p was filled in step s-1)
@@ -1021,9 +1017,9 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                 const f3 nbr[6] = {};
                 bool accepted;
                 const f3 fs = metropolis_site<false>(L, beta, am1[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zb,
-                                                     sweep, seed_lo, seed_hi, (uint32_t)slot, nullptr, act[m], accepted);
+                                                     sweep, seed_lo, seed_hi, (uint32_t)slot, nullptr, true, accepted);
+                bst3(rd, jj[m] * 4u, sH + (uint32_t)zb * sP, sN, fs);
                 if (act[m]) {
-                    bst3(rd, jj[m] * 4u, sH + (uint32_t)zb * sP, sN, fs);
                     if constexpr (kMeasure) {
                         const f3 c0 = bm1[m];
                         if constexpr (kExact) {
@@ -1040,7 +1036,7 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                         }
                     }
                 }
-                n_acc += (uint32_t)__popcll(__ballot(accepted));
+                n_acc += (uint32_t)__popcll(__ballot(accepted && act[m]));
             }
         }
     };
