@@ -824,12 +824,19 @@ constexpr int kBufferFormat = 0x00020000; // buffer descriptor word 3: 32-bit da
 // return zero and read no memory (every fused-sweep descriptor spans less
 // than 2^31 bytes).
 constexpr uint32_t kBufferOob = 0x80000000u;
+// Cache policy of the source reads: non-temporal (nt). Each source plane is
+// read once per sweep (twice at a chunk edge) while the sweep writes as many
+// bytes to the other buffer; streaming the reads made the fused sweep 8%
+// faster at 64^3 x 64 than the default policy (non-temporal writes did not
+// help).
+constexpr int kSweepLoadPolicy = 2;
 
 __device__ __forceinline__ f3 bld3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t s0, uint32_t sN)
 {
-    return f3{ __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)s0, 0)),
-               __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)(s0 + sN), 0)),
-               __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)(s0 + 2u * sN), 0)) };
+    return f3{ __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)s0, kSweepLoadPolicy)),
+               __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)(s0 + sN), kSweepLoadPolicy)),
+               __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)(s0 + 2u * sN),
+                                                                     kSweepLoadPolicy)) };
 }
 __device__ __forceinline__ void bst3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t s0, uint32_t sN, const f3& v)
 {
