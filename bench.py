@@ -59,7 +59,16 @@ inverse y and x transforms, gather), timed with a hipEvent pair
 (ising_time_demag), against the bytes its seven passes read and write once
 each.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|bcc|hk|demag]
+`--workload mix` shows the replica exchange mixing at the headline lattice
+size: the same 64^3 x 64 schedule on a ladder narrow enough for exchanges to
+be accepted (T 0.90..0.935 geometric; on the 0.2..2.0 ladder neighbouring
+temperatures are ~20 energy standard deviations apart and no exchange is
+accepted). 2000 equilibration sweeps, 2000 timed sweeps with the headline's
+records, then 2000 swap intervals with the temperature map read after each:
+acceptance per neighbouring pair and the replicas' round trips between the
+coldest and the hottest temperature.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c3|bcc|hk|demag|mix]
 """
 from __future__ import annotations
 
@@ -154,9 +163,10 @@ def main() -> int:
                     help="with --profile-sweeps: use these launch-tuning flags instead of autotuning")
     ap.add_argument("--eager", action="store_true",
                     help="launch every period eagerly instead of replaying a captured hipGraph (same kernels)")
-    ap.add_argument("--workload", choices=("c5", "c3", "bcc", "hk", "demag"), default="c5",
+    ap.add_argument("--workload", choices=("c5", "c3", "bcc", "hk", "demag", "mix"), default="c5",
                     help="c5: 64^3 x 64 PT, learned vs fixed (default); c3: 256 replicas of mixed lattices; "
-                         "bcc / hk: 64-replica PT on body-centred-cubic / hyper-kagome cells")
+                         "bcc / hk: 64-replica PT on body-centred-cubic / hyper-kagome cells; "
+                         "mix: replica-exchange mixing on a narrow 64^3 x 64 ladder")
     ap.add_argument("--profile-sweeps", type=int, default=0,
                     help="only run N plain sweeps, N learned sweeps (untrained network) and one 256 MiB "
                          "copy probe (for rocprofv3 counter runs)")
@@ -172,6 +182,8 @@ def main() -> int:
 
     if args.workload == "c3":
         return bench_c3(args)
+    if args.workload == "mix":
+        return bench_mix(args)
     if args.workload in ("bcc", "hk"):
         return bench_cell(args, args.workload)
     if args.workload == "demag":
@@ -432,6 +444,93 @@ def bench_c3(args) -> int:
     }
     print(json.dumps(line), flush=True)
     eng.close()
+    return 0
+
+
+MIX_T = (0.90, 0.935)
+MIX_EQUIL_SWEEPS = 2000
+MIX_TIMED_SWEEPS = 2000
+MIX_ROUNDS = 2000
+
+
+def round_trips(slot_of_temp: np.ndarray) -> np.ndarray:
+    """Per replica slot, the completed round trips coldest -> hottest ->
+    coldest in a sequence of temperature maps ([n, R], slot at each position):
+    a trip ends whenever a slot reaches the end opposite to the last end it
+    visited, and two trips make a round trip."""
+    n, r = slot_of_temp.shape
+    last = np.full(r, -1)
+    trips = np.zeros(r, dtype=np.int64)
+    for m in slot_of_temp:
+        for end, slot in ((0, m[0]), (1, m[r - 1])):
+            if last[slot] != end:
+                if last[slot] >= 0:
+                    trips[slot] += 1
+                last[slot] = end
+    return trips // 2
+
+
+def bench_mix(args) -> int:
+    """The headline schedule on a narrow ladder: swap acceptance per pair and
+    round trips of the replicas through the ladder."""
+    from ising_amd import Engine, Hamiltonian, Ladder, Lattice, geometric_temperatures
+    T = geometric_temperatures(MIX_T[0], MIX_T[1], N_REP)
+    lat = Lattice(L, L, L)
+    eng = Engine([Ladder(lat, Hamiltonian(J=1.0, D=0.5, dmi="bulk", Bz=0.2), list(1.0 / T), CONE)], seed=SEED)
+    eng.init_random()
+    tune_flags, _ = eng.autotune(10)
+    progress(f"mix autotune: flags {tune_flags}")
+    eng.pt_run(MIX_EQUIL_SWEEPS, SWAP_EVERY, 0, 0, use_graph=not args.eager, want=())
+    att0, acc0 = eng.swap_stats()
+    eng.synchronize()
+    t0 = time.perf_counter()
+    out = eng.pt_run(MIX_TIMED_SWEEPS, SWAP_EVERY, MEASURE_EVERY, WINDING_EVERY, use_graph=not args.eager,
+                     winding_layer=L // 2, want=("energy",))
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    progress(f"mix timed: {wall * 1e3:.1f} ms for {MIX_TIMED_SWEEPS} sweeps")
+    maps = [eng.temperature_map()]
+    for _ in range(MIX_ROUNDS):
+        eng.pt_run(SWAP_EVERY, SWAP_EVERY, 0, 0, use_graph=not args.eager, want=())
+        maps.append(eng.temperature_map())
+    att, acc = eng.swap_stats()
+    eng.close()
+    att, acc = att - att0, acc - acc0
+    rate = acc[:-1] / np.maximum(att[:-1], 1)
+    trips = round_trips(np.array(maps))
+    moved = np.array(maps)
+    n_sites = lat.sites * N_REP
+    line = {
+        "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
+        "value": n_sites * MIX_TIMED_SWEEPS / wall,
+        "unit": "site updates/s",
+        "n_gpus": 1,
+        "steps": MIX_TIMED_SWEEPS,
+        "warmup": MIX_EQUIL_SWEEPS,
+        "ms_per_step": wall * 1e3 / MIX_TIMED_SWEEPS,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 spins, fp64 reductions",
+        "data": "synthetic (seeded random-sphere start)",
+        "config": {"workload": f"PT mixing: 64^3 x 64 replicas, T {MIX_T[0]}-{MIX_T[1]} geometric, J=1 D=0.5 (bulk) "
+                               f"Bz=0.2, cone 0.5, swap every {SWAP_EVERY}, E/M and middle-layer Q every 2 in the "
+                               f"timed run; then {MIX_ROUNDS} swap intervals with the temperature map read after each",
+                   "step": "one sweep of every replica"},
+        "swap_acceptance": {"min": float(rate.min()), "mean": float(rate.mean()), "max": float(rate.max()),
+                            "per_pair": [round(float(x), 4) for x in rate],
+                            "attempts_per_pair": int(att[:-1].min()),
+                            "rounds": "timed run and mixing run"},
+        "round_trips": {"total": int(trips.sum()), "replicas_with_one": int((trips > 0).sum()),
+                        "swap_rounds": MIX_ROUNDS,
+                        "replicas_reaching_both_ends": int(sum(
+                            1 for s in range(N_REP)
+                            if (moved[:, 0] == s).any() and (moved[:, N_REP - 1] == s).any()))},
+        "energy_per_site": (out["energy"][-1][::16] / lat.sites).tolist(),
+        "tuning_flags": tune_flags,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
     return 0
 
 
