@@ -414,7 +414,7 @@ def bench_c3(args) -> int:
     bytes_launch = BYTES_PER_UPDATE * n_sites / 2
     achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
     line = {
-        "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
+        "metric": "site updates/s, batched uneven PT (256 replicas on 24^3/32^3/48^3)",
         "value": n_sites * C3_SWEEPS / wall,
         "unit": "site updates/s",
         "n_gpus": 1,
@@ -501,7 +501,7 @@ def bench_mix(args) -> int:
     moved = np.array(maps)
     n_sites = lat.sites * N_REP
     line = {
-        "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
+        "metric": "site updates/s and replica-exchange acceptance, 64^3x64 PT on a narrow ladder",
         "value": n_sites * MIX_TIMED_SWEEPS / wall,
         "unit": "site updates/s",
         "n_gpus": 1,
@@ -600,7 +600,7 @@ def bench_cell(args, name: str) -> int:
     traffic = counters_from_profiles().get("cells", {}).get(name, {}).get("hbm_bytes_per_launch")
     cpu = None if args.no_cpu else cpu_baseline_cell(cell, {"bcc": 24, "hk": 12}[name])
     line = {
-        "metric": "site updates/s; tau_int-corrected samples/s at 64^3x64 PT, learned vs fixed",
+        "metric": f"site updates/s, 64-replica PT on {desc}",
         "value": n_sites * CELL_SWEEPS / wall,
         "unit": "site updates/s",
         "n_gpus": 1,
