@@ -805,9 +805,10 @@ void k_half_sweep(
 //   * the planes the next step needs are loaded one step ahead, through buffer
 //     descriptors whose scalar offsets select colour, component and plane (the
 //     per-thread vector offset never changes).
-// The plain variant renames its register windows (the z loop is unrolled by
-// four); the measuring ones copy them after each step, which keeps them
-// within the 128 registers that 16 wavefronts per CU allow.
+// The plain and E/M variants rename their register windows (the z loop is
+// unrolled by four); the FP64 swap-energy variant copies them after each
+// step, which keeps it within the 128 registers that 16 wavefronts per CU
+// allow.
 // Colour-0 planes z0-1 and z1 are halo: recomputed from the same Philox
 // counters as by their owners and not written. Reads come from the source
 // buffer and every owned site is written to the other one, so neighbouring
@@ -940,11 +941,11 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     // Register windows: colour 1 before the sweep on planes s-1, s, s+1 and
     // the plane s+2 loaded during step s; colour 0 after its update on planes
     // s-2, s-1, before it (then after it) on plane s, and plane s+1 loaded
-    // during step s. Four slots each. The plain sweep rotates them by
+    // during step s. Four slots each. The plain and E/M sweeps rotate them by
     // renaming (a loop body of four steps whose window arguments shift by one
-    // slot, so no step copies a window); the measuring variants copy them
-    // after every step, which keeps their register use under the 128 that 16
-    // wavefronts per CU allow (renamed, they spill).
+    // slot, so no step copies a window: 109 and 117 registers); the FP64
+    // swap-energy variant copies them after every step, which keeps it under
+    // the 128 registers that 16 wavefronts per CU allow (renamed, it spills).
     f3 A[4][kFusedPasses], B[4][kFusedPasses];
 #pragma unroll
     for (int m = 0; m < kFusedPasses; ++m) {
@@ -1047,7 +1048,7 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
             }
         }
     };
-    if constexpr (!kMeasure) {
+    if constexpr (!kExact) {
         for (int s = z0 - 1; s <= z1; s += 4) {
             step(s, A[0], A[1], A[2], A[3], B[0], B[1], B[2], B[3]);
             if (s + 1 > z1) break;
