@@ -325,14 +325,20 @@ int  ising_bandwidth_probe(ising_ctx* ctx, int64_t bytes, double* ms);
 #define ISING_TUNE_XCD_REMAP 1u  /* contiguous block ranges per accelerator die */
 #define ISING_TUNE_NT_OWN    2u  /* non-temporal access to the updated colour (half sweeps) */
 /* One fused launch per sweep instead of two half-sweep launches: each
- * workgroup stages whole z planes in LDS and updates both colours of lc
- * consecutive planes (1..255); simple-cubic contexts only;
- * ISING_ERR_UNSUPPORTED when a plane of the largest replica has more than
- * 2048 sites of one colour (one workgroup) or a replica is not simple
- * cubic. While a ladder takes its cone radius from the network
- * (ising_mlp_enable) the sweeps run as half sweeps under the same flags. */
+ * workgroup stages z planes in LDS and updates both colours of lc
+ * consecutive planes (1..255); simple-cubic contexts only. A plane of up to
+ * 2048 sites of one colour is one workgroup's; larger planes, or any plane
+ * with ISING_TUNE_SLAB(hy) (2..255 rows, with a fused lc), are cut into y
+ * slabs of hy owned rows per workgroup (hy = 0: the tallest that fits), each
+ * recomputing two halo rows on either side. ISING_ERR_UNSUPPORTED when a
+ * replica is not simple cubic or a row has more than 341 sites of one colour
+ * (no window of six rows fits). While a ladder takes its cone radius from
+ * the network (ising_mlp_enable) the sweeps run as half sweeps under the
+ * same flags. */
 #define ISING_TUNE_FUSED_SHIFT 8
 #define ISING_TUNE_FUSED(lc) ((uint32_t)(lc) << ISING_TUNE_FUSED_SHIFT)
+#define ISING_TUNE_SLAB_SHIFT 16
+#define ISING_TUNE_SLAB(hy) ((uint32_t)(hy) << ISING_TUNE_SLAB_SHIFT)
 #define ISING_TUNE_MAX_CANDIDATES 16
 int  ising_set_tuning(ising_ctx* ctx, uint32_t flags);
 int  ising_get_tuning(const ising_ctx* ctx, uint32_t* flags);

@@ -30,6 +30,7 @@ MLP_RECORD = 32
 TUNE_XCD_REMAP = 1
 TUNE_NT_OWN = 2
 TUNE_FUSED_SHIFT = 8
+TUNE_SLAB_SHIFT = 16
 TUNE_MAX_CANDIDATES = 16
 MU0_REF = 1.256637006008532e-06
 STATIC_SCRATCH_BYTES = 16 << 20
@@ -38,6 +39,11 @@ STATIC_SCRATCH_BYTES = 16 << 20
 def tune_fused(planes: int) -> int:
     """Tuning flag: one fused launch per sweep, `planes` z planes per workgroup."""
     return int(planes) << TUNE_FUSED_SHIFT
+
+
+def tune_slab(rows: int) -> int:
+    """Tuning flag (with tune_fused): y slabs of `rows` owned rows per fused workgroup."""
+    return int(rows) << TUNE_SLAB_SHIFT
 
 
 

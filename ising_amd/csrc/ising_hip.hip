@@ -47,6 +47,8 @@ constexpr uint32_t kTuneXcdRemap = 1u;     // give each die a contiguous range o
 constexpr uint32_t kTuneNtOwn = 2u;        // non-temporal loads/stores of the updated colour
 constexpr uint32_t kTuneFusedShift = 8;    // bits 8..15: z planes per fused-sweep workgroup (0: half sweeps)
 constexpr uint32_t kTuneFusedMask = 0xFFu << kTuneFusedShift;
+constexpr uint32_t kTuneSlabShift = 16;    // bits 16..23: rows per y slab of the fused sweep (0: whole planes where they fit)
+constexpr uint32_t kTuneSlabMask = 0xFFu << kTuneSlabShift;
 
 // Logical workgroup index: with kTuneXcdRemap the blocks a die receives
 // (b mod 8 == die) map to one contiguous range, so neighbouring rows of a
@@ -104,6 +106,9 @@ struct Rep {
     int32_t f_nblocks;      // fused-sweep workgroups (z chunks)
     int32_t plane;          // ly*hx: sites of one colour on one z plane
     int32_t ncol;           // colours of the sweep: 2, or 3 on hyper-kagome
+    int32_t f_rows;         // fused sweep: rows of a workgroup's window (ly: the whole plane)
+    int32_t f_hy;           // fused sweep: rows a y slab owns (ly: the whole plane)
+    int32_t f_nslab;        // fused sweep: y slabs per z chunk (1: whole planes)
     uint32_t half;          // sites of one colour: N/2, or N/3 on hyper-kagome
     int64_t off;            // float offset of the x component block
     int64_t n;              // N
@@ -820,6 +825,8 @@ void k_half_sweep(
 // half sweeps' bond order (the energies a replica exchange takes).
 constexpr int kFusedMaxThreads = 1024;
 constexpr int kFusedPasses = 2;          // column pairs per thread on a plane
+constexpr int kFusedMaxPairs = kFusedMaxThreads * kFusedPasses;   // column pairs of one workgroup's window
+constexpr int kSlabHalo = 4;             // window rows of a y slab beyond the rows it owns
 constexpr int kBufferFormat = 0x00020000; // buffer descriptor word 3: 32-bit data
 // A vector offset at or past a descriptor's byte range: loads through it
 // return zero and read no memory (every fused-sweep descriptor spans less
@@ -851,7 +858,7 @@ __device__ __forceinline__ f3 flip_sign(const f3& v, uint32_t mask)
                __uint_as_float(__float_as_uint(v.z) ^ mask) };
 }
 
-template <bool kMeasure, bool kExact = false>
+template <bool kMeasure, bool kExact = false, bool kSlab = false>
 __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     SpinBuf sb, const Rep* __restrict__ reps, const Lad* __restrict__ lads,
     const float* __restrict__ beta_slot, const int32_t* __restrict__ fblock_rep,
@@ -859,7 +866,7 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     uint32_t seed_lo, uint32_t seed_hi, unsigned long long* __restrict__ acc_count,
     double* __restrict__ wave_part, uint32_t tune)
 {
-    extern __shared__ float4 lds[];   // [colour 2][parity 2][pstride] of (x, y, z, unused); slot P of each is zero
+    extern __shared__ float4 lds[];   // [colour 2][parity 2][pstride] of (x, y, z, unused); slot PW of each is zero
     const uint32_t blk = logical_block(blockIdx.x, gridDim.x, tune);
     const int slot = fblock_rep[blk];
     const Rep R = reps[slot];
@@ -873,8 +880,27 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     const int lz = R.lz;
     const bool pz = (R.per >> 2) & 1;
     const bool px = R.per & 1, py = (R.per >> 1) & 1;
-    const int z0 = ((int)blk - R.f_first_block) * lc;
+    // workgroup bi of the slot: z chunk bi / nslab, y slab bi % nslab
+    const int bi = (int)blk - R.f_first_block;
+    const int nslab = kSlab ? R.f_nslab : 1;
+    const int zc = kSlab ? bi / nslab : bi;
+    const int z0 = zc * lc;
     const int z1 = min(z0 + lc, lz);
+    // The window: LDS rows of one plane, PW column pairs, slot PW zero. A
+    // slab owning rows [y0, y0 + yown) reads rows y0-2 .. y0+hy+1 (global row
+    // of window row r: y0 - 2 + r, periodic image or none beyond an open
+    // face), updates colour 0 on rows y0-1 .. y0+hy as well (the halo rows
+    // its colour-1 updates need, recomputed from the same counters as by
+    // their owners) and writes only its own rows.
+    const uint32_t PW = kSlab ? (uint32_t)R.f_rows * hx : P;
+    bool slab = false;
+    int Y0 = 0, yown = (int)ly;
+    if constexpr (kSlab) {
+        slab = nslab > 1;
+        const int y0 = (bi - zc * nslab) * R.f_hy;
+        yown = min(R.f_hy, (int)ly - y0);
+        Y0 = slab ? y0 - 2 : 0;
+    }
     const bool small = R.n < (1 << 24);
     // the replica's three component blocks in the source and destination
     // buffers; byte offsets: component c at c*sN, colour 1 at sH, plane z at z*sP
@@ -891,7 +917,7 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
         return z;
     };
     auto lplane = [&](int c, int q) -> float4* { return lds + (size_t)(c * 2 + q) * pstride; };
-    if (tid < 4u) lds[(size_t)tid * pstride + P] = float4{ 0.f, 0.f, 0.f, 0.f };     // the zero slots
+    if (tid < 4u) lds[(size_t)tid * pstride + PW] = float4{ 0.f, 0.f, 0.f, 0.f };    // the zero slots
 
     // Per owned column pair: its index, row and the LDS indices of its
     // in-plane neighbours (the +x one of an odd site and the -x one of an even
@@ -900,24 +926,51 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     // pair exactly as its owner does (same inputs, same counters), so its
     // writes repeat the owner's values and the kernel needs no per-lane
     // branches; only the acceptance counts and the sums skip it.
+    // kSlab: vo / so are the byte offsets of the pair in the spin planes for
+    // loads (past the range for rows beyond an open face, which read zeros)
+    // and stores (past the range for rows the slab does not own, whose
+    // stores are dropped); own: counted and summed; okr: a lattice row.
     uint32_t jj[kFusedPasses], iyv[kFusedPasses], xnp[kFusedPasses], xnm[kFusedPasses], ynp[kFusedPasses],
         ynm[kFusedPasses], sb0[kFusedPasses], sb1[kFusedPasses];
     bool act[kFusedPasses];
+    [[maybe_unused]] uint32_t vo[kFusedPasses], so[kFusedPasses];
+    [[maybe_unused]] bool own[kFusedPasses], okr[kFusedPasses];
 #pragma unroll
     for (int m = 0; m < kFusedPasses; ++m) {
         const uint32_t k = (uint32_t)m * T + tid;
-        act[m] = k < P;
-        const uint32_t j = act[m] ? k : P - 1u;
+        act[m] = k < PW;
+        const uint32_t j = act[m] ? k : PW - 1u;
         jj[m] = j;
-        const uint32_t iy = fdiv(j, R.div_hx);
-        const uint32_t h = fmodq(j, iy, R.div_hx);
+        const uint32_t r = fdiv(j, R.div_hx);       // window row (the lattice row on whole planes)
+        const uint32_t h = fmodq(j, r, R.div_hx);
+        uint32_t iy = r;
+        if constexpr (kSlab) {
+            vo[m] = j * 4u; so[m] = j * 4u; own[m] = act[m]; okr[m] = true;
+            if (slab) {
+                int gy = Y0 + (int)r;
+                bool ok = true;
+                if (gy < 0) { gy += (int)ly; ok = py; }
+                else if (gy >= (int)ly) { gy -= (int)ly; ok = py; }
+                iy = (uint32_t)gy;
+                const bool mine = (int)r >= 2 && (int)r < 2 + yown;
+                vo[m] = ok ? (iy * hx + h) * 4u : kBufferOob;
+                so[m] = mine ? (iy * hx + h) * 4u : kBufferOob;
+                own[m] = mine && act[m];
+                okr[m] = ok;
+            }
+        }
         iyv[m] = iy;
         const bool xpw = h == hx - 1u, xmw = h == 0u;
-        xnp[m] = xpw ? (px ? j + 1u - hx : P) : j + 1u;
-        xnm[m] = xmw ? (px ? j - 1u + hx : P) : j - 1u;
-        const bool ypw = iy + 1u == ly, ymw = iy == 0u;
-        ynp[m] = ypw ? (py ? j + hx - P : P) : j + hx;
-        ynm[m] = ymw ? (py ? j - hx + P : P) : j - hx;
+        xnp[m] = xpw ? (px ? j + 1u - hx : PW) : j + 1u;
+        xnm[m] = xmw ? (px ? j - 1u + hx : PW) : j - 1u;
+        if (slab) {                  // uniform; window rows 0 and W-1 have no neighbour beyond
+            ynp[m] = j + hx < PW ? j + hx : PW;
+            ynm[m] = r > 0u ? j - hx : PW;
+        } else {
+            const bool ypw = iy + 1u == ly, ymw = iy == 0u;
+            ynp[m] = ypw ? (py ? j + hx - P : P) : j + hx;
+            ynm[m] = ymw ? (py ? j - hx + P : P) : j - hx;
+        }
         // site number (ix*ly + iy)*lz of ix = 2h and 2h + 1, without the plane
         sb0[m] = small ? __umul24(__umul24(2u * h, ly) + iy, (uint32_t)lz) : ((2u * h) * ly + iy) * (uint32_t)lz;
         sb1[m] = small ? __umul24(__umul24(2u * h + 1u, ly) + iy, (uint32_t)lz) : ((2u * h + 1u) * ly + iy) * (uint32_t)lz;
@@ -928,9 +981,11 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
     // returns zeros without a branch or a memory access.
     auto load = [&](int colour, int z, int m) -> f3 {
         const int zz = z > z1 + colour ? -1 : wrapz(z);      // uniform
-        return bld3(rs, jj[m] * 4u + (zz >= 0 ? 0u : kBufferOob),
+        return bld3(rs, (kSlab ? vo[m] : jj[m] * 4u) | (zz >= 0 ? 0u : kBufferOob),
                     zz >= 0 ? (colour ? sH : 0u) + (uint32_t)zz * sP : 0u, sN);
     };
+    auto store_off = [&](int m) -> uint32_t { return kSlab ? so[m] : jj[m] * 4u; };
+    auto counted = [&](int m) -> bool { return kSlab ? own[m] : act[m]; };
     // one 16-byte LDS access per site: a single address per neighbour
     auto put = [&](float4* b, int m, const f3& v) { b[jj[m]] = float4{ v.x, v.y, v.z, 0.f }; };
     auto get = [&](const float4* b, uint32_t i) -> f3 {
@@ -1000,10 +1055,13 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                 const f3 nbr[6] = {};      // (the learned proposal's input; never fused)
                 fs = metropolis_site<false>(L, beta, b0[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zs,
                                             sweep, seed_lo, seed_hi, (uint32_t)slot, nullptr, true, accepted);
-                if (owned0) bst3(rd, jj[m] * 4u, (uint32_t)zs * sP, sN, fs);
+                if (owned0) bst3(rd, store_off(m), (uint32_t)zs * sP, sN, fs);
             }
-            if (owned0) n_acc += (uint32_t)__popcll(__ballot(accepted && act[m]));
-            put(l0, m, fs);
+            if (owned0) n_acc += (uint32_t)__popcll(__ballot(accepted && counted(m)));
+            if constexpr (kSlab)        // a row beyond an open face stays zero as a neighbour
+                put(l0, m, okr[m] ? fs : f3{ 0.f, 0.f, 0.f });
+            else
+                put(l0, m, fs);
             b0[m] = fs;                 // colour 0 of plane s, after its update
         }
         // colour 1 on plane s-1 (owned planes only; l0p was filled in step s-1)
@@ -1026,8 +1084,8 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                 bool accepted;
                 const f3 fs = metropolis_site<false>(L, beta, am1[m], nbr, hf, (p ? sb1[m] : sb0[m]) + (uint32_t)zb,
                                                      sweep, seed_lo, seed_hi, (uint32_t)slot, nullptr, true, accepted);
-                bst3(rd, jj[m] * 4u, sH + (uint32_t)zb * sP, sN, fs);
-                if (act[m]) {
+                bst3(rd, store_off(m), sH + (uint32_t)zb * sP, sN, fs);
+                if (counted(m)) {
                     if constexpr (kMeasure) {
                         const f3 c0 = bm1[m];
                         if constexpr (kExact) {
@@ -1044,7 +1102,7 @@ __global__ __launch_bounds__(kFusedMaxThreads) void k_sweep_fused(
                         }
                     }
                 }
-                n_acc += (uint32_t)__popcll(__ballot(accepted && act[m]));
+                n_acc += (uint32_t)__popcll(__ballot(accepted && counted(m)));
             }
         }
     };
@@ -2733,7 +2791,9 @@ struct ising_ctx {
     uint32_t tune = 0;           // sweep tuning flags (kTune*, fused chunk in bits 8..15)
     // fused sweep (k_sweep_fused): available when every plane fits one workgroup and LDS
     bool fused_ok = false;
-    int fused_threads = 0, fused_wpb = 0, n_fblocks = 0, max_fblocks = 0;
+    bool fused_slab = false;     // some replica's planes are cut into y slabs (the kSlab kernels)
+    int fused_threads = 0, fused_wpb = 0, n_fblocks = 0;
+    size_t fblock_cap = 0, fwave_cap = 0;    // capacities of the block table and per-wavefront arrays
     uint32_t fused_pstride = 0;
     size_t fused_lds = 0;
     std::vector<int32_t> fblock_rep;
@@ -2796,10 +2856,10 @@ struct ising_ctx {
     unsigned long long* d_acc = nullptr;       // [R] per-slot totals (filled on request)
     unsigned long long* d_acc_wave = nullptr;  // [n_blocks * waves per block]
     double* d_wave_part = nullptr;             // [n_blocks * waves per block][4]
-    int32_t* d_fblock_rep = nullptr;           // [max_fblocks]
-    unsigned long long* d_acc_wave_f = nullptr;  // [max_fblocks * fused_wpb]
+    int32_t* d_fblock_rep = nullptr;           // [fblock_cap]
+    unsigned long long* d_acc_wave_f = nullptr;  // [fwave_cap]: one counter per fused wavefront
     unsigned long long* d_acc_base = nullptr;    // [R] counters of earlier fused layouts
-    double* d_wave_part_f = nullptr;             // [max_fblocks * fused_wpb][4]
+    double* d_wave_part_f = nullptr;             // [fwave_cap][4]
     unsigned long long* d_swap_att = nullptr;
     unsigned long long* d_swap_acc = nullptr;
     int32_t* d_sites = nullptr; float* d_props = nullptr; double* d_de = nullptr; int de_cap = 0;
@@ -3083,6 +3143,7 @@ float* cur_spins(const ising_ctx* c) { return c->d_spins + (int64_t)(c->host_fus
 
 // z planes per fused-sweep workgroup (0: two half-sweep launches per sweep).
 int fused_lc(const ising_ctx* c) { return (int)((c->tune >> 8) & 0xFFu); }
+int fused_hy(const ising_ctx* c) { return (int)((c->tune & kTuneSlabMask) >> kTuneSlabShift); }
 
 // Sweeps run as fused launches (which swap the spin buffers): the fused flag
 // is set, no ladder has the demagnetisation term, which only the in-place
@@ -3113,21 +3174,29 @@ void launch_half_kernel(ising_ctx* c, uint32_t off, int colour, hipEvent_t t0 = 
                        (const double*)c->d_demag);
 }
 
-template <bool kMeasure, bool kExact = false>
-void launch_fused_kernel(ising_ctx* c, uint32_t off, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr)
+template <bool kMeasure, bool kExact, bool kSlab>
+void launch_fused_variant(ising_ctx* c, uint32_t off, hipEvent_t t0, hipEvent_t t1)
 {
     if (t0) {
-        hipExtLaunchKernelGGL((k_sweep_fused<kMeasure, kExact>), dim3(c->n_fblocks), dim3(c->fused_threads),
+        hipExtLaunchKernelGGL((k_sweep_fused<kMeasure, kExact, kSlab>), dim3(c->n_fblocks), dim3(c->fused_threads),
                               (uint32_t)c->fused_lds, c->stream, t0, t1, 0u, spin_buf(c), c->d_reps, c->d_lads,
                               (const float*)c->d_beta_slot, (const int32_t*)c->d_fblock_rep, (const uint32_t*)c->d_ctr,
                               off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
                               c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu);
         return;
     }
-    hipLaunchKernelGGL((k_sweep_fused<kMeasure, kExact>), dim3(c->n_fblocks), dim3(c->fused_threads), c->fused_lds,
+    hipLaunchKernelGGL((k_sweep_fused<kMeasure, kExact, kSlab>), dim3(c->n_fblocks), dim3(c->fused_threads), c->fused_lds,
                        c->stream, spin_buf(c), c->d_reps, c->d_lads, c->d_beta_slot, c->d_fblock_rep,
                        (const uint32_t*)c->d_ctr, off, fused_lc(c), c->fused_pstride, (uint32_t)c->seed, (uint32_t)(c->seed >> 32),
                        c->d_acc_wave_f, kMeasure ? c->d_wave_part_f : (double*)nullptr, c->tune & 0xFFu);
+}
+
+// The slab kernels only where some replica's planes are cut into y slabs.
+template <bool kMeasure, bool kExact = false>
+void launch_fused_kernel(ising_ctx* c, uint32_t off, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr)
+{
+    if (c->fused_slab) launch_fused_variant<kMeasure, kExact, true>(c, off, t0, t1);
+    else launch_fused_variant<kMeasure, kExact, false>(c, off, t0, t1);
 }
 
 // Sweep number `off` of a period (relative to the device sweep counter):
@@ -3533,72 +3602,130 @@ int enqueue_period(ising_ctx* c, const ising_schedule& s, int count, const Serie
 }
 
 // Fused-sweep resources: available when every replica is simple cubic and
-// the LDS planes of the largest (two colours x two parities, 64 bytes per
-// site of one colour) fit, with two column pairs per thread in one
-// workgroup. Counter and partial arrays are sized for one plane per
-// workgroup, the finest chunking.
+// each z plane either fits one workgroup (two column pairs per thread, 2048
+// sites of one colour) or can be cut into y slabs whose windows of at least
+// six rows do. The per-layout arrays (block table, per-wavefront counters and
+// sums) are sized by fused_layout.
 int fused_setup(ising_ctx* c)
 {
-    int pmax = 0, zsum = 0;
-    bool all_sc = true;
+    bool all_sc = true, fits = true;
+    int64_t nmax = 0;
     for (const Rep& r : c->reps) {
-        pmax = std::max(pmax, r.plane); zsum += r.lz;
         all_sc = all_sc && r.cell == ISING_CELL_SC;
+        fits = fits && (r.plane <= kFusedMaxPairs || r.hx * (kSlabHalo + 2) <= kFusedMaxPairs);
+        nmax = std::max(nmax, r.n);
     }
     int lds_max = 0, optin = 0;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, c->device);
     lds_max = std::max(lds_max, optin);
-    // 2 colours x 2 parities of (plane + one zero slot), 16 bytes a site
-    const size_t lds = (size_t)4 * (pmax + 1) * sizeof(float4);
-    int threads = (int)((((pmax + kFusedPasses - 1) / kFusedPasses) + kWave - 1) / kWave * kWave);
+    // 2 colours x 2 parities of (window + one zero slot), 16 bytes a site
+    const size_t lds = (size_t)4 * (kFusedMaxPairs + 1) * sizeof(float4);
     // the kernel addresses a replica's spins through 32-bit buffer offsets
-    int64_t nmax = 0;
-    for (const Rep& r : c->reps) nmax = std::max(nmax, r.n);
-    c->fused_ok = all_sc && pmax > 0 && threads <= kFusedMaxThreads && lds <= (size_t)lds_max &&
-                  12 * nmax < ((int64_t)1 << 31);
-    c->fused_threads = std::max(threads, kWave);
-    c->fused_wpb = c->fused_threads / kWave;
-    c->fused_pstride = (uint32_t)pmax + 1u;
-    c->fused_lds = lds;
-    c->max_fblocks = zsum;
+    c->fused_ok = all_sc && fits && !c->reps.empty() && lds <= (size_t)lds_max && 12 * nmax < ((int64_t)1 << 31);
     if (c->fused_ok) {
         const void* fns[] = { (const void*)k_sweep_fused<false>, (const void*)k_sweep_fused<true>,
-                              (const void*)k_sweep_fused<true, true> };
+                              (const void*)k_sweep_fused<true, true>, (const void*)k_sweep_fused<false, false, true>,
+                              (const void*)k_sweep_fused<true, false, true>, (const void*)k_sweep_fused<true, true, true> };
         for (const void* f : fns)
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
                 c->fused_ok = false;
         (void)hipGetLastError();
     }
-    int rc;
-    if ((rc = dalloc(c, &c->d_fblock_rep, (size_t)zsum))) return rc;
-    if ((rc = dalloc(c, &c->d_acc_wave_f, (size_t)zsum * c->fused_wpb))) return rc;
-    if ((rc = dalloc(c, &c->d_wave_part_f, 4 * (size_t)zsum * c->fused_wpb))) return rc;
-    if ((rc = dalloc(c, &c->d_acc_base, (size_t)c->R))) return rc;
-    return ISING_OK;
+    return dalloc(c, &c->d_acc_base, (size_t)c->R);
 }
 
-// Chunks of lc z planes per fused workgroup (lc = 0: none). Acceptance
-// counters of the previous layout are folded into the per-slot base first.
-int fused_layout(ising_ctx* c, int lc)
+// Window of a replica under a requested slab height hy: whole planes (rows
+// = hy = ly, one slab) or slabs of hy owned rows in windows of hy + 4 rows.
+// hy = 0: whole planes where they fit; otherwise the fewest slabs whose
+// windows fit, of equal height (the last may own fewer rows), so that no
+// slab's window is mostly halo.
+struct SlabShape { int rows, hy, nslab; };
+
+SlabShape slab_shape(const Rep& r, int hy_req)
 {
+    const bool whole = r.plane <= kFusedMaxPairs && (hy_req == 0 || hy_req >= r.ly);
+    if (whole) return { r.ly, r.ly, 1 };
+    const int hmax = kFusedMaxPairs / r.hx - kSlabHalo;     // rows the tallest window owns
+    int hy = std::min(hy_req > 0 ? hy_req : hmax, hmax);
+    if (hy_req == 0 && hy >= 2) hy = (r.ly + (r.ly + hy - 1) / hy - 1) / ((r.ly + hy - 1) / hy);
+    if (hy < 2 || hy >= r.ly) return { 0, 0, 0 };
+    return { hy + kSlabHalo, hy, (r.ly + hy - 1) / hy };
+}
+
+// Chunks of lc z planes (and y slabs of hy rows) per fused workgroup (lc =
+// 0: none). Acceptance counters of the previous layout are folded into the
+// per-slot base first.
+int fused_layout(ising_ctx* c, int lc, int hy_req)
+{
+    std::vector<SlabShape> shape(c->reps.size());
+    int pw_max = 0, fb = 0;
+    bool any_slab = false;
+    for (size_t k = 0; k < c->reps.size(); ++k) {
+        const Rep& r = c->reps[k];
+        shape[k] = lc > 0 ? slab_shape(r, hy_req) : SlabShape{ r.ly, r.ly, 1 };
+        if (shape[k].nslab == 0)
+            return set_err(c, ISING_ERR_UNSUPPORTED, "fused sweep: no y slab of %d rows fits a workgroup on a "
+                           "%d x %d plane", hy_req, r.lx, r.ly);
+        pw_max = std::max(pw_max, shape[k].rows * r.hx);
+        any_slab = any_slab || shape[k].nslab > 1;
+        if (lc > 0) fb += (r.lz + lc - 1) / lc * shape[k].nslab;
+    }
+    const int threads = std::max(kWave, ((pw_max + kFusedPasses - 1) / kFusedPasses + kWave - 1) / kWave * kWave);
+    const int wpb = threads / kWave;
+    // Larger arrays for the block table and the per-wavefront counters and
+    // sums (never shrunk) are allocated before anything changes, so a
+    // failure leaves the previous layout in place.
+    int rc;
+    int32_t* nrep = nullptr;
+    unsigned long long* nacc = nullptr;
+    double* npart = nullptr;
+    const bool grow_b = (size_t)fb > c->fblock_cap, grow_w = (size_t)fb * wpb > c->fwave_cap;
+    if ((grow_b && (rc = dalloc(c, &nrep, (size_t)fb))) ||
+        (grow_w && ((rc = dalloc(c, &nacc, (size_t)fb * wpb)) || (rc = dalloc(c, &npart, 4 * (size_t)fb * wpb))))) {
+        if (nrep) (void)hipFree(nrep);
+        if (nacc) (void)hipFree(nacc);
+        return rc;
+    }
+    // the old layout's acceptance counts move to the per-slot base
     if (c->n_fblocks > 0) {
         hipLaunchKernelGGL(k_acc_fold, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
                            (const unsigned long long*)c->d_acc_wave_f, c->fused_wpb, c->d_acc_base);
         LAUNCH_CHECK(c);
-        HIP_TRY(c, hipMemsetAsync(c->d_acc_wave_f, 0, (size_t)c->max_fblocks * c->fused_wpb * sizeof(unsigned long long),
-                                  c->stream));
     }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (grow_b) {
+        if (c->d_fblock_rep) (void)hipFree(c->d_fblock_rep);
+        c->d_fblock_rep = nrep;
+        c->fblock_cap = (size_t)fb;
+    }
+    if (grow_w) {
+        if (c->d_acc_wave_f) (void)hipFree(c->d_acc_wave_f);
+        if (c->d_wave_part_f) (void)hipFree(c->d_wave_part_f);
+        c->d_acc_wave_f = nacc;
+        c->d_wave_part_f = npart;
+        c->fwave_cap = (size_t)fb * wpb;
+    }
+    if (c->fwave_cap)
+        HIP_TRY(c, hipMemsetAsync(c->d_acc_wave_f, 0, c->fwave_cap * sizeof(unsigned long long), c->stream));
     c->fblock_rep.clear();
-    int fb = 0;
+    fb = 0;
     for (size_t k = 0; k < c->reps.size(); ++k) {
         Rep& r = c->reps[k];
+        r.f_rows = shape[k].rows;
+        r.f_hy = shape[k].hy;
+        r.f_nslab = shape[k].nslab;
         r.f_first_block = fb;
-        r.f_nblocks = lc > 0 ? (r.lz + lc - 1) / lc : 0;
+        r.f_nblocks = lc > 0 ? (r.lz + lc - 1) / lc * r.f_nslab : 0;
         fb += r.f_nblocks;
         for (int b = 0; b < r.f_nblocks; ++b) c->fblock_rep.push_back((int32_t)k);
     }
     c->n_fblocks = fb;
+    c->fused_threads = threads;
+    c->fused_wpb = wpb;
+    c->fused_pstride = (uint32_t)pw_max + 1u;
+    c->fused_lds = (size_t)4 * (pw_max + 1) * sizeof(float4);
+    c->fused_slab = any_slab;
     HIP_TRY(c, hipMemcpyAsync(c->d_reps, c->reps.data(), c->reps.size() * sizeof(Rep), hipMemcpyHostToDevice, c->stream));
     if (fb > 0)
         HIP_TRY(c, hipMemcpyAsync(c->d_fblock_rep, c->fblock_rep.data(), (size_t)fb * sizeof(int32_t),
@@ -4069,7 +4196,7 @@ int ising_create(ising_ctx** out, const ising_ladder_desc* ladders, int n_ladder
 #undef UP
     if (hipMemsetAsync(c->d_spins, 0, 2 * (size_t)c->total_floats * sizeof(float), s) != hipSuccess ||
         hipMemsetAsync(c->d_acc_base, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
-        hipMemsetAsync(c->d_acc_wave_f, 0, (size_t)c->max_fblocks * c->fused_wpb * sizeof(unsigned long long), s) != hipSuccess ||
+        (c->fwave_cap && hipMemsetAsync(c->d_acc_wave_f, 0, c->fwave_cap * sizeof(unsigned long long), s) != hipSuccess) ||
         hipMemsetAsync(c->d_ctr, 0, kCounters * sizeof(uint32_t), s) != hipSuccess ||
         hipMemsetAsync(c->d_acc, 0, R * sizeof(unsigned long long), s) != hipSuccess ||
         hipMemsetAsync(c->d_acc_wave, 0, (size_t)blk * (kBlock / kWave) * sizeof(unsigned long long), s) != hipSuccess ||
@@ -4767,19 +4894,21 @@ int ising_bandwidth_probe(ising_ctx* c, int64_t bytes, double* ms)
 int ising_set_tuning(ising_ctx* c, uint32_t flags)
 {
     if (!c) return ISING_ERR_INVALID_ARG;
-    if (flags & ~(kTuneXcdRemap | kTuneNtOwn | kTuneFusedMask))
+    if (flags & ~(kTuneXcdRemap | kTuneNtOwn | kTuneFusedMask | kTuneSlabMask))
         return set_err(c, ISING_ERR_INVALID_ARG, "unknown tuning flags %u", flags);
     const int lc = (int)((flags & kTuneFusedMask) >> kTuneFusedShift);
+    const int hy = (int)((flags & kTuneSlabMask) >> kTuneSlabShift);
+    if (hy > 0 && lc == 0)
+        return set_err(c, ISING_ERR_INVALID_ARG, "y slabs (tuning flags %u) are a shape of the fused sweep", flags);
     if (lc > 0 && c->any_demag)
         return set_err(c, ISING_ERR_UNSUPPORTED, "fused sweep: a ladder has the demagnetisation term");
     if (lc > 0 && !c->fused_ok)
         return set_err(c, ISING_ERR_UNSUPPORTED,
-                       "fused sweep unavailable: it needs simple-cubic replicas whose z plane of one colour "
-                       "(%u sites here) fits one workgroup (%d sites) and LDS", c->fused_pstride - 1u,
-                       kFusedMaxThreads * kFusedPasses);
+                       "fused sweep unavailable: it needs simple-cubic replicas whose z planes of one colour "
+                       "fit one workgroup (%d sites) or y slabs of at least two rows of it", kFusedMaxPairs);
     HIP_TRY(c, hipSetDevice(c->device));
-    if (lc != fused_lc(c)) {
-        int rc = fused_layout(c, lc);
+    if (lc != fused_lc(c) || hy != fused_hy(c)) {
+        int rc = fused_layout(c, lc, hy);
         if (rc) return rc;
     }
     c->tune = flags;
@@ -4793,22 +4922,67 @@ int ising_get_tuning(const ising_ctx* c, uint32_t* flags)
     return ISING_OK;
 }
 
+// Estimated time of a fused layout (lc planes, y slabs of hy rows): a sweep
+// runs in rounds of one workgroup per compute unit, and a round lasts as long
+// as its largest workgroup, whose work is its (lc + 2) planes times its window
+// rows. 0 when some replica has no such layout.
+static double fused_layout_cost(const ising_ctx* c, int lc, int hy, int n_cu)
+{
+    int64_t nwg = 0;
+    int64_t work = 0;
+    for (const Rep& r : c->reps) {
+        const SlabShape sh = slab_shape(r, hy);
+        if (sh.nslab == 0) return 0.0;
+        nwg += (int64_t)((r.lz + lc - 1) / lc) * sh.nslab;
+        work = std::max<int64_t>(work, (int64_t)(std::min(lc, r.lz) + 2) * sh.rows * r.hx);
+    }
+    return (double)((nwg + n_cu - 1) / n_cu) * (double)work;
+}
+
 // Flag sets the autotuner times: the half-sweep pair with and without
-// die-contiguous blocks and non-temporal own-colour access, then (when the
-// planes fit one workgroup) the fused sweep with 8, 16, 32 or all z planes per
-// workgroup, with and without die-contiguous blocks.
+// die-contiguous blocks and non-temporal own-colour access, then the fused
+// sweep with 8, 16, 32 or all z planes per workgroup, with and without
+// die-contiguous blocks. When every replica's planes are cut into y slabs,
+// each chunk size takes the slab height of the lowest fused_layout_cost
+// (balanced heights, from the tallest window down to a quarter of it), and
+// a chunk of 24 planes joins the list: the layouts whose workgroup count
+// fills the last round win (on 128^3 x 8 replicas, 32 planes x 16 rows).
 static std::vector<uint32_t> tuning_candidates(const ising_ctx* c)
 {
     std::vector<uint32_t> v = { 0u, kTuneXcdRemap, kTuneNtOwn, kTuneXcdRemap | kTuneNtOwn };
     if (c->fused_ok && !c->any_demag) {
+        bool all_slab = true;
+        for (const Rep& r : c->reps) all_slab = all_slab && r.plane > kFusedMaxPairs;
         std::vector<int> lcs;
-        for (int lc : { 8, 16, 32, c->zmax }) {
+        const std::initializer_list<int> base = { 8, 16, 32, c->zmax }, slabbed = { 8, 16, 24, 32, c->zmax };
+        for (int lc : all_slab ? slabbed : base) {
             lc = std::min(std::min(lc, c->zmax), 255);
             if (lc > 0 && std::find(lcs.begin(), lcs.end(), lc) == lcs.end()) lcs.push_back(lc);
         }
+        int n_cu = 0;
+        if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || n_cu < 1)
+            n_cu = 256;
         for (int lc : lcs) {
-            v.push_back((uint32_t)lc << kTuneFusedShift);
-            v.push_back(((uint32_t)lc << kTuneFusedShift) | kTuneXcdRemap);
+            uint32_t f = (uint32_t)lc << kTuneFusedShift;
+            if (all_slab) {
+                // balanced heights: ceil(ly / n) for n slabs, n from the fewest up to four times as many
+                const Rep& r0 = *std::max_element(c->reps.begin(), c->reps.end(),
+                                                  [](const Rep& a, const Rep& b) { return a.plane < b.plane; });
+                const int hmax = kFusedMaxPairs / r0.hx - kSlabHalo;
+                const int n0 = (r0.ly + hmax - 1) / hmax;
+                double best = 0.0;
+                int best_hy = 0;
+                for (int n = n0; n <= 4 * n0 && hmax >= 2; ++n) {
+                    const int hy = (r0.ly + n - 1) / n;
+                    if (hy < 2) break;
+                    const double cost = fused_layout_cost(c, lc, hy, n_cu);
+                    if (cost > 0.0 && (best_hy == 0 || cost < best)) { best = cost; best_hy = hy; }
+                }
+                if (best_hy == 0) continue;
+                f |= (uint32_t)std::min(best_hy, 255) << kTuneSlabShift;
+            }
+            v.push_back(f);
+            v.push_back(f | kTuneXcdRemap);
         }
     }
     if (v.size() > ISING_TUNE_MAX_CANDIDATES) v.resize(ISING_TUNE_MAX_CANDIDATES);

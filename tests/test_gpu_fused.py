@@ -1,6 +1,7 @@
-"""GPU tests of the fused sweep: one launch per sweep, whole z planes of each
-colour staged in LDS, z-chunked workgroups that recompute their halo planes,
-results written to the other spin buffer.
+"""GPU tests of the fused sweep: one launch per sweep, z planes of each colour
+staged in LDS (whole planes, or y slabs that recompute two halo rows on
+either side), z-chunked workgroups that recompute their halo planes, results
+written to the other spin buffer.
 
 The fused launch must give the same chains bit for bit as the two half-sweep
 launches (same Philox counters, same per-site arithmetic), for every chunk
@@ -14,7 +15,7 @@ import pytest
 
 from gpu_util import engine
 from ising_amd import Engine, Hamiltonian, IsingError, Ladder, Lattice, geometric_temperatures
-from ising_amd._capi import TUNE_XCD_REMAP, tune_fused
+from ising_amd._capi import TUNE_XCD_REMAP, tune_fused, tune_slab
 from ising_amd.learned import initial_params
 
 pytestmark = pytest.mark.gpu
@@ -157,30 +158,111 @@ def test_fused_measurement_matches_exact(lname):
 
 def test_accept_counts_survive_layout_changes():
     lat = Lattice(16, 16, 16)
-    ref = run_sweeps(lat, HAM, 0, n=6)
+    ref = run_sweeps(lat, HAM, 0, n=8)
     with engine(lat, HAM, BETAS, seed=77) as e:
         e.init_random()
-        for flags in (tune_fused(4), tune_fused(8) | TUNE_XCD_REMAP, 0, tune_fused(16), tune_fused(3), 0):
+        for flags in (tune_fused(4), tune_fused(8) | TUNE_XCD_REMAP, 0, tune_fused(16), tune_fused(3) | tune_slab(5),
+                      tune_fused(3), tune_fused(2) | tune_slab(2), 0):
             e.set_tuning(flags)
             e.sweep(1)
         assert np.array_equal(e.accept_counts(), ref[1])
         assert np.array_equal(e.get_spins(1), ref[0][1])
 
 
+# y slabs: (lattice, Hamiltonian, owned rows per slab (0: the tallest window
+# that fits), z planes per workgroup). Windows cross periodic images and open
+# faces; the last slab of a lattice owns fewer rows.
+SLABS = {
+    "cube_16_slab2": (Lattice(16, 16, 16), HAM, 2, 4),
+    "cube_16_slab5": (Lattice(16, 16, 16), HAM, 5, 255),
+    "open_y_12x21x6_slab4": (Lattice(12, 21, 6, (True, False, False)), HAM, 4, 3),
+    "open_xy_10x15x8_slab3": (Lattice(10, 15, 8, (False, False, True)), HAM_IF, 3, 5),
+    "film_128x128x4_auto": (Lattice(128, 128, 4, (True, True, False)), HAM, 0, 4),
+    "open_y_128x37x5_auto": (Lattice(128, 37, 5, (True, False, False)), HAM, 0, 2),
+    "wide_600x10x4_auto": (Lattice(600, 10, 4), HAM, 0, 4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SLABS))
+def test_fused_slabs_equal_half_sweeps(name):
+    lat, ham, hy, lc = SLABS[name]
+    ref = run_sweeps(lat, ham, 0, n=4)
+    got = run_sweeps(lat, ham, tune_fused(lc) | tune_slab(hy) | TUNE_XCD_REMAP, n=4)
+    for r in range(len(BETAS)):
+        assert np.array_equal(ref[0][r], got[0][r]), r
+    assert np.array_equal(ref[1], got[1])
+    assert np.array_equal(ref[2], got[2])
+
+
+def test_fused_slabs_beside_whole_planes():
+    """One context holding replicas whose planes fit a workgroup and a film
+    whose planes are cut into slabs: one launch, one workgroup shape."""
+    lads = [Ladder(Lattice(16, 16, 16), HAM, [1.0, 0.5], 0.5),
+            Ladder(Lattice(128, 128, 4, (True, True, False)), HAM, [2.0, 0.7], 0.4),
+            Ladder(Lattice(12, 21, 6, (True, False, False)), HAM_IF, [1.5], 0.5)]
+    out = []
+    for flags in (0, tune_fused(4)):
+        with Engine(lads, seed=12) as e:
+            e.set_tuning(flags)
+            e.init_random()
+            e.sweep(3)
+            out.append(([e.get_spins(r) for r in range(e.n_replicas)], e.accept_counts(), e.energy()))
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_fused_slabs_graph_and_records():
+    """Replayed graphs of a 3-sweep period on y slabs against eager half
+    sweeps (spins, swaps and Q series bit for bit), and the slab launch's E/M
+    records against the FP64 measure kernel."""
+    lat = Lattice(16, 20, 8, (True, False, True))
+    betas = 1.0 / geometric_temperatures(0.3, 2.0, 4)
+    outs = []
+    for flags, graph in ((0, False), (tune_fused(4) | tune_slab(3), True)):
+        with engine(lat, HAM, betas, seed=6) as e:
+            e.set_tuning(flags)
+            e.init_random()
+            o = e.pt_run(n_sweeps=33, swap_every=3, measure_every=1, winding_every=3, use_graph=graph)
+            o["map"] = e.temperature_map()
+            o["spins"] = e.get_spins(1)
+            sot = e.temperature_map()
+            o["exact"] = e.energy()[sot]
+            o["mag"] = e.magnetisation()[sot]
+            outs.append(o)
+    for k in ("winding", "winding_raw", "map", "spins"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    for k in ("energy", "magnetisation"):
+        assert np.allclose(outs[0][k], outs[1][k], rtol=1e-12, atol=1e-9), k
+    got = outs[1]
+    scale = np.maximum(np.abs(got["exact"]), lat.sites)
+    assert np.all(np.abs(got["energy"][-1] - got["exact"]) <= 1e-6 * scale)
+    assert np.allclose(got["magnetisation"][-1], got["mag"], atol=1e-6)
+
+
 def test_tuning_flags_are_checked():
+    # planes beyond one workgroup run as y slabs
     with engine(Lattice(128, 128, 4, (True, True, False)), HAM, [1.0]) as e:
+        e.set_tuning(tune_fused(2))
+        assert any((f >> 8) & 0xFF for f in e.tuning_candidates())
+        with pytest.raises(IsingError):
+            e.set_tuning(tune_fused(2) | tune_slab(1))      # a slab owns at least two rows
+        with pytest.raises(IsingError):
+            e.set_tuning(tune_slab(4))                      # slabs only with a fused chunk
+    # the limit: a window of six rows of one colour (341 column pairs a row)
+    with engine(Lattice(684, 8, 4), HAM, [1.0]) as e:
         with pytest.raises(IsingError) as ei:
-            e.set_tuning(tune_fused(2))        # 8192 sites of one colour per plane: more than a workgroup holds
+            e.set_tuning(tune_fused(2))
         assert "UNSUPPORTED" in str(ei.value)
         assert all((f >> 8) == 0 for f in e.tuning_candidates())
-    # the limit: 2048 sites of one colour per plane (1024 threads x 2 column pairs)
-    with engine(Lattice(64, 66, 3, (True, True, False)), HAM, [1.0]) as e:
-        with pytest.raises(IsingError):
-            e.set_tuning(tune_fused(2))
+    with engine(Lattice(682, 8, 4), HAM, [1.0]) as e:
+        e.set_tuning(tune_fused(2))
     with engine(Lattice(64, 64, 3, (True, True, False)), HAM, [1.0]) as e:
         e.set_tuning(tune_fused(2))
+        e.set_tuning(tune_fused(2) | tune_slab(62))
     with engine(Lattice(8, 8, 8), HAM, [1.0]) as e:
         with pytest.raises(IsingError):
-            e.set_tuning(1 << 20)
+            e.set_tuning(1 << 24)
         cand = e.tuning_candidates()
         assert tune_fused(8) in cand and 0 in cand
