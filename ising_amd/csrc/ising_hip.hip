@@ -3688,12 +3688,19 @@ int fused_layout(ising_ctx* c, int lc, int hy_req)
         return rc;
     }
     // the old layout's acceptance counts move to the per-slot base
+    hipError_t e = hipSuccess;
     if (c->n_fblocks > 0) {
         hipLaunchKernelGGL(k_acc_fold, dim3(c->R), dim3(kBlock), 0, c->stream, c->d_reps,
                            (const unsigned long long*)c->d_acc_wave_f, c->fused_wpb, c->d_acc_base);
-        LAUNCH_CHECK(c);
+        e = hipGetLastError();
     }
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {            // the previous layout stays
+        if (nrep) (void)hipFree(nrep);
+        if (nacc) (void)hipFree(nacc);
+        if (npart) (void)hipFree(npart);
+        return set_err(c, ISING_ERR_HIP, "fused layout: %s", hipGetErrorString(e));
+    }
     if (grow_b) {
         if (c->d_fblock_rep) (void)hipFree(c->d_fblock_rep);
         c->d_fblock_rep = nrep;
@@ -3706,8 +3713,6 @@ int fused_layout(ising_ctx* c, int lc, int hy_req)
         c->d_wave_part_f = npart;
         c->fwave_cap = (size_t)fb * wpb;
     }
-    if (c->fwave_cap)
-        HIP_TRY(c, hipMemsetAsync(c->d_acc_wave_f, 0, c->fwave_cap * sizeof(unsigned long long), c->stream));
     c->fblock_rep.clear();
     fb = 0;
     for (size_t k = 0; k < c->reps.size(); ++k) {
@@ -3726,11 +3731,21 @@ int fused_layout(ising_ctx* c, int lc, int hy_req)
     c->fused_pstride = (uint32_t)pw_max + 1u;
     c->fused_lds = (size_t)4 * (pw_max + 1) * sizeof(float4);
     c->fused_slab = any_slab;
-    HIP_TRY(c, hipMemcpyAsync(c->d_reps, c->reps.data(), c->reps.size() * sizeof(Rep), hipMemcpyHostToDevice, c->stream));
-    if (fb > 0)
-        HIP_TRY(c, hipMemcpyAsync(c->d_fblock_rep, c->fblock_rep.data(), (size_t)fb * sizeof(int32_t),
-                                  hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->fwave_cap) e = hipMemsetAsync(c->d_acc_wave_f, 0, c->fwave_cap * sizeof(unsigned long long), c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_reps, c->reps.data(), c->reps.size() * sizeof(Rep), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && fb > 0)
+        e = hipMemcpyAsync(c->d_fblock_rep, c->fblock_rep.data(), (size_t)fb * sizeof(int32_t), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        // neither layout is on the device for sure: leave the fused sweep off
+        // (half sweeps need none of it) until a later ising_set_tuning
+        for (Rep& r : c->reps) r.f_nblocks = 0;
+        c->n_fblocks = 0;
+        c->tune &= ~(kTuneFusedMask | kTuneSlabMask);
+        return set_err(c, ISING_ERR_HIP, "fused layout: %s", hipGetErrorString(e));
+    }
     return ISING_OK;
 }
 
