@@ -22,10 +22,12 @@ module ising_gpu_c
         integer(c_int32_t), parameter :: ISING_DMI_BULK = 1
         integer(c_int32_t), parameter :: ISING_DMI_INTERFACIAL_REF = 2
 
-        ! launch tuning flags (ISING_TUNE_FUSED(lc) = lc * 2**ISING_TUNE_FUSED_SHIFT)
+        ! launch tuning flags (ISING_TUNE_FUSED(lc) = lc * 2**ISING_TUNE_FUSED_SHIFT,
+        ! ISING_TUNE_SLAB(hy) = hy * 2**ISING_TUNE_SLAB_SHIFT)
         integer(c_int32_t), parameter :: ISING_TUNE_XCD_REMAP = 1
         integer(c_int32_t), parameter :: ISING_TUNE_NT_OWN = 2
         integer(c_int32_t), parameter :: ISING_TUNE_FUSED_SHIFT = 8
+        integer(c_int32_t), parameter :: ISING_TUNE_SLAB_SHIFT = 16
         integer(c_int), parameter :: ISING_TUNE_MAX_CANDIDATES = 16
         integer(c_int), parameter :: ISING_MLP_PARAMS = 1793
         integer(c_int), parameter :: ISING_MLP_RECORD = 32
