@@ -343,7 +343,12 @@ int  ising_bandwidth_probe(ising_ctx* ctx, int64_t bytes, double* ms);
 int  ising_set_tuning(ising_ctx* ctx, uint32_t flags);
 int  ising_get_tuning(const ising_ctx* ctx, uint32_t* flags);
 /* The flag sets ising_autotune times, in order (at most
- * ISING_TUNE_MAX_CANDIDATES); flags may be NULL to query the count. */
+ * ISING_TUNE_MAX_CANDIDATES); flags may be NULL to query the count: the half
+ * sweeps with and without ISING_TUNE_XCD_REMAP and ISING_TUNE_NT_OWN, then
+ * (simple-cubic contexts without the demagnetising term) fused chunks of 8,
+ * 16, 32 or all z planes with and without ISING_TUNE_XCD_REMAP; when every
+ * replica's planes exceed one workgroup, also 24 planes, each chunk size with
+ * the slab height of the fewest estimated rounds of workgroups. */
 int  ising_tuning_candidates(const ising_ctx* ctx, uint32_t* flags, int* count);
 /* Times every candidate over n_sweeps real sweeps each (they advance the
  * chains like ising_sweep) and keeps the fastest; ms (may be NULL, room for
