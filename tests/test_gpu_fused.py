@@ -241,6 +241,32 @@ def test_fused_slabs_graph_and_records():
     assert np.allclose(got["magnetisation"][-1], got["mag"], atol=1e-6)
 
 
+def test_autotune_ranks_slab_shapes():
+    """A context whose planes all exceed one workgroup: the candidates carry
+    cost-ranked slab heights for each chunk size (and 24-plane chunks), every
+    candidate is timed, and the chain after autotuning equals the half
+    sweeps' chain over the same sweeps (tuning changes no result)."""
+    lat = Lattice(80, 80, 16)                    # 40 x 80 = 3200 sites of one colour per plane
+    cand = None
+    with engine(lat, HAM, BETAS, seed=21) as e:
+        cand = e.tuning_candidates()
+        fused = [f for f in cand if (f >> 8) & 0xFF]
+        assert fused and all((f >> 16) & 0xFF >= 2 for f in fused)
+        assert any(((f >> 8) & 0xFF) == 16 for f in fused)
+        e.init_random()
+        best, ms = e.autotune(2)
+        assert set(ms) == set(cand) and best in cand
+        e.sweep(3)
+        got = [e.get_spins(r) for r in range(len(BETAS))], e.accept_counts()
+    with engine(lat, HAM, BETAS, seed=21) as e:
+        e.init_random()
+        e.sweep(2 * len(cand) + 3)
+        ref = [e.get_spins(r) for r in range(len(BETAS))], e.accept_counts()
+    for a, b in zip(ref[0], got[0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(ref[1], got[1])
+
+
 def test_tuning_flags_are_checked():
     # planes beyond one workgroup run as y slabs
     with engine(Lattice(128, 128, 4, (True, True, False)), HAM, [1.0]) as e:
