@@ -2789,7 +2789,7 @@ struct ising_ctx {
     uint32_t host_sweep = 0;
     uint32_t host_fused = 0;     // fused sweeps enqueued: parity selects the current spin buffer
     uint32_t tune = 0;           // sweep tuning flags (kTune*, fused chunk in bits 8..15)
-    // fused sweep (k_sweep_fused): available when every plane fits one workgroup and LDS
+    // fused sweep (k_sweep_fused): simple-cubic contexts whose planes fit one workgroup, or y slabs of them
     bool fused_ok = false;
     bool fused_slab = false;     // some replica's planes are cut into y slabs (the kSlab kernels)
     int fused_threads = 0, fused_wpb = 0, n_fblocks = 0;
