@@ -359,6 +359,12 @@ def main() -> int:
                      "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-10:].mean()),
                      "prep_seconds": prep_s},
         "observables": {"fixed_E_per_site": fixed["mean_E_per_site"][::8], "learned_E_per_site": learned["mean_E_per_site"][::8]},
+        # tau_int(E) in sweeps at every 8th temperature position (negative: the window did not close)
+        "tau_int_E_by_temperature": {
+            "T": [round(float(x), 4) for x in T[::8]],
+            "fixed": [round(float(t if c else -t), 2) for t, c in zip(fixed["tau_int_E_sweeps"][::8], fixed["conv_E"][::8])],
+            "learned": [round(float(t if c else -t), 2)
+                        for t, c in zip(learned["tau_int_E_sweeps"][::8], learned["conv_E"][::8])]},
         "swap_acceptance": {"min": float((acc[:-1] / np.maximum(att[:-1], 1)).min()),
                             "mean": float((acc[:-1] / np.maximum(att[:-1], 1)).mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
