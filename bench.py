@@ -371,7 +371,7 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "valu_busy": counters.get("fused", {}).get("valu_busy") if fused else counters.get("valu_busy"),
                      "learned_valu_busy": counters.get("learned_valu_busy"),
-                     "kernel": "k_sweep_fused<false, false>" if fused else "k_half_sweep<false, false, false, false>",
+                     "kernel": "k_sweep_fused<false, false, false>" if fused else "k_half_sweep<false, false, false, false>",
                      "ms_per_launch": ms_launch, "site_updates_per_launch": n_sites // 2 * halves,
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "copy_probe_GBs": 2 * (256 << 20) / (copy_ms * 1e-3) / 1e9,
